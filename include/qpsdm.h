@@ -1,0 +1,212 @@
+/*
+ * qpsdm.h -- C ABI of the MI355X Q-compensated Kirchhoff PSDM imaging path.
+ *
+ * Plain C types only: device buffers are passed as raw device pointers
+ * (void* / float*), HIP streams as void* (NULL = default stream).  Every
+ * entry point returns 0 on success or a negative QPSDM_E* code; the text of
+ * the last error is available from qpsdm_last_error().  Nothing here
+ * allocates device memory except the functions whose name says so
+ * (qpsdm_ctx_create, qpsdm_load_tables_files).
+ *
+ * The reference is one C++/CUDA program whose imaging operator is
+ * psdm_kernel() (src/q-psdm.cu:252-283, 2063-2244) driving image_depth_gpu
+ * (src/q-psdm.cu:2246-2380); each entry point below names the part of it
+ * that it replaces.
+ */
+#ifndef QPSDM_H_
+#define QPSDM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPSDM_OK 0
+#define QPSDM_EINVAL -1   /* bad argument / shape                     */
+#define QPSDM_EHIP -2     /* HIP runtime error                        */
+#define QPSDM_EIO -3      /* file missing, short read, size mismatch  */
+#define QPSDM_ERANGE -4   /* size beyond a documented maximum         */
+
+/* Bank / table storage types. */
+#define QPSDM_F32 0
+#define QPSDM_BF16 1 /* bank only                         */
+#define QPSDM_F16B 2 /* tables: FP16 offset+scale per 8^3 brick */
+
+/* Two-window spectral set-up of one offset class; replaces the window block
+ * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
+ * (src/q-psdm.cu:1640-1644).  Same float/int semantics as the reference. */
+typedef struct {
+  int nsb, itb1, itb2, nf1, nf4, nww, ncoef1, nf3;
+  float tdstart1, tdstart2, tdmid, df, dw;
+} qpsdm_window;
+
+int qpsdm_window_params(int ntl, float dt, float f1, float f3, float f4, qpsdm_window *out);
+
+/* Reference FFT length choice: npfar (src/fft.cc:412-439) via
+ * get_opt_n_fftw (src/fft.cc:3721). */
+int qpsdm_npfar(int nmin);
+
+/* Gain-limiter coefficients lnG, c1..c4 (src/q-psdm.cu:1277-1283). */
+int qpsdm_gain_coefs(float threshold, float out5[5]);
+
+/* Azimuth-sector table (src/q-psdm.cu:1167-1193); n = (int)(2/0.001 + 1.5). */
+int qpsdm_iaztab(int *tab, int n);
+
+/* Per-trace geometry of psdm_kernel (src/q-psdm.cu:2107-2144): scaled
+ * coordinates in FP64, extent reject, offset, midpoint CDP bins, azimuth
+ * sector; then the image-origin shift done in FP64 before the FP32 cast
+ * (src/q-psdm.cu:2221-2224).  hdr_xy = {sx, sy, gx, gy} as read from the
+ * SU header (int), scalco applied through `scaled`.  Outputs per trace:
+ * geom[4] = image-relative (sx, sy, gx, gy) float, info[4] = {keep, cdpx,
+ * cdpy, iaz}. */
+typedef struct {
+  double scaled;               /* coordinate scale (getscale(scalco))          */
+  double fxv1, exv1, fyv1, eyv1; /* model extent with margins (src/q-psdm.cu:1379-1383) */
+  double fxi, cyi;             /* image origin x and line y (src/q-psdm.cu:1430-1431) */
+  double ox, oy;               /* origin subtracted before the FP32 cast         */
+  float dxi, dyi;              /* CDP and line spacing                          */
+} qpsdm_geom_params;
+
+int qpsdm_trace_geometry(const qpsdm_geom_params *p, const int32_t *hdr_xy, int ntr,
+                         const int *iaztab, float *geom, int32_t *info);
+
+/* ---------------------------------------------------------------------- */
+/* Tables.                                                                 */
+/* ---------------------------------------------------------------------- */
+/* Geometry of a relative-coordinate table set: one 3-D (t, t*) table per
+ * surface node; a surface position p uses its nearest node's table at the
+ * relative position (x - px, y - py, z).  FP32 layout:
+ * float2 [node][nv][nu][nw] (z fastest).  FP16-brick layout: per node a
+ * grid of 8x8x8 bricks [bv][bu][bw], each a float4 header {t_off, t_scale,
+ * s_off, s_scale} followed by 512 half2 values [v][u][w]. */
+typedef struct {
+  int nnx, nny;
+  float n0x, n0y, ndx, ndy;
+  int nu, nv, nw;
+  float u0, v0, du, dv, dw;
+} qpsdm_tgeom;
+
+/* Velocity/Q model for the table builder. kind 0: v(z)=v0+k z, constant Q
+ * (closed form); kind 1: layers of thickness layer_dz with a Gaussian
+ * anomaly (8-point Gauss-Legendre per layer piece). */
+typedef struct {
+  int kind;
+  double v0, kgrad, q0;
+  int nlayer;
+  double layer_dz;
+  double vl[16], ql[16];
+  double ax, ay, az, asig, av, aq;
+} qpsdm_model;
+
+/* Table builder (no counterpart in the reference, which ships tables but no
+ * generator): straight-ray t and t* into the FP32 layout, on the device. */
+int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab_dev, void *stream);
+
+/* Size in bytes of the FP16-brick form of a table set. */
+size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g);
+
+/* Quantise an FP32 table set into the FP16-brick form (offset+scale per
+ * 8^3 brick). */
+int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, void *stream);
+
+/* Line interpolation of two table files, t = t1 (1-c) + t2 c, as
+ * get_ttt_all does on the host (src/q-psdm.cu:2582-2594); on the device. */
+int qpsdm_tables_blend(const float *t1_dev, const float *t2_dev, size_t n, float c,
+                       float *out_dev, void *stream);
+
+/* Reference table-file layout (src/q-psdm.cu:103-109, 2382-2622):
+ * Lxxx.tidx|qidx header + descriptors, L<line>.ttt|qqq payloads. */
+typedef struct {
+  int32_t cdp1, cdp2;
+  float dx, dz;
+  int32_t nzg, nxg, nzi0, nxi0;
+} qpsdm_tidx_header;
+
+typedef struct {
+  int64_t idx;
+  uint16_t nys, nxs, dys, dxs;
+} qpsdm_ttt_desc;
+
+/* Read an index file; desc may be NULL to query the header only. */
+int qpsdm_read_tidx(const char *path, qpsdm_tidx_header *hdr, qpsdm_ttt_desc *desc, int maxdesc);
+int qpsdm_write_tidx(const char *path, const qpsdm_tidx_header *hdr, const qpsdm_ttt_desc *desc);
+/* get_ttt_all's host part: find the bracketing existing L<n>.<suffix> in
+ * [line1, line2] around `line`, load one or blend two (linear in line),
+ * into out (nfloat floats).  Returns the number of floats, or < 0. */
+long qpsdm_load_line_table(const char *ttpath, const char *suffix, int line, int line1,
+                           int line2, float *out, size_t maxfloat, int *line_lo, int *line_hi);
+
+/* ---------------------------------------------------------------------- */
+/* Q filter bank (kernel a).  Replaces psdm_kernel's host pre-processing   */
+/* and the per-trace H2D copies (src/q-psdm.cu:2173-2226) plus the PFA FFT */
+/* (src/fft.cc), and tabulates the per-point Q-compensated inverse         */
+/* transform that image_depth_gpu's argument list implies                  */
+/* (src/q-psdm.cu:2246-2259: red/imd, nf1, fc, dw, nww, lnG, coef1-4).     */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int nt;
+  float dt;
+  qpsdm_window win;
+  int K;          /* t* levels                                  */
+  float dtstar;   /* t* level spacing (s)                       */
+  float fc;       /* dispersion reference frequency (Hz)        */
+  float threshold;/* gain cap G                                 */
+} qpsdm_bank_params;
+
+/* Reference pre-processing only: re/im spectra [ntr][2*nww] of psdm_kernel
+ * (src/q-psdm.cu:2173-2216) for a batch of decoded traces. */
+int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces_dev, int ntr,
+                        float *re_dev, float *im_dev, void *stream);
+
+/* Q filter bank: traces [ntr][nt] FP32 -> bank [ntr][K][nt] of dtype
+ * QPSDM_F32 or QPSDM_BF16. */
+int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces_dev, int ntr, void *bank_dev,
+                int bank_dtype, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Imaging (kernel b).  Replaces image_depth_gpu (src/q-psdm.cu:2246-2380) */
+/* and its per-trace launch (src/q-psdm.cu:2228-2243).                    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int nx, ny, nz;
+  float dx, dy, dz;
+  int nt;
+  float dt;
+  int K;
+  float dtstar;
+  float tanx1, tanx2, tany1, tany2; /* dip bounds (tan), src/q-psdm.cu:1464-1472 */
+  float taperzone;                  /* radians, src/q-psdm.cu:1287                */
+  float tmin;                       /* floor on each leg's traveltime (s)         */
+  qpsdm_tgeom tg;
+} qpsdm_image_params;
+
+/* Launch shape picked by the autotuner (0 = default). */
+typedef struct {
+  int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
+  int trace_chunk;            /* traces evaluated together per depth point            */
+} qpsdm_launch;
+
+/* One shot: traces [tr0, tr0+ntr) of geom/bank, source (sx, sy); image
+ * float [ny][nx][nz] accumulated in place.  table_dtype QPSDM_F32 or
+ * QPSDM_F16B; bank_dtype QPSDM_F32 or QPSDM_BF16. */
+int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
+                     const float *geom_dev, int tr0, int ntr, const void *bank_dev,
+                     int bank_dtype, const void *tab_dev, int table_dtype, void *stream);
+
+/* Update count of one shot (pairs inside the dip aperture), added to
+ * *count_dev (uint64). */
+int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom_dev, int tr0, int ntr,
+                     unsigned long long *count_dev, void *stream);
+
+/* Output normalisation img /= nsb (src/q-psdm.cu:1760-1766). */
+int qpsdm_image_scale(float *img_dev, size_t n, float s, void *stream);
+
+const char *qpsdm_last_error(void);
+const char *qpsdm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

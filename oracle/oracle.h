@@ -1,0 +1,110 @@
+/*
+ * CPU oracle for the Q-compensated Kirchhoff PSDM imaging path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product path (libqpsdm.so) never links,
+ * loads or calls anything here.
+ *
+ * Every function restates the reference algorithm from its text; the comment
+ * on each names the reference file:line it follows.  Where the reference is a
+ * stub (its imaging kernel fixes traveltimes at 1.0, src/q-psdm.cu:2350-2352)
+ * the oracle states this build's own formula and says so.
+ */
+#ifndef QPSDM_ORACLE_H_
+#define QPSDM_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Segment windows and filter bins of the reference's two-window spectral
+ * pre-processing (src/q-psdm.cu:1593-1645). */
+typedef struct {
+  int nsb;       /* per-segment FFT length, npfar(...)                  */
+  int itb1;      /* first sample of segment 1                           */
+  int itb2;      /* first sample of segment 2                           */
+  int nf1;       /* first kept bin                                      */
+  int nf4;       /* last kept bin                                       */
+  int nww;       /* kept bins, nf4 - nf1 + 1                            */
+  int ncoef1;    /* (int)(10/df), kernel argument of the reference      */
+  int nf3;       /* high-cut taper start bin, (int)(f3/df + 0.5)        */
+  float tdstart1, tdstart2, tdmid;
+  float df, dw;
+} or_window;
+
+/* Velocity / Q model used to build traveltime and t* tables. */
+typedef struct {
+  int kind;            /* 0: v(z) = v0 + k z, constant Q;  1: layers + Gaussian anomaly */
+  double v0, kgrad, q0;
+  int nlayer;
+  double layer_dz;
+  double vl[16], ql[16];
+  double ax, ay, az, asig; /* anomaly centre and sigma (image-relative metres) */
+  double av, aq;           /* fractional velocity drop and Q drop at the centre */
+} or_model;
+
+/* Geometry of a relative-coordinate table set (one 3-D table per surface
+ * node; a position p looks up its nearest node's table at x - p). */
+typedef struct {
+  int nnx, nny;
+  float n0x, n0y, ndx, ndy;
+  int nu, nv, nw;
+  float u0, v0, du, dv, dw;
+} or_tgeom;
+
+/* Parameters of the imaging sum. */
+typedef struct {
+  int nx, ny, nz;
+  float dx, dy, dz;
+  int nt;
+  float dt;
+  int K;
+  float dtstar;                        /* t* spacing of the bank levels       */
+  float tanx1, tanx2, tany1, tany2;    /* dip bounds (tan)                     */
+  float taperzone;                     /* radians                              */
+  float tmin;                          /* floor on each leg's traveltime (s)   */
+  or_tgeom tg;
+} or_iparams;
+
+/* Parameters of the Q filter bank. */
+typedef struct {
+  int nt;
+  float dt;
+  or_window win;
+  int K;
+  float dtstar;
+  float fc;          /* dispersion reference frequency (Hz) */
+  float threshold;   /* gain cap G of the gain limiter      */
+} or_bparams;
+
+int or_npfa(int nmin);
+int or_npfar(int nmin);
+void or_window_params(int ntl, float dt, float f1, float f3, float f4, or_window *w);
+void or_gain_coefs(float threshold, float *lnG, float *c1, float *c2, float *c3, float *c4);
+double or_gain_limit(double x, double G);
+void or_iaztab(int *tab, int n);
+int or_azimuth_index(const int *tab, double sx, double sy, double gx, double gy);
+int or_fetch_and_taper(const float *data, int ns, int it1, int it2, float *out);
+void or_trace_spectra(const float *trace, int nt, float dt, const or_window *w,
+                      float *re, float *im);
+void or_qbank_f64(const float *traces, int ntr, const or_bparams *p, double *bank);
+void or_qbank_f32(const float *traces, int ntr, const or_bparams *p, float *bank);
+void or_ray_times(const or_model *m, double sx, double sy, double x, double y,
+                  double z, double *t, double *tstar);
+void or_build_tables(const or_model *m, const or_tgeom *g, float *tab);
+void or_lookup(const or_tgeom *g, const float *tab, double px, double py,
+               double x, double y, double z, double *t, double *tstar);
+int64_t or_image_f64(const double *bank, const float *geom, int ntr,
+                     const float *tab, const or_iparams *p, double *img);
+int64_t or_image_f32(const float *bank, const float *geom, int ntr,
+                     const float *tab, const or_iparams *p, float *img);
+int64_t or_count(const float *geom, int ntr, const or_iparams *p);
+int or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

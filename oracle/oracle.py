@@ -1,0 +1,250 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and of the reference's own
+host sources (oracle/_ref/libpsdmref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline.  The product package (q_psdm_amd) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libpsdmref.so")
+
+
+class Window(C.Structure):
+    _fields_ = [("nsb", C.c_int), ("itb1", C.c_int), ("itb2", C.c_int), ("nf1", C.c_int),
+                ("nf4", C.c_int), ("nww", C.c_int), ("ncoef1", C.c_int), ("nf3", C.c_int),
+                ("tdstart1", C.c_float), ("tdstart2", C.c_float), ("tdmid", C.c_float),
+                ("df", C.c_float), ("dw", C.c_float)]
+
+
+class Model(C.Structure):
+    _fields_ = [("kind", C.c_int), ("v0", C.c_double), ("kgrad", C.c_double), ("q0", C.c_double),
+                ("nlayer", C.c_int), ("layer_dz", C.c_double), ("vl", C.c_double * 16),
+                ("ql", C.c_double * 16), ("ax", C.c_double), ("ay", C.c_double),
+                ("az", C.c_double), ("asig", C.c_double), ("av", C.c_double), ("aq", C.c_double)]
+
+
+class TGeom(C.Structure):
+    _fields_ = [("nnx", C.c_int), ("nny", C.c_int), ("n0x", C.c_float), ("n0y", C.c_float),
+                ("ndx", C.c_float), ("ndy", C.c_float), ("nu", C.c_int), ("nv", C.c_int),
+                ("nw", C.c_int), ("u0", C.c_float), ("v0", C.c_float), ("du", C.c_float),
+                ("dv", C.c_float), ("dw", C.c_float)]
+
+
+class IParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz", C.c_float), ("nt", C.c_int), ("dt", C.c_float),
+                ("K", C.c_int), ("dtstar", C.c_float), ("tanx1", C.c_float),
+                ("tanx2", C.c_float), ("tany1", C.c_float), ("tany2", C.c_float),
+                ("taperzone", C.c_float), ("tmin", C.c_float), ("tg", TGeom)]
+
+
+class BParams(C.Structure):
+    _fields_ = [("nt", C.c_int), ("dt", C.c_float), ("win", Window), ("K", C.c_int),
+                ("dtstar", C.c_float), ("fc", C.c_float), ("threshold", C.c_float)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        fp, ip, dp = C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.or_npfa.restype = C.c_int
+        L.or_npfar.restype = C.c_int
+        L.or_window_params.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                       C.POINTER(Window)]
+        L.or_gain_coefs.argtypes = [C.c_float, fp, fp, fp, fp, fp]
+        L.or_gain_limit.restype = C.c_double
+        L.or_gain_limit.argtypes = [C.c_double, C.c_double]
+        L.or_iaztab.argtypes = [ip, C.c_int]
+        L.or_azimuth_index.restype = C.c_int
+        L.or_azimuth_index.argtypes = [ip, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.or_fetch_and_taper.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp]
+        L.or_trace_spectra.argtypes = [fp, C.c_int, C.c_float, C.POINTER(Window), fp, fp]
+        L.or_qbank_f64.argtypes = [fp, C.c_int, C.POINTER(BParams), dp]
+        L.or_qbank_f32.argtypes = [fp, C.c_int, C.POINTER(BParams), fp]
+        L.or_ray_times.argtypes = [C.POINTER(Model)] + [C.c_double] * 5 + [dp, dp]
+        L.or_build_tables.argtypes = [C.POINTER(Model), C.POINTER(TGeom), fp]
+        L.or_lookup.argtypes = [C.POINTER(TGeom), fp] + [C.c_double] * 5 + [dp, dp]
+        L.or_image_f64.restype = C.c_int64
+        L.or_image_f64.argtypes = [dp, fp, C.c_int, fp, C.POINTER(IParams), dp]
+        L.or_image_f32.restype = C.c_int64
+        L.or_image_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(IParams), fp]
+        L.or_count.restype = C.c_int64
+        L.or_count.argtypes = [fp, C.c_int, C.POINTER(IParams)]
+        L.or_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own segy.cc/psdmpkg.cc/fft.cc, built by oracle/Makefile."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libpsdmref.so not built")
+        R = C.CDLL(REF_PATH)
+        fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int)
+        R.ref_parseint.restype = C.POINTER(C.c_int)
+        R.ref_parsetuple.restype = C.POINTER(C.c_float)
+        R.ref_getscale.restype = C.c_float
+        R.ref_getscale.argtypes = [C.c_short]
+        R.ref_time_offset_mute.restype = C.c_float
+        R.ref_time_offset_mute.argtypes = [fp, C.c_int, C.c_float]
+        R.ref_strrtrim.restype = C.c_char_p
+        _ref = R
+    return _ref
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def window_params(ntl, dt, f1=1.0, f3=130.0, f4=140.0):
+    w = Window()
+    lib().or_window_params(ntl, dt, f1, f3, f4, C.byref(w))
+    return w
+
+
+def gain_coefs(threshold):
+    v = [C.c_float() for _ in range(5)]
+    lib().or_gain_coefs(threshold, *[C.byref(x) for x in v])
+    return [x.value for x in v]
+
+
+def gain_limit(x, G):
+    return lib().or_gain_limit(float(x), float(G))
+
+
+def iaztab(n=2001):
+    t = np.zeros(n, np.int32)
+    lib().or_iaztab(_i(t), n)
+    return t
+
+
+def azimuth_index(tab, sx, sy, gx, gy):
+    return lib().or_azimuth_index(_i(tab), sx, sy, gx, gy)
+
+
+def fetch_and_taper(data, it1, it2):
+    data = np.ascontiguousarray(data, np.float32)
+    out = np.zeros(it2 - it1 + 1, np.float32)
+    lib().or_fetch_and_taper(_f(data), data.size, it1, it2, _f(out))
+    return out
+
+
+def trace_spectra(trace, dt, win):
+    trace = np.ascontiguousarray(trace, np.float32)
+    re = np.zeros(2 * win.nww, np.float32)
+    im = np.zeros(2 * win.nww, np.float32)
+    lib().or_trace_spectra(_f(trace), trace.size, dt, C.byref(win), _f(re), _f(im))
+    return re, im
+
+
+def bparams(nt, dt, win, K, dtstar, fc, threshold):
+    return BParams(nt, dt, win, K, dtstar, fc, threshold)
+
+
+def qbank(traces, bp, dtype=np.float64):
+    traces = np.ascontiguousarray(traces, np.float32)
+    ntr = traces.shape[0]
+    out = np.zeros((ntr, bp.K, bp.nt), dtype)
+    if dtype == np.float64:
+        lib().or_qbank_f64(_f(traces), ntr, C.byref(bp), _d(out))
+    else:
+        lib().or_qbank_f32(_f(traces), ntr, C.byref(bp), _f(out))
+    return out
+
+
+def model_vz(v0, k, q):
+    m = Model()
+    m.kind, m.v0, m.kgrad, m.q0 = 0, v0, k, q
+    return m
+
+
+def model_layers(vl, ql, layer_dz, anomaly=None):
+    m = Model()
+    m.kind = 1
+    m.nlayer = len(vl)
+    m.layer_dz = layer_dz
+    for i, (v, q) in enumerate(zip(vl, ql)):
+        m.vl[i], m.ql[i] = v, q
+    if anomaly is None:
+        m.ax = m.ay = m.az = 0.0
+        m.asig, m.av, m.aq = 1.0, 0.0, 0.0
+    else:
+        m.ax, m.ay, m.az, m.asig, m.av, m.aq = anomaly
+    return m
+
+
+def ray_times(m, sx, sy, x, y, z):
+    t, ts = C.c_double(), C.c_double()
+    lib().or_ray_times(C.byref(m), sx, sy, x, y, z, C.byref(t), C.byref(ts))
+    return t.value, ts.value
+
+
+def tgeom(**kw):
+    g = TGeom()
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def build_tables(m, g):
+    tab = np.zeros((g.nnx * g.nny, g.nv, g.nu, g.nw, 2), np.float32)
+    lib().or_build_tables(C.byref(m), C.byref(g), _f(tab))
+    return tab
+
+
+def lookup(g, tab, px, py, x, y, z):
+    t, ts = C.c_double(), C.c_double()
+    lib().or_lookup(C.byref(g), _f(tab), px, py, x, y, z, C.byref(t), C.byref(ts))
+    return t.value, ts.value
+
+
+def image(bank, geom, tab, ip, fp32=False):
+    """Kirchhoff sum; returns (image[ny,nx,nz], update count)."""
+    geom = np.ascontiguousarray(geom, np.float32)
+    tab = np.ascontiguousarray(tab, np.float32)
+    ntr = geom.shape[0]
+    if fp32:
+        bank = np.ascontiguousarray(bank, np.float32)
+        img = np.zeros((ip.ny, ip.nx, ip.nz), np.float32)
+        n = lib().or_image_f32(_f(bank), _f(geom), ntr, _f(tab), C.byref(ip), _f(img))
+    else:
+        bank = np.ascontiguousarray(bank, np.float64)
+        img = np.zeros((ip.ny, ip.nx, ip.nz), np.float64)
+        n = lib().or_image_f64(_d(bank), _f(geom), ntr, _f(tab), C.byref(ip), _d(img))
+    return img, n
+
+
+def count(geom, ip):
+    geom = np.ascontiguousarray(geom, np.float32)
+    return lib().or_count(_f(geom), geom.shape[0], C.byref(ip))
+
+
+def num_threads():
+    return lib().or_num_threads()

@@ -1,0 +1,543 @@
+/*
+ * CPU oracle for the Q-compensated Kirchhoff PSDM imaging path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libqpsdm.so and the timed CPU
+ * baseline.  Nothing in the product path links or loads this file's library.
+ *
+ * Each function cites the reference text it restates.  The reference's
+ * imaging kernel is a stub (src/q-psdm.cu:2350-2352 fixes both traveltimes
+ * at 1.0 and the trace never reaches the device), so the imaging formula and
+ * the Q bank are this build's own, written in the reference's conventions
+ * (windows, taper, gain, -i*omega filter, gain limiter, dip-taper weight,
+ * (tr + 1/tr) obliquity weight, z-fastest image, img /= nsb).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------------- */
+/* FFT lengths.  The reference's valid prime-factor lengths are the 240     */
+/* divisors of 720720 = 16*9*5*7*11*13 (src/fft.cc:274-349, NTAB 240);      */
+/* npfa returns the smallest one >= nmin, or 720720 (src/fft.cc:351-372);   */
+/* npfar = 2*npfa((nmin+1)/2) (src/fft.cc:412-439); get_opt_n_fftw = npfar */
+/* (src/fft.cc:3721).  Restated as a divisor search.                       */
+/* ---------------------------------------------------------------------- */
+int or_npfa(int nmin) {
+  const int big = 720720;
+  if (nmin >= big) return big;
+  for (int n = (nmin < 1 ? 1 : nmin); n < big; ++n)
+    if (big % n == 0) return n;
+  return big;
+}
+
+int or_npfar(int nmin) { return 2 * or_npfa((nmin + 1) / 2); }
+
+/* Two-window set-up of one offset class, src/q-psdm.cu:1593-1645, with the */
+/* reference's own float/int types so truncations land on the same values.  */
+void or_window_params(int ntl, float dt, float f1, float f3, float f4, or_window *w) {
+  float t00, t11, tdmid, tdstart1, tdstart2, df, dw;
+  int it00pre = 0.48 / dt;
+  int it00suf = 0.36 / dt;
+  int itb1, itb2, nsb;
+  t00 = 0;
+  t11 = (ntl - 1) * dt;
+  nsb = (t11 - t00 - it00pre * dt - it00suf * dt) / 2.0 / dt;
+  itb1 = t00 / dt;
+  tdstart1 = itb1 * dt;
+  tdmid = t00 + it00pre * dt + nsb * dt;
+  nsb = nsb + it00pre + it00suf;
+  nsb = or_npfar(nsb);
+  itb2 = t11 / dt - nsb + 1;
+  tdstart2 = itb2 * dt;
+  df = 1.0 / nsb / dt;
+  dw = 2.0 * M_PI * df;
+  w->nsb = nsb;
+  w->itb1 = itb1;
+  w->itb2 = itb2;
+  w->tdstart1 = tdstart1;
+  w->tdstart2 = tdstart2;
+  w->tdmid = tdmid;
+  w->df = df;
+  w->dw = dw;
+  w->nf1 = (int)(f1 / df);
+  w->nf4 = (int)(f4 / df);
+  w->nww = w->nf4 - w->nf1 + 1;
+  w->ncoef1 = (int)(10.0 / df);
+  /* per-point high-cut bin, src/q-psdm.cu:1640-1644 (f3 -> nf3) */
+  w->nf3 = (int)(f3 / df + 0.5);
+}
+
+/* Gain-limiter coefficients, src/q-psdm.cu:1277-1283. */
+void or_gain_coefs(float threshold, float *lnG, float *c1, float *c2, float *c3, float *c4) {
+  float l = logf(threshold);
+  *lnG = l;
+  *c1 = threshold * (1.0 - l - 2.5 * l * l);
+  *c2 = threshold * (1.0 + 5.0 * l);
+  *c3 = -2.5 * threshold;
+  *c4 = 1.1 * threshold;
+}
+
+/* Gain-limited exp(x): e^x up to ln G, then the quadratic c1 + c2 x + c3 x^2
+ * (C1-continuous at ln G, reaching its maximum 1.1 G at ln G + 0.2), then
+ * the cap 1.1 G.  The coefficients are the reference's (src/q-psdm.cu:
+ * 1277-1283); evaluated here in FP64. */
+double or_gain_limit(double x, double G) {
+  double L = log(G);
+  if (x <= L) return exp(x);
+  if (x <= L + 0.2) {
+    double c1 = G * (1.0 - L - 2.5 * L * L), c2 = G * (1.0 + 5.0 * L), c3 = -2.5 * G;
+    return c1 + c2 * x + c3 * x * x;
+  }
+  return 1.1 * G;
+}
+
+/* Azimuth-sector table, src/q-psdm.cu:1167-1193 (same float/double mix). */
+void or_iaztab(int *iaztab, int naztab) {
+  float eps = 0.001f, c1, c2, c3;
+  int i, i1, i2, iaz;
+  c2 = (1.0 - cosf(10.0 * 0.0174532925199)) * M_PI * 0.5;
+  c2 = 1.0 - cosf(c2 * 0.5);
+  i2 = (int)(c2 / eps);
+  for (i = 0; i < i2 && i < naztab; ++i) iaztab[i] = 0;
+  for (iaz = 1; iaz < 18; ++iaz) {
+    c2 = iaz * 10.0;
+    c1 = c2 - 10.0;
+    c3 = c2 + 10.0;
+    c1 = (1.0 - cosf(c1 * 0.0174532925199)) * M_PI * 0.5;
+    c2 = (1.0 - cosf(c2 * 0.0174532925199)) * M_PI * 0.5;
+    c3 = (1.0 - cosf(c3 * 0.0174532925199)) * M_PI * 0.5;
+    c1 = 1.0 - cosf(c2 - (c2 - c1) * 0.5);
+    c2 = 1.0 - cosf(c2 + (c3 - c2) * 0.5);
+    i1 = (int)(c1 / eps);
+    i2 = (int)(c2 / eps);
+    for (i = i1; i < i2 && i < naztab; ++i) iaztab[i] = iaz;
+  }
+  for (i = i2; i < naztab; ++i) iaztab[i] = 0;
+}
+
+/* Azimuth index of a trace, src/q-psdm.cu:2125-2144. */
+int or_azimuth_index(const int *iaztab, double sx, double sy, double gx, double gy) {
+  double ddx = gx - sx, ddy = gy - sy;
+  double offset = sqrtf(ddx * ddx + ddy * ddy);
+  float cosa, cosx, c1;
+  if (offset > 1.0) {
+    cosa = ddx / offset;
+    cosx = fabs(cosa);
+    if (ddx * ddy < 0.0) cosx = -cosx;
+    c1 = 1.0 - cosx;
+    return iaztab[(int)(c1 / 0.001f)];
+  }
+  return 0;
+}
+
+/* Window copy with zero fill and 200-sample raised-cosine end tapers,
+ * src/psdmpkg.cc:208-231. */
+int or_fetch_and_taper(const float *data, int ns, int it1, int it2, float *out) {
+  int it, ns1;
+  float c;
+  for (it = it1; it <= it2; ++it) out[it - it1] = (it < 0 || it >= ns) ? 0.0f : data[it];
+  ns1 = it2 - it1 + 1;
+  for (it = 0; it < 200 && it < ns1; ++it) {
+    c = 0.5 - 0.5 * cosf(M_PI * it / 200.0);
+    out[it] *= c;
+    out[ns1 - 1 - it] *= c;
+  }
+  return 1;
+}
+
+/* Forward real DFT (sign -1, unnormalised, the PFA convention documented at
+ * src/fft.cc:1838-1850) of bins [m0, m0+nb), in FP64 with exact index
+ * reduction of the twiddles. */
+static void dft_bins(const float *x, int n, int m0, int nb, double *re, double *im) {
+  double *cs = (double *)malloc(sizeof(double) * 2 * n);
+  for (int a = 0; a < n; ++a) {
+    cs[2 * a] = cos(2.0 * M_PI * a / n);
+    cs[2 * a + 1] = sin(2.0 * M_PI * a / n);
+  }
+  for (int k = 0; k < nb; ++k) {
+    long m = m0 + k;
+    double sr = 0, si = 0;
+    for (int j = 0; j < n; ++j) {
+      long a = (m * j) % n;
+      sr += x[j] * cs[2 * a];
+      si -= x[j] * cs[2 * a + 1];
+    }
+    re[k] = sr;
+    im[k] = si;
+  }
+  free(cs);
+}
+
+/* Per-trace Q pre-processing of the reference, src/q-psdm.cu:2173-2216:
+ * t^2 gain (x0.01 before 0.1 s), full-trace taper, two windowed and tapered
+ * segments, forward FFT of each, bins nf1.. multiplied by -i*omega
+ * (re = w*Im, im = -w*Re, with the reference's float accumulation of w).
+ * Output re[2*nww], im[2*nww]: segment 1 then segment 2. */
+void or_trace_spectra(const float *trace, int nt, float dt, const or_window *w,
+                      float *re, float *im) {
+  float *g = (float *)malloc(sizeof(float) * nt);
+  float *d = (float *)malloc(sizeof(float) * nt);
+  float *seg = (float *)malloc(sizeof(float) * w->nsb);
+  double *xr = (double *)malloc(sizeof(double) * w->nww);
+  double *xi = (double *)malloc(sizeof(double) * w->nww);
+  for (int it = 0; it < nt; ++it) {
+    g[it] = trace[it];
+    if (it * dt < 0.1)
+      g[it] *= 0.1 * 0.1;
+    else
+      g[it] *= (it * dt) * (it * dt);
+  }
+  or_fetch_and_taper(g, nt, 0, nt - 1, d);
+  for (int s = 0; s < 2; ++s) {
+    int itb = s == 0 ? w->itb1 : w->itb2;
+    or_fetch_and_taper(d, nt, itb, itb + w->nsb - 1, seg);
+    dft_bins(seg, w->nsb, w->nf1, w->nww, xr, xi);
+    float wv = w->nf1 * w->dw;
+    for (int iw = 0; iw < w->nww; ++iw, wv += w->dw) {
+      re[s * w->nww + iw] = +(float)xi[iw] * wv;
+      im[s * w->nww + iw] = -(float)xr[iw] * wv;
+    }
+  }
+  free(g);
+  free(d);
+  free(seg);
+  free(xr);
+  free(xi);
+}
+
+/* High-cut taper of bin m: the reference's derivative-filter shape
+ * (src/q-psdm.cu:1236-1247): 1 below nf3, cos^2 ramp from nf3 to nf4,
+ * 0 above nf4. */
+static double hicut(int m, int nf3, int nf4) {
+  if (m > nf4) return 0.0;
+  if (m < nf3 || nf4 <= nf3) return 1.0;
+  double c = cos((double)(m - nf3) / (nf4 - nf3) * M_PI * 0.5);
+  return c * c;
+}
+
+/* Q filter bank of one trace (this build's formula, in the reference's
+ * conventions).  For t* level j (t*_j = j*dtstar) and segment s the
+ * filtered spectrum is C_s(w) * H_j(w) with
+ *     H_j(w) = hicut(w) * g(w t*_j / 2) * exp(-i w t*_j ln(w/w_r) / pi),
+ * g the gain limiter and w_r = 2 pi fc; the level's time trace at sample it
+ * is the unnormalised one-sided inverse transform of segment s at
+ * n = it - itb_s, where s = 2 when it*dt > tdmid (the kernel's segment
+ * select, src/q-psdm.cu:2361-2365):
+ *     bank_j[it] = 2 Re sum_k C_s[k] H_j(w_k) exp(+2 pi i (nf1+k) n / nsb).
+ * The image is divided by nsb on output (src/q-psdm.cu:1760-1766). */
+#define QBANK_BODY(REAL)                                                          \
+  const or_window *w = &p->win;                                                   \
+  int nww = w->nww, nsb = w->nsb, K = p->K, nt = p->nt;                           \
+  double *cs = (double *)malloc(sizeof(double) * 2 * nsb);                        \
+  for (int a = 0; a < nsb; ++a) {                                                 \
+    cs[2 * a] = cos(2.0 * M_PI * a / nsb);                                        \
+    cs[2 * a + 1] = sin(2.0 * M_PI * a / nsb);                                    \
+  }                                                                               \
+  double dwx = 2.0 * M_PI / ((double)nsb * p->dt);                                \
+  double wr = 2.0 * M_PI * p->fc;                                                 \
+  double *hr = (double *)malloc(sizeof(double) * K * nww);                        \
+  double *hi = (double *)malloc(sizeof(double) * K * nww);                        \
+  for (int j = 0; j < K; ++j)                                                     \
+    for (int k = 0; k < nww; ++k) {                                               \
+      int m = w->nf1 + k;                                                         \
+      double om = m * dwx, ts = j * (double)p->dtstar;                            \
+      double amp = hicut(m, w->nf3, w->nf4) * or_gain_limit(om * ts * 0.5, p->threshold); \
+      double ph = om > 0 ? -om * ts * log(om / wr) / M_PI : 0.0;                   \
+      hr[j * nww + k] = amp * cos(ph);                                            \
+      hi[j * nww + k] = amp * sin(ph);                                            \
+    }                                                                             \
+  _Pragma("omp parallel") {                                                       \
+    float *re = (float *)malloc(sizeof(float) * 2 * nww);                         \
+    float *im = (float *)malloc(sizeof(float) * 2 * nww);                         \
+    REAL *fr = (REAL *)malloc(sizeof(REAL) * 2 * nww);                            \
+    REAL *fi = (REAL *)malloc(sizeof(REAL) * 2 * nww);                            \
+    _Pragma("omp for schedule(dynamic)")                                          \
+    for (int itr = 0; itr < ntr; ++itr) {                                         \
+      or_trace_spectra(traces + (size_t)itr * nt, nt, p->dt, w, re, im);          \
+      for (int j = 0; j < K; ++j) {                                               \
+        for (int q = 0; q < 2 * nww; ++q) {                                       \
+          int k = q % nww;                                                        \
+          double a = re[q], b = im[q], c = hr[j * nww + k], d = hi[j * nww + k];  \
+          fr[q] = (REAL)(a * c - b * d);                                          \
+          fi[q] = (REAL)(a * d + b * c);                                          \
+        }                                                                         \
+        for (int it = 0; it < nt; ++it) {                                         \
+          int s = ((float)it * p->dt > w->tdmid) ? 1 : 0;                         \
+          int n = it - (s ? w->itb2 : w->itb1);                                   \
+          REAL acc = 0;                                                           \
+          if (n >= 0 && n < nsb) {                                                \
+            for (int k = 0; k < nww; ++k) {                                       \
+              long a = ((long)(w->nf1 + k) * n) % nsb;                            \
+              acc += fr[s * nww + k] * (REAL)cs[2 * a] - fi[s * nww + k] * (REAL)cs[2 * a + 1]; \
+            }                                                                     \
+          }                                                                       \
+          bank[((size_t)itr * K + j) * nt + it] = 2 * acc;                        \
+        }                                                                         \
+      }                                                                           \
+    }                                                                             \
+    free(re);                                                                     \
+    free(im);                                                                     \
+    free(fr);                                                                     \
+    free(fi);                                                                     \
+  }                                                                               \
+  free(cs);                                                                       \
+  free(hr);                                                                       \
+  free(hi);
+
+void or_qbank_f64(const float *traces, int ntr, const or_bparams *p, double *bank) {
+  QBANK_BODY(double)
+}
+
+void or_qbank_f32(const float *traces, int ntr, const or_bparams *p, float *bank) {
+  QBANK_BODY(float)
+}
+
+/* ---------------------------------------------------------------------- */
+/* Straight-ray traveltime and t* (this build's table builder; the         */
+/* reference ships tables but no builder).  t = int slowness dl and         */
+/* t* = int slowness / Q dl along the straight segment from the surface     */
+/* point (sx, sy, 0) to (x, y, z).                                          */
+/* ---------------------------------------------------------------------- */
+static const double GLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                              -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
+                              0.7966664774136267,  0.9602898564975363};
+static const double GLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
+                              0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
+                              0.2223810344533745, 0.1012285362903763};
+
+void or_ray_times(const or_model *m, double sx, double sy, double x, double y, double z,
+                  double *t, double *tstar) {
+  double hx = x - sx, hy = y - sy;
+  double L = sqrt(hx * hx + hy * hy + z * z);
+  if (m->kind == 0) {
+    /* linear v(z): t = L/(k z) ln(v(z)/v0), L/v0 when k z -> 0 */
+    double tt;
+    if (m->kgrad * z > 1e-9 * m->v0)
+      tt = L / (m->kgrad * z) * log((m->v0 + m->kgrad * z) / m->v0);
+    else
+      tt = L / m->v0;
+    *t = tt;
+    *tstar = tt / m->q0;
+    return;
+  }
+  /* layers + Gaussian anomaly: split at layer boundaries, 8-point
+     Gauss-Legendre on each piece */
+  double acc_t = 0, acc_q = 0;
+  int nb = 0;
+  double lam[20];
+  lam[nb++] = 0.0;
+  for (int l = 1; l < m->nlayer; ++l) {
+    double zb = l * m->layer_dz;
+    if (zb < z) lam[nb++] = zb / z;
+  }
+  lam[nb++] = 1.0;
+  double is2 = 1.0 / (2.0 * m->asig * m->asig);
+  for (int sgi = 0; sgi + 1 < nb; ++sgi) {
+    double a = lam[sgi], b = lam[sgi + 1];
+    if (b <= a) continue;
+    double zm = 0.5 * (a + b) * z;
+    int l = (int)(zm / m->layer_dz);
+    if (l >= m->nlayer) l = m->nlayer - 1;
+    if (l < 0) l = 0;
+    double half = 0.5 * (b - a), mid = 0.5 * (a + b);
+    for (int q = 0; q < 8; ++q) {
+      double la = mid + half * GLX[q];
+      double px = sx + la * hx, py = sy + la * hy, pz = la * z;
+      double dx = px - m->ax, dy = py - m->ay, dz = pz - m->az;
+      double G = exp(-(dx * dx + dy * dy + dz * dz) * is2);
+      double v = m->vl[l] * (1.0 - m->av * G);
+      double qq = m->ql[l] * (1.0 - m->aq * G);
+      acc_t += GLW[q] * half / v;
+      acc_q += GLW[q] * half / (v * qq);
+    }
+  }
+  *t = acc_t * L;
+  *tstar = acc_q * L;
+}
+
+/* Table set: node (jn, in) sits at (n0x + in*ndx, n0y + jn*ndy); its table
+ * holds (t, t*) at relative offsets (u, v) = (u0 + iu*du, v0 + iv*dv) and
+ * depth w = iw*dw, layout [node][iv][iu][iw][2] (z fastest, as the
+ * reference's "z contiguous" tables, src/q-psdm.cu:1158-1159). */
+void or_build_tables(const or_model *m, const or_tgeom *g, float *tab) {
+  int nn = g->nnx * g->nny;
+  size_t per = (size_t)g->nv * g->nu * g->nw;
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int n = 0; n < nn; ++n)
+    for (int iv = 0; iv < g->nv; ++iv) {
+      double px = g->n0x + (double)(n % g->nnx) * g->ndx;
+      double py = g->n0y + (double)(n / g->nnx) * g->ndy;
+      for (int iu = 0; iu < g->nu; ++iu)
+        for (int iw = 0; iw < g->nw; ++iw) {
+          double t, ts;
+          or_ray_times(m, px, py, px + g->u0 + (double)iu * g->du,
+                       py + g->v0 + (double)iv * g->dv, (double)iw * g->dw, &t, &ts);
+          size_t o = (size_t)n * per + ((size_t)iv * g->nu + iu) * g->nw + iw;
+          tab[2 * o] = (float)t;
+          tab[2 * o + 1] = (float)ts;
+        }
+    }
+}
+
+static int nearest_node(double p, float o, float d, int n) {
+  int i = (int)floor((p - o) / d + 0.5);
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+
+static void frac_index(double q, int n, int *i0, double *a) {
+  if (q < 0) q = 0;
+  if (q > n - 1) q = n - 1;
+  int i = (int)q;
+  if (i > n - 2) i = n - 2;
+  *i0 = i;
+  *a = q - i;
+}
+
+/* Nearest-node, shifted-table lookup: (t, t*) from position (px, py, 0) to
+ * (x, y, z) = trilinear interpolation in the nearest node's table at the
+ * relative position (x - px, y - py, z). */
+void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double x,
+               double y, double z, double *t, double *tstar) {
+  int in = nearest_node(px, g->n0x, g->ndx, g->nnx);
+  int jn = nearest_node(py, g->n0y, g->ndy, g->nny);
+  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+  int iu, iv, iw;
+  double a, b, c;
+  frac_index((x - px - g->u0) / g->du, g->nu, &iu, &a);
+  frac_index((y - py - g->v0) / g->dv, g->nv, &iv, &b);
+  frac_index(z / g->dw, g->nw, &iw, &c);
+  double r[2] = {0, 0};
+  for (int k = 0; k < 8; ++k) {
+    int du = k & 1, dv = (k >> 1) & 1, dz = (k >> 2) & 1;
+    double wgt = (du ? a : 1 - a) * (dv ? b : 1 - b) * (dz ? c : 1 - c);
+    size_t o = ((size_t)(iv + dv) * g->nu + (iu + du)) * g->nw + (iw + dz);
+    r[0] += wgt * T[2 * o];
+    r[1] += wgt * T[2 * o + 1];
+  }
+  *t = r[0];
+  *tstar = r[1];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Kirchhoff sum.  Per (trace, point): distances and bisector as in        */
+/* src/q-psdm.cu:2303-2320; dip-bound taper widened by taperzone*(1+tan^2) */
+/* with linear ramps, src/q-psdm.cu:2322-2340; traveltimes and t* from the */
+/* tables (the reference stubs them at 1.0); segment/level pick of the     */
+/* filtered trace by t*, linear interpolation in time with the reference's */
+/* bounds it1 > 0 && it2 < nt (src/q-psdm.cu:2354-2357); weight            */
+/* w*(tr + 1/tr), tr = (ds/dg)^3 tg/ts (src/q-psdm.cu:2372-2374); image    */
+/* z fastest, traces accumulated in order.                                  */
+/* geom: per trace (sx, sy, gx, gy) in image-relative metres.               */
+/* Returns the update count (pairs inside the dip aperture).                */
+/* ---------------------------------------------------------------------- */
+#define IMAGE_BODY(REAL, SQRT)                                                             \
+  int64_t count = 0;                                                                       \
+  const or_tgeom *g = &p->tg;                                                              \
+  REAL tz = p->taperzone;                                                                  \
+  REAL tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;                     \
+  REAL tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);               \
+  REAL ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);               \
+  _Pragma("omp parallel for schedule(dynamic) reduction(+:count)")                         \
+  for (int iy = 0; iy < p->ny; ++iy) {                                                     \
+    for (int itr = 0; itr < ntr; ++itr) {                                                  \
+      REAL sx = geom[4 * itr], sy = geom[4 * itr + 1];                                     \
+      REAL gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];                                 \
+      const BANK *b = bank + (size_t)itr * p->K * p->nt;                                   \
+      REAL y = (REAL)iy * p->dy;                                                           \
+      for (int ix = 0; ix < p->nx; ++ix) {                                                 \
+        REAL x = (REAL)ix * p->dx;                                                         \
+        REAL dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;                       \
+        for (int iz = 1; iz < p->nz; ++iz) {                                               \
+          REAL z = (REAL)iz * p->dz;                                                       \
+          REAL ds = SQRT(dsx * dsx + dsy * dsy + z * z);                                   \
+          REAL dg = SQRT(dgx * dgx + dgy * dgy + z * z);                                   \
+          REAL X = dsx / ds + dgx / dg, Y = dsy / ds + dgy / dg, Z = z / ds + z / dg;      \
+          REAL tanx = X / Z, tany = Y / Z;                                                 \
+          if (tanx <= tx1c || tx2c <= tanx || tany <= ty1c || ty2c <= tany) continue;      \
+          ++count;                                                                         \
+          REAL wt = 1;                                                                     \
+          if (tanx < tx1) wt *= (tanx - tx1c) / (tx1 - tx1c);                              \
+          if (tanx > tx2) wt *= (tx2c - tanx) / (tx2c - tx2);                              \
+          if (tany < ty1) wt *= (tany - ty1c) / (ty1 - ty1c);                              \
+          if (tany > ty2) wt *= (ty2c - tany) / (ty2c - ty2);                              \
+          double a1, a2, b1, b2;                                                           \
+          or_lookup(g, tab, sx, sy, x, y, z, &a1, &a2);                                    \
+          or_lookup(g, tab, gx, gy, x, y, z, &b1, &b2);                                    \
+          REAL ts = (REAL)a1, tg = (REAL)b1;                                               \
+          if (ts < p->tmin) ts = p->tmin;                                                  \
+          if (tg < p->tmin) tg = p->tmin;                                                  \
+          REAL t = ts + tg, tst = (REAL)a2 + (REAL)b2;                                     \
+          int j = (int)(tst / p->dtstar + (REAL)0.5);                                      \
+          if (j < 0) j = 0;                                                                \
+          if (j > p->K - 1) j = p->K - 1;                                                  \
+          REAL f = t / p->dt;                                                              \
+          int i0 = (int)f;                                                                 \
+          if (!(i0 > 0 && i0 + 1 < p->nt)) continue;                                       \
+          REAL fr = f - (REAL)i0;                                                          \
+          const BANK *bj = b + (size_t)j * p->nt;                                          \
+          REAL sv = ((REAL)1 - fr) * (REAL)bj[i0] + fr * (REAL)bj[i0 + 1];                 \
+          REAL r = ds / dg;                                                                \
+          REAL tr = r * r * r * tg / ts;                                                   \
+          REAL W = wt * (tr + (REAL)1 / tr);                                               \
+          img[((size_t)iy * p->nx + ix) * p->nz + iz] += W * sv;                           \
+        }                                                                                  \
+      }                                                                                    \
+    }                                                                                      \
+  }                                                                                        \
+  return count;
+
+int64_t or_image_f64(const double *bank, const float *geom, int ntr, const float *tab,
+                     const or_iparams *p, double *img) {
+  typedef double BANK;
+  IMAGE_BODY(double, sqrt)
+}
+
+int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float *tab,
+                     const or_iparams *p, float *img) {
+  typedef float BANK;
+  IMAGE_BODY(float, sqrtf)
+}
+
+/* Update count alone: pairs (trace, point) with z > 0 inside the widened
+ * dip bounds, evaluated in FP32 as the imaging sum does. */
+int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
+  int64_t count = 0;
+  float tz = p->taperzone;
+  float tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;
+  float tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);
+  float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
+#pragma omp parallel for schedule(dynamic) reduction(+ : count)
+  for (int iy = 0; iy < p->ny; ++iy)
+    for (int itr = 0; itr < ntr; ++itr) {
+      float sx = geom[4 * itr], sy = geom[4 * itr + 1];
+      float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
+      float y = (float)iy * p->dy;
+      for (int ix = 0; ix < p->nx; ++ix) {
+        float x = (float)ix * p->dx;
+        float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
+        for (int iz = 1; iz < p->nz; ++iz) {
+          float z = (float)iz * p->dz;
+          float ds = sqrtf(dsx * dsx + dsy * dsy + z * z);
+          float dg = sqrtf(dgx * dgx + dgy * dgy + z * z);
+          float X = dsx / ds + dgx / dg, Y = dsy / ds + dgy / dg, Z = z / ds + z / dg;
+          float tanx = X / Z, tany = Y / Z;
+          if (tanx <= tx1c || tx2c <= tanx || tany <= ty1c || ty2c <= tany) continue;
+          ++count;
+        }
+      }
+    }
+  return count;
+}

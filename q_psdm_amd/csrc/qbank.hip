@@ -1,0 +1,374 @@
+// Kernel (a): batched per-trace Q-compensation filter bank for gfx950.
+//
+// One work-group per trace.  In LDS:
+//   1. t^2 gain, full-trace raised-cosine taper and the two tapered windows
+//      of the reference's pre-processing (src/q-psdm.cu:2173-2216,
+//      src/psdmpkg.cc:208-231), packed as one complex sequence
+//      z = seg1 + i seg2;
+//   2. forward mixed-radix Stockham FFT of length nsb (radices 2,3,4,5,7,
+//      11,13 cover every reference length 2*divisor(720720)), split into
+//      the two segment spectra and multiplied by -i*omega on bins nf1..nf4
+//      (the reference's re = w Im, im = -w Re);
+//   3. for each t* level j: filter H_j (FP64-generated on the host: high-cut
+//      taper, gain-limited exp(w t*/2), Kolsky dispersion phase), pack both
+//      segments as one Hermitian pair, inverse FFT, and write the level's
+//      time trace, segment 1 up to tdmid and segment 2 after it.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxPasses = 16;
+
+struct Plan {
+  int npass;
+  int radix[kMaxPasses];
+};
+
+struct BankDev {
+  int nt;
+  float dt;
+  int nsb, itb1, itb2, nf1, nww;
+  float tdmid;
+  int K;
+  Plan plan;
+  const float2 *tw;     // [nsb] exp(-2 pi i a / nsb)
+  const float2 *H;      // [K][nww]
+  const float *wtab;    // [nww] omega of each kept bin
+  const float *taper;   // [200] raised-cosine coefficients
+};
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// One Stockham pass of radix R (SIGN -1 forward, +1 inverse).
+template <int R, int SIGN>
+__device__ void stockham_pass(const float2 *__restrict__ in, float2 *__restrict__ out,
+                              const float2 *__restrict__ tw, int N, int Ns) {
+  const int nb = N / R;
+  const int step = N / (Ns * R);
+  float2 wr[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    float2 w = tw[m * (N / R)];
+    wr[m] = SIGN > 0 ? cconj(w) : w;
+  }
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const int k = b % Ns;
+    float2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = in[b + q * nb];
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      float2 w = tw[q * k * step];
+      v[q] = cmul(v[q], SIGN > 0 ? cconj(w) : w);
+    }
+    const int base = (b / Ns) * Ns * R + k;
+#pragma unroll
+    for (int p = 0; p < R; ++p) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const float2 w = wr[(p * q) % R];
+        acc.x += v[q].x * w.x - v[q].y * w.y;
+        acc.y += v[q].x * w.y + v[q].y * w.x;
+      }
+      out[base + p * Ns] = acc;
+    }
+  }
+}
+
+// Runs the plan; data starts in a, the result pointer is returned.
+template <int SIGN>
+__device__ float2 *fft_run(float2 *a, float2 *b, const float2 *tw, int N, const Plan &pl) {
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.radix[p];
+    switch (R) {
+      case 2: stockham_pass<2, SIGN>(a, b, tw, N, Ns); break;
+      case 3: stockham_pass<3, SIGN>(a, b, tw, N, Ns); break;
+      case 4: stockham_pass<4, SIGN>(a, b, tw, N, Ns); break;
+      case 5: stockham_pass<5, SIGN>(a, b, tw, N, Ns); break;
+      case 7: stockham_pass<7, SIGN>(a, b, tw, N, Ns); break;
+      case 11: stockham_pass<11, SIGN>(a, b, tw, N, Ns); break;
+      default: stockham_pass<13, SIGN>(a, b, tw, N, Ns); break;
+    }
+    Ns *= R;
+    __syncthreads();
+    float2 *t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// fetch_and_taper's factor for position i of an n-sample window: the
+// reference multiplies by c[i] at step i and by c[n-1-i] at step n-1-i
+// (src/psdmpkg.cc:224-229); the earlier step is applied first.
+__device__ inline float taper_apply(float x, int i, int n, const float *c) {
+  const int j = n - 1 - i;
+  const bool a = i < 200, b = j < 200;
+  if (i <= j) {
+    if (a) x *= c[i];
+    if (b) x *= c[j];
+  } else {
+    if (b) x *= c[j];
+    if (a) x *= c[i];
+  }
+  return x;
+}
+
+__device__ inline float gain_t2(float x, int it, float dt) {
+  const float t = it * dt;
+  if (t < 0.1) return (float)((double)x * (0.1 * 0.1));
+  return x * (t * t);
+}
+
+__device__ inline void store_out(float *p, float v) { *p = v; }
+__device__ inline void store_out(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+
+// MODE 0: spectra only (re/im of the reference pre-processing).
+// MODE 1: full bank into OutT.
+template <int MODE, typename OutT>
+__global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float *__restrict__ traces,
+                                                         OutT *__restrict__ bank,
+                                                         float *__restrict__ re_out,
+                                                         float *__restrict__ im_out) {
+  extern __shared__ float2 lds[];
+  const int N = P.nsb;
+  float2 *A = lds;
+  float2 *B = lds + N;
+  float2 *tw = lds + 2 * N;
+  float2 *C1 = tw + N;
+  float2 *C2 = C1 + P.nww;
+  __shared__ float tap[200];
+  const int tr = blockIdx.x;
+  const float *x = traces + (size_t)tr * P.nt;
+
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < 200; i += blockDim.x) tap[i] = P.taper[i];
+  float *d = reinterpret_cast<float *>(B);  // gained + tapered trace, aliases B
+  __syncthreads();
+  for (int it = threadIdx.x; it < P.nt; it += blockDim.x)
+    d[it] = taper_apply(gain_t2(x[it], it, P.dt), it, P.nt, tap);
+  __syncthreads();
+  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+    const int i1 = P.itb1 + n, i2 = P.itb2 + n;
+    const float s1 = (i1 >= 0 && i1 < P.nt) ? d[i1] : 0.0f;
+    const float s2 = (i2 >= 0 && i2 < P.nt) ? d[i2] : 0.0f;
+    A[n] = make_float2(taper_apply(s1, n, N, tap), taper_apply(s2, n, N, tap));
+  }
+  __syncthreads();
+  const float2 *Z = fft_run<-1>(A, B, tw, N, P.plan);
+  for (int k = threadIdx.x; k < P.nww; k += blockDim.x) {
+    const int m = P.nf1 + k;
+    const float2 zp = Z[m], zm = cconj(Z[N - m]);
+    const float2 x1 = make_float2(0.5f * (zp.x + zm.x), 0.5f * (zp.y + zm.y));
+    const float2 x2 = make_float2(0.5f * (zp.y - zm.y), -0.5f * (zp.x - zm.x));
+    const float w = P.wtab[k];
+    C1[k] = make_float2(x1.y * w, -x1.x * w);
+    C2[k] = make_float2(x2.y * w, -x2.x * w);
+    if (MODE == 0) {
+      re_out[(size_t)tr * 2 * P.nww + k] = C1[k].x;
+      im_out[(size_t)tr * 2 * P.nww + k] = C1[k].y;
+      re_out[(size_t)tr * 2 * P.nww + P.nww + k] = C2[k].x;
+      im_out[(size_t)tr * 2 * P.nww + P.nww + k] = C2[k].y;
+    }
+  }
+  if (MODE == 0) return;
+  __syncthreads();
+  for (int j = 0; j < P.K; ++j) {
+    const float2 *Hj = P.H + (size_t)j * P.nww;
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+      float2 v = make_float2(0.f, 0.f);
+      const int kp = n - P.nf1, km = N - n - P.nf1;
+      if (kp >= 0 && kp < P.nww) {
+        const float2 h = Hj[kp];
+        const float2 s1 = cmul(C1[kp], h), s2 = cmul(C2[kp], h);
+        v = make_float2(s1.x - s2.y, s1.y + s2.x);  // s1 + i s2
+      } else if (km >= 0 && km < P.nww) {
+        const float2 h = Hj[km];
+        const float2 s1 = cconj(cmul(C1[km], h)), s2 = cconj(cmul(C2[km], h));
+        v = make_float2(s1.x - s2.y, s1.y + s2.x);
+      }
+      A[n] = v;
+    }
+    __syncthreads();
+    const float2 *Y = fft_run<+1>(A, B, tw, N, P.plan);
+    OutT *o = bank + ((size_t)tr * P.K + j) * P.nt;
+    for (int it = threadIdx.x; it < P.nt; it += blockDim.x) {
+      const bool s2 = it * P.dt > P.tdmid;
+      const int n = it - (s2 ? P.itb2 : P.itb1);
+      float v = 0.f;
+      if (n >= 0 && n < N) v = s2 ? Y[n].y : Y[n].x;
+      store_out(o + it, v);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- host side --------------------------------------------------------
+struct Coefs {
+  float2 *tw = nullptr;
+  float2 *H = nullptr;
+  float *wtab = nullptr;
+  float *taper = nullptr;
+  Plan plan{};
+};
+
+std::mutex g_mu;
+std::map<std::vector<double>, Coefs> g_cache;
+
+bool make_plan(int n, Plan *pl) {
+  pl->npass = 0;
+  const int order[] = {4, 2, 3, 5, 7, 11, 13};
+  for (int r : order) {
+    while (n % r == 0 && pl->npass < kMaxPasses) {
+      if (r == 2 && n % 4 == 0) break;
+      pl->radix[pl->npass++] = r;
+      n /= r;
+    }
+  }
+  return n == 1;
+}
+
+double gain_limit(double x, double G) {
+  const double L = log(G);
+  if (x <= L) return exp(x);
+  if (x <= L + 0.2) {
+    const double c1 = G * (1.0 - L - 2.5 * L * L), c2 = G * (1.0 + 5.0 * L), c3 = -2.5 * G;
+    return c1 + c2 * x + c3 * x * x;
+  }
+  return 1.1 * G;
+}
+
+int get_coefs(const qpsdm_bank_params *p, Coefs **out) {
+  const qpsdm_window &w = p->win;
+  std::vector<double> key = {(double)p->nt, p->dt, (double)w.nsb, (double)w.nf1, (double)w.nww,
+                             (double)w.nf3, (double)w.nf4, w.dw, (double)p->K, p->dtstar, p->fc,
+                             p->threshold};
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) {
+    *out = &it->second;
+    return QPSDM_OK;
+  }
+  Coefs c;
+  if (!make_plan(w.nsb, &c.plan))
+    return qpsdm::set_error(QPSDM_EINVAL, "nsb=%d has a prime factor above 13", w.nsb);
+  const int N = w.nsb, K = p->K, nww = w.nww;
+  std::vector<float2> tw(N), H((size_t)K * nww);
+  std::vector<float> wt(nww), tap(200);
+  for (int a = 0; a < N; ++a)
+    tw[a] = make_float2((float)cos(2.0 * M_PI * a / N), (float)(-sin(2.0 * M_PI * a / N)));
+  // coefficient generation in FP64: high cut (the reference's derivative
+  // filter shape, src/q-psdm.cu:1236-1247), gain limiter, dispersion
+  const double dwx = 2.0 * M_PI / ((double)N * p->dt), wr = 2.0 * M_PI * p->fc;
+  for (int j = 0; j < K; ++j)
+    for (int k = 0; k < nww; ++k) {
+      const int m = w.nf1 + k;
+      double hc = 1.0;
+      if (m > w.nf4) hc = 0.0;
+      else if (m >= w.nf3 && w.nf4 > w.nf3) {
+        const double c = cos((double)(m - w.nf3) / (w.nf4 - w.nf3) * M_PI * 0.5);
+        hc = c * c;
+      }
+      const double om = m * dwx, ts = j * (double)p->dtstar;
+      const double amp = hc * gain_limit(om * ts * 0.5, p->threshold);
+      const double ph = -om * ts * log(om / wr) / M_PI;
+      H[(size_t)j * nww + k] = make_float2((float)(amp * cos(ph)), (float)(amp * sin(ph)));
+    }
+  // the reference's float accumulation of omega (src/q-psdm.cu:2186-2191)
+  float wv = w.nf1 * w.dw;
+  for (int k = 0; k < nww; ++k, wv += w.dw) wt[k] = wv;
+  for (int i = 0; i < 200; ++i) tap[i] = 0.5 - 0.5 * cosf(M_PI * i / 200.0);
+  QPSDM_HIP(hipMalloc(&c.tw, sizeof(float2) * N));
+  QPSDM_HIP(hipMalloc(&c.H, sizeof(float2) * H.size()));
+  QPSDM_HIP(hipMalloc(&c.wtab, sizeof(float) * nww));
+  QPSDM_HIP(hipMalloc(&c.taper, sizeof(float) * 200));
+  QPSDM_HIP(hipMemcpy(c.tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  QPSDM_HIP(hipMemcpy(c.H, H.data(), sizeof(float2) * H.size(), hipMemcpyHostToDevice));
+  QPSDM_HIP(hipMemcpy(c.wtab, wt.data(), sizeof(float) * nww, hipMemcpyHostToDevice));
+  QPSDM_HIP(hipMemcpy(c.taper, tap.data(), sizeof(float) * 200, hipMemcpyHostToDevice));
+  auto res = g_cache.emplace(key, c);
+  *out = &res.first->second;
+  return QPSDM_OK;
+}
+
+int check_params(const qpsdm_bank_params *p) {
+  QPSDM_REQUIRE(p, "bank: null params");
+  const qpsdm_window &w = p->win;
+  QPSDM_REQUIRE(p->nt > 1 && p->dt > 0 && p->K >= 1, "bank: bad nt/dt/K");
+  QPSDM_REQUIRE(w.nsb >= 4 && w.nsb <= 8192, "bank: nsb=%d outside [4, 8192]", w.nsb);
+  QPSDM_REQUIRE(2 * w.nsb >= p->nt, "bank: trace longer than 2*nsb");
+  QPSDM_REQUIRE(w.nf1 >= 1 && w.nww >= 1 && w.nf1 + w.nww - 1 < w.nsb / 2,
+                "bank: kept bins must lie strictly between DC and Nyquist");
+  return QPSDM_OK;
+}
+
+int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank, int dtype,
+           float *re, float *im, void *stream) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  QPSDM_REQUIRE(traces && ntr >= 0, "bank: bad traces");
+  if (ntr == 0) return QPSDM_OK;
+  Coefs *c = nullptr;
+  rc = get_coefs(p, &c);
+  if (rc) return rc;
+  BankDev P;
+  P.nt = p->nt;
+  P.dt = p->dt;
+  P.nsb = p->win.nsb;
+  P.itb1 = p->win.itb1;
+  P.itb2 = p->win.itb2;
+  P.nf1 = p->win.nf1;
+  P.nww = p->win.nww;
+  P.tdmid = p->win.tdmid;
+  P.K = p->K;
+  P.plan = c->plan;
+  P.tw = c->tw;
+  P.H = c->H;
+  P.wtab = c->wtab;
+  P.taper = c->taper;
+  const size_t lds = sizeof(float2) * (3 * (size_t)P.nsb + 2 * P.nww);
+  hipStream_t s = (hipStream_t)stream;
+  if (re) {
+    hipLaunchKernelGGL((qbank_kernel<0, float>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
+                       (float *)nullptr, re, im);
+  } else if (dtype == QPSDM_F32) {
+    hipLaunchKernelGGL((qbank_kernel<1, float>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
+                       (float *)bank, (float *)nullptr, (float *)nullptr);
+  } else if (dtype == QPSDM_BF16) {
+    hipLaunchKernelGGL((qbank_kernel<1, __hip_bfloat16>), dim3(ntr), dim3(kThreads), lds, s, P,
+                       traces, (__hip_bfloat16 *)bank, (float *)nullptr, (float *)nullptr);
+  } else {
+    return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16", dtype);
+  }
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+}  // namespace
+
+extern "C" int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces, int ntr,
+                                   float *re, float *im, void *stream) {
+  QPSDM_REQUIRE(re && im, "spectra: null outputs");
+  return launch(p, traces, ntr, nullptr, QPSDM_F32, re, im, stream);
+}
+
+extern "C" int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
+                           int dtype, void *stream) {
+  QPSDM_REQUIRE(bank, "bank: null output");
+  return launch(p, traces, ntr, bank, dtype, nullptr, nullptr, stream);
+}

@@ -1,0 +1,204 @@
+// Traveltime / t* table kernels: straight-ray builder (FP64), FP16
+// offset+scale brick quantisation, and the line blend of get_ttt_all.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.h"
+#include "image_dev.h"
+
+namespace {
+
+struct ModelDev {
+  int kind;
+  double v0, kgrad, q0;
+  int nlayer;
+  double layer_dz;
+  double vl[16], ql[16];
+  double ax, ay, az, is2, av, aq;
+};
+
+__constant__ double kGLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                               -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
+                               0.7966664774136267,  0.9602898564975363};
+__constant__ double kGLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
+                               0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
+                               0.2223810344533745, 0.1012285362903763};
+
+// t = int slowness dl, t* = int slowness/Q dl on the straight segment from
+// (sx, sy, 0) to (x, y, z): closed form for linear v(z), else 8-point
+// Gauss-Legendre on each layer piece.
+__device__ void ray_times(const ModelDev &m, double sx, double sy, double x, double y, double z,
+                          double *t, double *tstar) {
+  const double hx = x - sx, hy = y - sy;
+  const double L = sqrt(hx * hx + hy * hy + z * z);
+  if (m.kind == 0) {
+    double tt;
+    if (m.kgrad * z > 1e-9 * m.v0)
+      tt = L / (m.kgrad * z) * log((m.v0 + m.kgrad * z) / m.v0);
+    else
+      tt = L / m.v0;
+    *t = tt;
+    *tstar = tt / m.q0;
+    return;
+  }
+  double at = 0, aq = 0;
+  double prev = 0.0;
+  for (int l = 0; l < m.nlayer; ++l) {
+    double nxt = 1.0;
+    if (l + 1 < m.nlayer) {
+      const double zb = (l + 1) * m.layer_dz;
+      if (zb < z) nxt = zb / z;
+    }
+    if (nxt > prev) {
+      const double zm = 0.5 * (prev + nxt) * z;
+      int li = (int)(zm / m.layer_dz);
+      li = li >= m.nlayer ? m.nlayer - 1 : (li < 0 ? 0 : li);
+      const double half = 0.5 * (nxt - prev), mid = 0.5 * (prev + nxt);
+      for (int q = 0; q < 8; ++q) {
+        const double la = mid + half * kGLX[q];
+        const double dx = sx + la * hx - m.ax, dy = sy + la * hy - m.ay, dz = la * z - m.az;
+        const double G = exp(-(dx * dx + dy * dy + dz * dz) * m.is2);
+        const double v = m.vl[li] * (1.0 - m.av * G);
+        const double qq = m.ql[li] * (1.0 - m.aq * G);
+        at += kGLW[q] * half / v;
+        aq += kGLW[q] * half / (v * qq);
+      }
+    }
+    prev = nxt;
+    if (nxt >= 1.0) break;
+  }
+  *t = at * L;
+  *tstar = aq * L;
+}
+
+__global__ void build_kernel(ModelDev m, qpsdm_tgeom g, float2 *tab) {
+  const size_t per = (size_t)g.nv * g.nu * g.nw;
+  const size_t total = per * g.nnx * g.nny;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t node = i / per;
+    size_t r = i - node * per;
+    const int iw = (int)(r % g.nw);
+    r /= g.nw;
+    const int iu = (int)(r % g.nu);
+    const int iv = (int)(r / g.nu);
+    const double px = g.n0x + (double)(node % g.nnx) * g.ndx;
+    const double py = g.n0y + (double)(node / g.nnx) * g.ndy;
+    double t, ts;
+    ray_times(m, px, py, px + g.u0 + (double)iu * g.du, py + g.v0 + (double)iv * g.dv,
+              (double)iw * g.dw, &t, &ts);
+    tab[i] = make_float2((float)t, (float)ts);
+  }
+}
+
+// One work-group per brick: min/max of t and t* over the brick's valid
+// cells, then value = off + scale * h with h in [0, 1] stored as FP16.
+__global__ __launch_bounds__(512) void quant_kernel(qpsdm_tgeom g, const float2 *tab,
+                                                    unsigned char *out) {
+  const int nbv = (g.nv + 7) / 8, nbu = (g.nu + 7) / 8, nbw = (g.nw + 7) / 8;
+  const size_t nb_node = (size_t)nbv * nbu * nbw;
+  const size_t b = blockIdx.x;
+  const size_t node = b / nb_node;
+  size_t r = b - node * nb_node;
+  const int bw = (int)(r % nbw);
+  r /= nbw;
+  const int bu = (int)(r % nbu), bv = (int)(r / nbu);
+  const int tid = threadIdx.x;
+  const int iv = bv * 8 + (tid >> 6), iu = bu * 8 + ((tid >> 3) & 7), iw = bw * 8 + (tid & 7);
+  const bool in = iv < g.nv && iu < g.nu && iw < g.nw;
+  float2 v = make_float2(0.f, 0.f);
+  if (in) v = tab[node * (size_t)g.nv * g.nu * g.nw + ((size_t)iv * g.nu + iu) * g.nw + iw];
+  __shared__ float red[4][512];
+  red[0][tid] = in ? v.x : INFINITY;
+  red[1][tid] = in ? v.x : -INFINITY;
+  red[2][tid] = in ? v.y : INFINITY;
+  red[3][tid] = in ? v.y : -INFINITY;
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (tid < s) {
+      red[0][tid] = fminf(red[0][tid], red[0][tid + s]);
+      red[1][tid] = fmaxf(red[1][tid], red[1][tid + s]);
+      red[2][tid] = fminf(red[2][tid], red[2][tid + s]);
+      red[3][tid] = fmaxf(red[3][tid], red[3][tid + s]);
+    }
+    __syncthreads();
+  }
+  const float t0 = red[0][0], ts = fmaxf(red[1][0] - t0, 1e-30f);
+  const float s0 = red[2][0], ss = fmaxf(red[3][0] - s0, 1e-30f);
+  unsigned char *o = out + b * qpsdm_dev::kBrickBytes;
+  if (tid == 0) *(float4 *)o = make_float4(t0, ts, s0, ss);
+  const __half2 h = in ? __floats2half2_rn((v.x - t0) / ts, (v.y - s0) / ss) : __floats2half2_rn(0.f, 0.f);
+  *(__half2 *)(o + 16 + tid * 4) = h;
+}
+
+__global__ void blend_kernel(const float *a, const float *b, size_t n, float c1, float c2, float *o) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    o[i] = a[i] * c1 + b[i] * c2;
+}
+
+int check_geom(const qpsdm_tgeom *g) {
+  QPSDM_REQUIRE(g && g->nnx >= 1 && g->nny >= 1 && g->nu >= 2 && g->nv >= 2 && g->nw >= 2,
+                "tables: bad geometry");
+  return QPSDM_OK;
+}
+
+}  // namespace
+
+extern "C" int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab,
+                                  void *stream) {
+  int rc = check_geom(g);
+  if (rc) return rc;
+  QPSDM_REQUIRE(m && tab, "tables_build: null args");
+  QPSDM_REQUIRE(m->kind == 0 || (m->kind == 1 && m->nlayer >= 1 && m->nlayer <= 16 && m->layer_dz > 0),
+                "tables_build: bad model");
+  ModelDev d;
+  d.kind = m->kind;
+  d.v0 = m->v0;
+  d.kgrad = m->kgrad;
+  d.q0 = m->q0;
+  d.nlayer = m->nlayer;
+  d.layer_dz = m->layer_dz;
+  for (int i = 0; i < 16; ++i) d.vl[i] = m->vl[i], d.ql[i] = m->ql[i];
+  d.ax = m->ax;
+  d.ay = m->ay;
+  d.az = m->az;
+  d.is2 = m->asig > 0 ? 1.0 / (2.0 * m->asig * m->asig) : 0.0;
+  d.av = m->av;
+  d.aq = m->aq;
+  hipLaunchKernelGGL(build_kernel, dim3(4096), dim3(256), 0, (hipStream_t)stream, d, *g,
+                     (float2 *)tab);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+extern "C" size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g) {
+  if (check_geom(g)) return 0;
+  return qpsdm_dev::f16b_node_bytes(*g) * g->nnx * g->nny;
+}
+
+extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, void *out,
+                                     void *stream) {
+  int rc = check_geom(g);
+  if (rc) return rc;
+  QPSDM_REQUIRE(tab && out, "quantize: null args");
+  const size_t nb = (size_t)((g->nv + 7) / 8) * ((g->nu + 7) / 8) * ((g->nw + 7) / 8) * g->nnx *
+                    g->nny;
+  QPSDM_REQUIRE(nb < (1ull << 31), "quantize: too many bricks for one launch");
+  hipLaunchKernelGGL(quant_kernel, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
+                     (const float2 *)tab, (unsigned char *)out);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_tables_blend(const float *a, const float *b, size_t n, float c, float *o,
+                                  void *stream) {
+  QPSDM_REQUIRE(a && b && o, "blend: null args");
+  if (n == 0) return QPSDM_OK;
+  // same float weights as get_ttt_all: c2 = c, c1 = 1 - c2
+  const float c2 = c, c1 = 1.0f - c2;
+  hipLaunchKernelGGL(blend_kernel, dim3(2048), dim3(256), 0, (hipStream_t)stream, a, b, n, c1, c2, o);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}

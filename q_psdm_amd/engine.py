@@ -1,0 +1,155 @@
+"""Host-side mirror of the reference's imaging operator over libqpsdm.so.
+
+The reference drives one trace at a time through psdm_kernel()
+(src/q-psdm.cu:2063-2244) with ~85 scalar arguments; here the same state is
+grouped into a Survey (window, bank, image and table parameters derived once
+per offset class, as qpsdm() does at src/q-psdm.cu:1593-1665) and the
+operators work on batches of device-resident traces.  torch supplies device
+memory and the current stream; every computation runs in libqpsdm's HIP
+kernels (no CPU or torch fallback).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import lib as L
+from .lib import check
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+def window(nt, dt, f1, f3, f4):
+    w = L.Window()
+    check(L.lib().qpsdm_window_params(nt, dt, f1, f3, f4, C.byref(w)))
+    return w
+
+
+def npfar(n):
+    return L.lib().qpsdm_npfar(n)
+
+
+def gain_coefs(threshold):
+    out = (C.c_float * 5)()
+    check(L.lib().qpsdm_gain_coefs(threshold, out))
+    return list(out)
+
+
+def iaztab(n=2001):
+    t = (C.c_int * n)()
+    check(L.lib().qpsdm_iaztab(t, n))
+    return np.frombuffer(t, dtype=np.int32).copy()
+
+
+def model_struct(m):
+    s = L.Model()
+    s.kind, s.v0, s.kgrad, s.q0 = m["kind"], m["v0"], m["kgrad"], m["q0"]
+    s.nlayer, s.layer_dz = m["nlayer"], m["layer_dz"]
+    for i, (v, q) in enumerate(zip(m["vl"], m["ql"])):
+        s.vl[i], s.ql[i] = v, q
+    s.ax, s.ay, s.az, s.asig, s.av, s.aq = m["ax"], m["ay"], m["az"], m["asig"], m["av"], m["aq"]
+    return s
+
+
+def tgeom_struct(g):
+    s = L.TGeom()
+    for k, v in g.items():
+        setattr(s, k, v)
+    return s
+
+
+class Survey:
+    """Per-offset-class state of the imaging path for one configuration."""
+
+    def __init__(self, cfg, tstar_max):
+        self.cfg = cfg
+        self.win = window(cfg.nt, cfg.dt, cfg.f1, cfg.f3, cfg.f4)
+        self.tstar_max = float(tstar_max)
+        self.dtstar = self.tstar_max / max(cfg.K - 1, 1)
+        bp = L.BankParams()
+        bp.nt, bp.dt, bp.win, bp.K = cfg.nt, cfg.dt, self.win, cfg.K
+        bp.dtstar, bp.fc, bp.threshold = self.dtstar, cfg.fc, cfg.threshold
+        self.bp = bp
+        ip = L.ImageParams()
+        ip.nx, ip.ny, ip.nz, ip.dx, ip.dy, ip.dz = cfg.nx, cfg.ny, cfg.nz, cfg.dx, cfg.dy, cfg.dz
+        ip.nt, ip.dt, ip.K, ip.dtstar = cfg.nt, cfg.dt, cfg.K, self.dtstar
+        ip.tanx1, ip.tanx2, ip.tany1, ip.tany2, ip.taperzone = cfg.dip_bounds()
+        ip.tmin = cfg.tmin
+        ip.tg = tgeom_struct(cfg.tgeom)
+        self.ip = ip
+        self.tg = ip.tg
+
+    # -- tables ----------------------------------------------------------
+    def table_shape(self):
+        g = self.cfg.tgeom
+        return (g["nnx"] * g["nny"], g["nv"], g["nu"], g["nw"], 2)
+
+    def build_tables(self, stream=None):
+        tab = torch.empty(self.table_shape(), dtype=torch.float32, device="cuda")
+        check(L.lib().qpsdm_tables_build(C.byref(model_struct(self.cfg.model)), C.byref(self.tg),
+                                         _ptr(tab), _stream(stream)))
+        return tab
+
+    def quantize_tables(self, tab, stream=None):
+        nbytes = L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
+        out = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        check(L.lib().qpsdm_tables_quantize(C.byref(self.tg), _ptr(tab), _ptr(out), _stream(stream)))
+        return out
+
+    # -- kernel (a) --------------------------------------------------------
+    def trace_spectra(self, traces, stream=None):
+        n = traces.shape[0]
+        re = torch.empty((n, 2 * self.win.nww), dtype=torch.float32, device="cuda")
+        im = torch.empty_like(re)
+        check(L.lib().qpsdm_trace_spectra(C.byref(self.bp), _ptr(traces), n, _ptr(re), _ptr(im),
+                                          _stream(stream)))
+        return re, im
+
+    def qbank(self, traces, dtype=torch.float32, out=None, stream=None):
+        n = traces.shape[0]
+        assert traces.dtype == torch.float32 and traces.shape[1] == self.cfg.nt
+        if out is None:
+            out = torch.empty((n, self.cfg.K, self.cfg.nt), dtype=dtype, device="cuda")
+        code = L.QPSDM_F32 if out.dtype == torch.float32 else L.QPSDM_BF16
+        check(L.lib().qpsdm_qbank(C.byref(self.bp), _ptr(traces), n, _ptr(out), code, _stream(stream)))
+        return out
+
+    # -- kernel (b) --------------------------------------------------------
+    def new_image(self):
+        c = self.cfg
+        return torch.zeros((c.ny, c.nx, c.nz), dtype=torch.float32, device="cuda")
+
+    def image_shot(self, img, geom, tr0, ntr, bank, tab, launch=None, stream=None):
+        bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
+        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else L.QPSDM_F16B
+        lp = None
+        if launch is not None:
+            lp = L.Launch(*launch)
+        check(L.lib().qpsdm_image_shot(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img),
+                                       _ptr(geom), tr0, ntr, _ptr(bank), bcode, _ptr(tab), tcode,
+                                       _stream(stream)))
+
+    def count_shot(self, geom, tr0, ntr, counter=None, stream=None):
+        own = counter is None
+        if own:
+            counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+        check(L.lib().qpsdm_count_shot(C.byref(self.ip), _ptr(geom), tr0, ntr, _ptr(counter),
+                                       _stream(stream)))
+        return int(counter.item()) if own else counter
+
+    def scale_image(self, img, s, stream=None):
+        check(L.lib().qpsdm_image_scale(_ptr(img), img.numel(), s, _stream(stream)))
+
+
+def tstar_max_from_tables(tab):
+    """Largest two-leg t* the tables allow (2 x the largest table t*)."""
+    return 2.0 * float(tab[..., 1].max())

@@ -1,0 +1,116 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from q_psdm_amd import engine, synth
+from tests.helpers import o_bparams, o_iparams, o_model, o_tgeom, o_window, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _survey(cfg):
+    probe = engine.Survey(cfg, 0.01)
+    tab = probe.build_tables()
+    return engine.Survey(cfg, engine.tstar_max_from_tables(tab)), tab
+
+
+@pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
+def test_tables_match_oracle(gpu, mk):
+    cfg = mk()
+    sv, tab = _survey(cfg)
+    ref = O.build_tables(o_model(cfg.model), o_tgeom(cfg.tgeom))
+    got = tab.cpu().numpy()
+    # FP64 quadrature on both sides, stored FP32: equal to a few ulp
+    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("nt,f1", [(1001, 1.0), (2001, 1.0), (1250, 1.0)])
+def test_trace_spectra_match_reference_preprocessing(gpu, nt, f1):
+    cfg = synth.config1()
+    cfg.nt = nt
+    cfg.dt = 0.004 if nt == 1250 else 0.002
+    sv = engine.Survey(cfg, 0.02)
+    rng = np.random.default_rng(7)
+    traces = rng.standard_normal((5, nt)).astype(np.float32)
+    re, im = sv.trace_spectra(torch.from_numpy(traces).cuda())
+    for i in range(5):
+        ore, oim = O.trace_spectra(traces[i], cfg.dt, o_window(sv.win))
+        scale = np.abs(np.concatenate([ore, oim])).max()
+        assert np.abs(re[i].cpu().numpy() - ore).max() < 2e-5 * scale
+        assert np.abs(im[i].cpu().numpy() - oim).max() < 2e-5 * scale
+
+
+@pytest.mark.parametrize("mk", [synth.tiny, synth.config1])
+def test_qbank_matches_fp64_oracle(gpu, mk):
+    cfg = mk()
+    sv, tab = _survey(cfg)
+    rng = np.random.default_rng(11)
+    traces = rng.standard_normal((6, cfg.nt)).astype(np.float32)
+    bank = sv.qbank(torch.from_numpy(traces).cuda()).cpu().numpy()
+    ref = O.qbank(traces, o_bparams(sv))
+    for i in range(traces.shape[0]):
+        for j in range(cfg.K):
+            assert rel_l2(bank[i, j], ref[i, j]) < 2e-5, (i, j)
+
+
+def test_qbank_bf16_rounds_fp32(gpu):
+    cfg = synth.config1()
+    sv, tab = _survey(cfg)
+    traces = torch.from_numpy(np.random.default_rng(3).standard_normal((4, cfg.nt)).astype(np.float32)).cuda()
+    b32 = sv.qbank(traces).float()
+    b16 = sv.qbank(traces, dtype=torch.bfloat16).float()
+    assert rel_l2(b16.cpu().numpy(), b32.cpu().numpy()) < 4e-3
+
+
+@pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
+def test_image_matches_oracle(gpu, mk):
+    cfg = mk()
+    sv, tab = _survey(cfg)
+    geom, ranges = synth.geometry(synth.shots_tiny(3, 8, extent=((cfg.nx - 1) * cfg.dx, (cfg.ny - 1) * cfg.dy)))
+    rng = np.random.default_rng(5)
+    traces = rng.standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
+    bank = sv.qbank(torch.from_numpy(traces).cuda())
+    g_d = torch.from_numpy(geom).cuda()
+    img = sv.new_image()
+    cnt = 0
+    for tr0, n in ranges:
+        sv.image_shot(img, g_d, tr0, n, bank, tab)
+        cnt += sv.count_shot(g_d, tr0, n)
+    b = bank.cpu().numpy()
+    t = tab.cpu().numpy()
+    ref32, n32 = O.image(b, geom, t, o_iparams(sv), fp32=True)
+    ref64, n64 = O.image(b.astype(np.float64), geom, t, o_iparams(sv))
+    got = img.cpu().numpy()
+    assert rel_l2(got, ref32) < 1e-5
+    assert rel_l2(got, ref64) < 1e-5
+    assert abs(cnt - n64) <= 1e-5 * n64 + 2
+
+
+def test_count_matches_oracle_config1_shot(gpu):
+    cfg = synth.config1()
+    sv, tab = _survey(cfg)
+    shots = synth.shots_config1()[:1]
+    geom, ranges = synth.geometry(shots)
+    geom = geom[::16]  # 64 of the 1024 receivers
+    g_d = torch.from_numpy(np.ascontiguousarray(geom)).cuda()
+    got = sv.count_shot(g_d, 0, geom.shape[0])
+    ref = O.count(geom, o_iparams(sv))
+    assert abs(got - ref) <= 1e-5 * ref + 2
+
+
+def test_end_to_end_config1_subset_vs_fp64(gpu):
+    """Traces -> GPU bank -> GPU image against FP64 bank -> FP64 image."""
+    cfg = synth.config1()
+    sv, tab = _survey(cfg)
+    shots = synth.shots_config1()[5:6]
+    geom, _ = synth.geometry(shots)
+    geom = np.ascontiguousarray(geom[::8])
+    traces = synth.diffraction_traces(cfg, geom, synth.scatterers(cfg))
+    bank = sv.qbank(torch.from_numpy(traces).cuda())
+    img = sv.new_image()
+    sv.image_shot(img, torch.from_numpy(geom).cuda(), 0, geom.shape[0], bank, tab)
+    ob = O.qbank(traces, o_bparams(sv))
+    ref, _ = O.image(ob, geom, tab.cpu().numpy(), o_iparams(sv))
+    assert rel_l2(img.cpu().numpy(), ref) < 1e-5
