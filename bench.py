@@ -39,7 +39,9 @@ def main():
     ap.add_argument("--form", choices=["fp32", "mixed"], default="fp32")
     ap.add_argument("--ztile", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-traces", type=int, default=16)
+    ap.add_argument("--cpu-traces", type=int, default=256)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="device copy of this many MiB before the steps (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
@@ -77,6 +79,14 @@ def main():
     torch.cuda.synchronize()
     log(f"updates per shot: {counts}")
 
+    if args.calib_copy_mib:
+        a = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        c = torch.empty_like(a)
+        torch.cuda.synchronize()
+        c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
     img = sv.new_image()
     bank = torch.empty((max(n for _, n in ranges), cfg.K, cfg.nt), dtype=bank_dtype, device="cuda")
     launch = (8, 8, args.ztile, 1)

@@ -44,6 +44,10 @@ typedef struct {
 
 int qpsdm_window_params(int ntl, float dt, float f1, float f3, float f4, qpsdm_window *out);
 
+/* Band sanity of main() (src/q-psdm.cu:711-723): f3 <= Nyquist - 10 Hz,
+ * f4 <= Nyquist - 5 Hz, and f3 <= f4 - 5 Hz. */
+int qpsdm_clamp_band(float dt, float *f3, float *f4);
+
 /* Reference FFT length choice: npfar (src/fft.cc:412-439) via
  * get_opt_n_fftw (src/fft.cc:3721). */
 int qpsdm_npfar(int nmin);
@@ -188,15 +192,19 @@ typedef struct {
   int trace_chunk;            /* traces evaluated together per depth point            */
 } qpsdm_launch;
 
-/* One shot: traces [tr0, tr0+ntr) of geom/bank, source (sx, sy); image
- * float [ny][nx][nz] accumulated in place.  table_dtype QPSDM_F32 or
- * QPSDM_F16B; bank_dtype QPSDM_F32 or QPSDM_BF16. */
+/* A run of traces: geom_dev holds (sx, sy, gx, gy) per trace, image-relative,
+ * and the run is geom rows [tr0, tr0+ntr); bank_dev holds the run's ntr
+ * filtered traces [ntr][K][nt] (its first row is trace tr0).  Consecutive
+ * traces with the same source share one source-leg evaluation (a shot
+ * gather pays it once).  A trace whose sx is NaN is skipped.  Image float
+ * [ny][nx][nz] accumulated in place.  table_dtype QPSDM_F32 or QPSDM_F16B;
+ * bank_dtype QPSDM_F32 or QPSDM_BF16. */
 int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                      const float *geom_dev, int tr0, int ntr, const void *bank_dev,
                      int bank_dtype, const void *tab_dev, int table_dtype, void *stream);
 
-/* Update count of one shot (pairs inside the dip aperture), added to
- * *count_dev (uint64). */
+/* Update count of a run (pairs (trace, point) inside the dip aperture, z > 0),
+ * added to *count_dev (uint64). */
 int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom_dev, int tr0, int ntr,
                      unsigned long long *count_dev, void *stream);
 

@@ -505,10 +505,101 @@ int64_t or_image_f64(const double *bank, const float *geom, int ntr, const float
   IMAGE_BODY(double, sqrt)
 }
 
+
+/* FP32 CPU path of the sum (the timed CPU baseline and the FP32 parity
+ * partner of the GPU form): the same formula evaluated in float with the
+ * GPU's order of operations -- reciprocal spacings, bilinear (x, y) table
+ * interpolation at two depth nodes then a lerp in z, 1/sqrt distances,
+ * reciprocal weights. */
+static inline void lookup_f32(const or_tgeom *g, const float *tab, float px, float py, float x,
+                              float y, float z, float *t, float *ts) {
+  const float inx = 1.0f / g->ndx, iny = 1.0f / g->ndy, idu = 1.0f / g->du, idv = 1.0f / g->dv,
+              idw = 1.0f / g->dw;
+  int in = (int)floorf((px - g->n0x) * inx + 0.5f);
+  int jn = (int)floorf((py - g->n0y) * iny + 0.5f);
+  in = in < 0 ? 0 : (in > g->nnx - 1 ? g->nnx - 1 : in);
+  jn = jn < 0 ? 0 : (jn > g->nny - 1 ? g->nny - 1 : jn);
+  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+  float q = fminf(fmaxf((x - px - g->u0) * idu, 0.0f), (float)(g->nu - 1));
+  int iu = (int)q < g->nu - 2 ? (int)q : g->nu - 2;
+  const float a = q - (float)iu;
+  q = fminf(fmaxf((y - py - g->v0) * idv, 0.0f), (float)(g->nv - 1));
+  int iv = (int)q < g->nv - 2 ? (int)q : g->nv - 2;
+  const float b = q - (float)iv;
+  const float fw = fminf(z * idw, (float)(g->nw - 1));
+  int w = (int)fw < g->nw - 2 ? (int)fw : g->nw - 2;
+  const float c = fw - (float)w;
+  float col[2][2];
+  for (int k = 0; k < 2; ++k) {
+#define TV(dv_, du_, comp_) T[2 * (((size_t)(iv + dv_) * g->nu + (iu + du_)) * g->nw + (w + k)) + comp_]
+    for (int comp = 0; comp < 2; ++comp) {
+      const float lx = TV(0, 0, comp) + a * (TV(0, 1, comp) - TV(0, 0, comp));
+      const float hx = TV(1, 0, comp) + a * (TV(1, 1, comp) - TV(1, 0, comp));
+      col[k][comp] = lx + b * (hx - lx);
+    }
+#undef TV
+  }
+  *t = col[0][0] + c * (col[1][0] - col[0][0]);
+  *ts = col[0][1] + c * (col[1][1] - col[0][1]);
+}
+
 int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float *tab,
                      const or_iparams *p, float *img) {
-  typedef float BANK;
-  IMAGE_BODY(float, sqrtf)
+  int64_t count = 0;
+  const or_tgeom *g = &p->tg;
+  const float tz = p->taperzone;
+  const float tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;
+  const float tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);
+  const float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
+  const float rx1 = 1.0f / (tx1 - tx1c), rx2 = 1.0f / (tx2c - tx2);
+  const float ry1 = 1.0f / (ty1 - ty1c), ry2 = 1.0f / (ty2c - ty2);
+  const float inv_dt = 1.0f / p->dt, inv_dts = 1.0f / p->dtstar;
+#pragma omp parallel for schedule(dynamic) reduction(+ : count)
+  for (int iy = 0; iy < p->ny; ++iy) {
+    for (int itr = 0; itr < ntr; ++itr) {
+      const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
+      const float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
+      const float *bt = bank + (size_t)itr * p->K * p->nt;
+      const float y = (float)iy * p->dy;
+      for (int ix = 0; ix < p->nx; ++ix) {
+        const float x = (float)ix * p->dx;
+        const float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
+        const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
+        for (int iz = 1; iz < p->nz; ++iz) {
+          const float z = (float)iz * p->dz;
+          const float d2 = hs2 + z * z;
+          const float rds = 1.0f / sqrtf(d2), rdg = 1.0f / sqrtf(hg2 + z * z);
+          const float X = dsx * rds + dgx * rdg, Y = dsy * rds + dgy * rdg, Z = z * (rds + rdg);
+          const float rz = 1.0f / Z;
+          const float tanx = X * rz, tany = Y * rz;
+          if (tanx <= tx1c || tx2c <= tanx || tany <= ty1c || ty2c <= tany) continue;
+          ++count;
+          float wt = 1.0f;
+          if (tanx < tx1) wt *= (tanx - tx1c) * rx1;
+          if (tanx > tx2) wt *= (tx2c - tanx) * rx2;
+          if (tany < ty1) wt *= (tany - ty1c) * ry1;
+          if (tany > ty2) wt *= (ty2c - tany) * ry2;
+          float ts, tss, tg, tsg;
+          lookup_f32(g, tab, sx, sy, x, y, z, &ts, &tss);
+          lookup_f32(g, tab, gx, gy, x, y, z, &tg, &tsg);
+          ts = fmaxf(ts, p->tmin);
+          tg = fmaxf(tg, p->tmin);
+          int j = (int)((tss + tsg) * inv_dts + 0.5f);
+          j = j < 0 ? 0 : (j > p->K - 1 ? p->K - 1 : j);
+          const float f = (ts + tg) * inv_dt;
+          const int i0 = (int)f;
+          if (!(i0 > 0 && i0 + 1 < p->nt)) continue;
+          const float fr = f - (float)i0;
+          const float *b = bt + (size_t)j * p->nt + i0;
+          const float ds = d2 * rds;
+          const float rr = ds * rdg;
+          const float tr = rr * rr * rr * tg * (1.0f / ts);
+          img[((size_t)iy * p->nx + ix) * p->nz + iz] += wt * (tr + 1.0f / tr) * (b[0] + fr * (b[1] - b[0]));
+        }
+      }
+    }
+  }
+  return count;
 }
 
 /* Update count alone: pairs (trace, point) with z > 0 inside the widened

@@ -76,6 +76,15 @@ extern "C" int qpsdm_window_params(int ntl, float dt, float f1, float f3, float 
   return QPSDM_OK;
 }
 
+// main()'s band sanity (src/q-psdm.cu:711-723).
+extern "C" int qpsdm_clamp_band(float dt, float *f3, float *f4) {
+  QPSDM_REQUIRE(dt > 0 && f3 && f4, "clamp_band: bad args");
+  if (*f3 > 0.5 / dt - 10.0) *f3 = 0.5 / dt - 10.0;
+  if (*f4 > 0.5 / dt - 5.0) *f4 = 0.5 / dt - 5.0;
+  if (*f4 - *f3 <= 5.0) *f3 = *f4 - 5.0;
+  return QPSDM_OK;
+}
+
 // lnG and c1..c4 of the gain limiter (src/q-psdm.cu:1277-1283).
 extern "C" int qpsdm_gain_coefs(float G, float out[5]) {
   QPSDM_REQUIRE(out && G > 1, "gain: threshold must exceed 1");

@@ -2,8 +2,9 @@
 //
 // Output-stationary: a one-wave work-group owns an image tile of 8 x 8
 // columns x ZP depth points (one column per lane, ZP accumulators in
-// registers) and streams the traces of one shot.  Per shot the source leg
-// (t, t*) of every tile point is interpolated once into registers.  Per
+// registers) and streams a run of traces.  The source leg (t, t*) of every
+// tile point is interpolated into registers and reused while consecutive
+// traces share their source (once per shot gather).  Per
 // trace and lane the receiver table column is interpolated bilinearly in
 // (x, y), walked down the table depths the tile spans (two nodes live in
 // registers) and lerped in z; each point then applies the dip-bound taper and
@@ -47,43 +48,49 @@ __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float 
 #pragma unroll
   for (int i = 0; i < ZP; ++i) acc[i] = (valid && i < nzv) ? img[col + i] : 0.f;
 
-  // source leg of the shot, once per tile
-  const float4 g0 = geom[P.tr0];
-  const float sx = g0.x, sy = g0.y;
-  const float dsx = sx - x, dsy = sy - y;
-  const float hs2 = dsx * dsx + dsy * dsy;
+  // Source leg (t, t*, 1/t, 1/ds, ds) of every tile point: computed for the
+  // first trace and again only when the source moves (a shot gather computes
+  // it once; a common-offset gather per trace).  A trace whose geometry is
+  // NaN (rejected outside the model extent) is skipped.
   float ts[ZP], tss[ZP], rts[ZP], rds[ZP], ds[ZP];
-  {
-    const qk::Column c = qk::column_setup(P, sx, sy, x, y);
-    float fr;
-    int cur = qk::depth_node(P, (float)z0 * P.dz, &fr);
-    float2 lo = qk::column_at(tab, c, cur), hi = qk::column_at(tab, c, cur + 1);
-#pragma unroll
-    for (int i = 0; i < ZP; ++i) {
-      const float z = (float)(z0 + i) * P.dz;
-      const int w = qk::depth_node(P, z, &fr);
-      while (cur < w) {
-        ++cur;
-        lo = hi;
-        hi = qk::column_at(tab, c, cur + 1);
-      }
-      ts[i] = fmaxf(lo.x + fr * (hi.x - lo.x), P.tmin);
-      tss[i] = lo.y + fr * (hi.y - lo.y);
-      rts[i] = __builtin_amdgcn_rcpf(ts[i]);
-      const float d2 = hs2 + z * z;
-      rds[i] = __builtin_amdgcn_rsqf(d2);
-      ds[i] = d2 * rds[i];
-    }
-  }
+  float sx = NAN, sy = NAN, dsx = 0.f, dsy = 0.f;
   const qk::TileBox box = qk::tile_box(P, tx0, ty0, z0, ZP);
 
   for (int t = P.tr0; t < P.tr0 + P.ntr; ++t) {
     const float4 g = geom[t];
-    if (qk::tile_culled(P, box, sx, sy, g.z, g.w)) continue;
+    if (!(g.x == g.x)) continue;
+    if (qk::tile_culled(P, box, g.x, g.y, g.z, g.w)) continue;
+    if (g.x != sx || g.y != sy) {
+      sx = g.x;
+      sy = g.y;
+      dsx = sx - x;
+      dsy = sy - y;
+      const float hs2 = dsx * dsx + dsy * dsy;
+      const qk::Column c = qk::column_setup(P, sx, sy, x, y);
+      float fr;
+      int cur = qk::depth_node(P, (float)z0 * P.dz, &fr);
+      float2 lo = qk::column_at(tab, c, cur), hi = qk::column_at(tab, c, cur + 1);
+#pragma unroll
+      for (int i = 0; i < ZP; ++i) {
+        const float z = (float)(z0 + i) * P.dz;
+        const int w = qk::depth_node(P, z, &fr);
+        while (cur < w) {
+          ++cur;
+          lo = hi;
+          hi = qk::column_at(tab, c, cur + 1);
+        }
+        ts[i] = fmaxf(lo.x + fr * (hi.x - lo.x), P.tmin);
+        tss[i] = lo.y + fr * (hi.y - lo.y);
+        rts[i] = __builtin_amdgcn_rcpf(ts[i]);
+        const float d2 = hs2 + z * z;
+        rds[i] = __builtin_amdgcn_rsqf(d2);
+        ds[i] = d2 * rds[i];
+      }
+    }
     const float dgx = g.z - x, dgy = g.w - y;
     const float hg2 = dgx * dgx + dgy * dgy;
     const qk::Column c = qk::column_setup(P, g.z, g.w, x, y);
-    const BankT *bt = bank + (size_t)t * P.K * P.nt;
+    const BankT *bt = bank + (size_t)(t - P.tr0) * P.K * P.nt;
     float fr;
     int cur = qk::depth_node(P, (float)z0 * P.dz, &fr);
     float2 lo = qk::column_at(tab, c, cur), hi = qk::column_at(tab, c, cur + 1);
@@ -141,12 +148,12 @@ __global__ __launch_bounds__(64) void count_kernel(qk::ImgDev P, const float4 *_
   const bool valid = cx < P.nx && cy < P.ny;
   const float x = (float)cx * P.dx, y = (float)cy * P.dy;
   const qk::TileBox box = qk::tile_box(P, tx0, ty0, z0, ZP);
-  const float4 g0 = geom[P.tr0];
-  const float dsx = g0.x - x, dsy = g0.y - y, hs2 = dsx * dsx + dsy * dsy;
   unsigned int n = 0;
   for (int t = P.tr0; t < P.tr0 + P.ntr; ++t) {
     const float4 g = geom[t];
-    if (qk::tile_culled(P, box, g0.x, g0.y, g.z, g.w)) continue;
+    if (!(g.x == g.x)) continue;
+    if (qk::tile_culled(P, box, g.x, g.y, g.z, g.w)) continue;
+    const float dsx = g.x - x, dsy = g.y - y, hs2 = dsx * dsx + dsy * dsy;
     const float dgx = g.z - x, dgy = g.w - y, hg2 = dgx * dgx + dgy * dgy;
 #pragma unroll
     for (int i = 0; i < ZP; ++i) {

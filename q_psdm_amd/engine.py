@@ -67,12 +67,19 @@ def tgeom_struct(g):
     return s
 
 
+def clamp_band(dt, f3, f4):
+    a, b = C.c_float(f3), C.c_float(f4)
+    check(L.lib().qpsdm_clamp_band(dt, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
 class Survey:
     """Per-offset-class state of the imaging path for one configuration."""
 
     def __init__(self, cfg, tstar_max):
         self.cfg = cfg
-        self.win = window(cfg.nt, cfg.dt, cfg.f1, cfg.f3, cfg.f4)
+        f3, f4 = clamp_band(cfg.dt, cfg.f3, cfg.f4)
+        self.win = window(cfg.nt, cfg.dt, cfg.f1, f3, f4)
         self.tstar_max = float(tstar_max)
         self.dtstar = self.tstar_max / max(cfg.K - 1, 1)
         bp = L.BankParams()
@@ -116,9 +123,10 @@ class Survey:
 
     def qbank(self, traces, dtype=torch.float32, out=None, stream=None):
         n = traces.shape[0]
-        assert traces.dtype == torch.float32 and traces.shape[1] == self.cfg.nt
+        assert traces.dtype == torch.float32 and traces.dim() == 2 and traces.shape[1] == self.cfg.nt
         if out is None:
             out = torch.empty((n, self.cfg.K, self.cfg.nt), dtype=dtype, device="cuda")
+        assert out.shape == (n, self.cfg.K, self.cfg.nt) and out.is_contiguous()
         code = L.QPSDM_F32 if out.dtype == torch.float32 else L.QPSDM_BF16
         check(L.lib().qpsdm_qbank(C.byref(self.bp), _ptr(traces), n, _ptr(out), code, _stream(stream)))
         return out
@@ -129,6 +137,16 @@ class Survey:
         return torch.zeros((c.ny, c.nx, c.nz), dtype=torch.float32, device="cuda")
 
     def image_shot(self, img, geom, tr0, ntr, bank, tab, launch=None, stream=None):
+        """Accumulate the run geom[tr0:tr0+ntr] into img; bank[0:ntr] are the
+        run's filtered traces."""
+        c = self.cfg
+        assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
+        assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
+        assert bank.dim() == 3 and bank.shape[0] >= ntr and bank.shape[1:] == (c.K, c.nt)
+        if tab.dtype == torch.float32:
+            assert tuple(tab.shape) == self.table_shape()
+        else:
+            assert tab.numel() == L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
         bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
         tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else L.QPSDM_F16B
         lp = None
@@ -139,6 +157,7 @@ class Survey:
                                        _stream(stream)))
 
     def count_shot(self, geom, tr0, ntr, counter=None, stream=None):
+        assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
         own = counter is None
         if own:
             counter = torch.zeros(1, dtype=torch.int64, device="cuda")

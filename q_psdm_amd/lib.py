@@ -83,7 +83,7 @@ class CoorT(C.Structure):
 
 # Every symbol the headers in include/ declare with C linkage.
 EXPORTS = [
-    "qpsdm_window_params", "qpsdm_npfar", "qpsdm_gain_coefs", "qpsdm_iaztab",
+    "qpsdm_window_params", "qpsdm_clamp_band", "qpsdm_npfar", "qpsdm_gain_coefs", "qpsdm_iaztab",
     "qpsdm_trace_geometry", "qpsdm_tables_build", "qpsdm_tables_f16b_bytes",
     "qpsdm_tables_quantize", "qpsdm_tables_blend", "qpsdm_read_tidx", "qpsdm_write_tidx",
     "qpsdm_load_line_table", "qpsdm_trace_spectra", "qpsdm_qbank", "qpsdm_image_shot",
@@ -118,6 +118,7 @@ def lib():
         L.qpsdm_window_params.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                           C.POINTER(Window)]
         L.qpsdm_npfar.argtypes = [C.c_int]
+        L.qpsdm_clamp_band.argtypes = [C.c_float, fp, fp]
         L.qpsdm_gain_coefs.argtypes = [C.c_float, fp]
         L.qpsdm_iaztab.argtypes = [ip, C.c_int]
         L.qpsdm_trace_geometry.argtypes = [C.POINTER(GeomParams), vp, C.c_int, vp, vp, vp]
