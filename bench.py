@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--form", choices=["fp32", "mixed"], default="fp32")
-    ap.add_argument("--ztile", type=int, default=8)
+    ap.add_argument("--ztile", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-traces", type=int, default=256)
     ap.add_argument("--calib-copy-mib", type=int, default=0,

@@ -32,7 +32,8 @@ extern "C" {
 /* Bank / table storage types. */
 #define QPSDM_F32 0
 #define QPSDM_BF16 1 /* bank only                         */
-#define QPSDM_F16B 2 /* tables: FP16 offset+scale per 8^3 brick */
+#define QPSDM_F16B 2 /* tables: FP16 codes, offset+scale per 8^3 brick            */
+#define QPSDM_U16B 3 /* tables: unsigned 16-bit codes, offset+scale per 8^3 brick */
 
 /* Two-window spectral set-up of one offset class; replaces the window block
  * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
@@ -108,12 +109,15 @@ typedef struct {
  * generator): straight-ray t and t* into the FP32 layout, on the device. */
 int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab_dev, void *stream);
 
-/* Size in bytes of the FP16-brick form of a table set. */
+/* Size in bytes of the 16-bit brick form of a table set (QPSDM_F16B and
+ * QPSDM_U16B have the same size). */
 size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g);
 
-/* Quantise an FP32 table set into the FP16-brick form (offset+scale per
- * 8^3 brick). */
-int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, void *stream);
+/* Quantise an FP32 table set into 16-bit bricks: per 8^3 brick of (t, t*)
+ * an offset and a scale per quantity, codes h = (v - off)/scale in [0, 1]
+ * stored as FP16 (QPSDM_F16B) or as round(h * 65535) (QPSDM_U16B). */
+int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, int code,
+                          void *stream);
 
 /* Line interpolation of two table files, t = t1 (1-c) + t2 c, as
  * get_ttt_all does on the host (src/q-psdm.cu:2582-2594); on the device. */

@@ -570,15 +570,17 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
           const float d2 = hs2 + z * z;
           const float rds = 1.0f / sqrtf(d2), rdg = 1.0f / sqrtf(hg2 + z * z);
           const float X = dsx * rds + dgx * rdg, Y = dsy * rds + dgy * rdg, Z = z * (rds + rdg);
-          const float rz = 1.0f / Z;
-          const float tanx = X * rz, tany = Y * rz;
-          if (tanx <= tx1c || tx2c <= tanx || tany <= ty1c || ty2c <= tany) continue;
+          if (X <= tx1c * Z || tx2c * Z <= X || Y <= ty1c * Z || ty2c * Z <= Y) continue;
           ++count;
           float wt = 1.0f;
-          if (tanx < tx1) wt *= (tanx - tx1c) * rx1;
-          if (tanx > tx2) wt *= (tx2c - tanx) * rx2;
-          if (tany < ty1) wt *= (tany - ty1c) * ry1;
-          if (tany > ty2) wt *= (ty2c - tany) * ry2;
+          if (X < tx1 * Z || X > tx2 * Z || Y < ty1 * Z || Y > ty2 * Z) {
+            const float rz = 1.0f / Z;
+            const float tanx = X * rz, tany = Y * rz;
+            if (tanx < tx1) wt *= (tanx - tx1c) * rx1;
+            if (tanx > tx2) wt *= (tx2c - tanx) * rx2;
+            if (tany < ty1) wt *= (tany - ty1c) * ry1;
+            if (tany > ty2) wt *= (ty2c - tany) * ry2;
+          }
           float ts, tss, tg, tsg;
           lookup_f32(g, tab, sx, sy, x, y, z, &ts, &tss);
           lookup_f32(g, tab, gx, gy, x, y, z, &tg, &tsg);
@@ -603,7 +605,7 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
 }
 
 /* Update count alone: pairs (trace, point) with z > 0 inside the widened
- * dip bounds, evaluated in FP32 as the imaging sum does. */
+ * dip bounds, evaluated in FP32 with the FP32 path's order of operations. */
 int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
   int64_t count = 0;
   float tz = p->taperzone;
@@ -621,11 +623,10 @@ int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
         float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
         for (int iz = 1; iz < p->nz; ++iz) {
           float z = (float)iz * p->dz;
-          float ds = sqrtf(dsx * dsx + dsy * dsy + z * z);
-          float dg = sqrtf(dgx * dgx + dgy * dgy + z * z);
-          float X = dsx / ds + dgx / dg, Y = dsy / ds + dgy / dg, Z = z / ds + z / dg;
-          float tanx = X / Z, tany = Y / Z;
-          if (tanx <= tx1c || tx2c <= tanx || tany <= ty1c || ty2c <= tany) continue;
+          const float rds = 1.0f / sqrtf(dsx * dsx + dsy * dsy + z * z);
+          const float rdg = 1.0f / sqrtf(dgx * dgx + dgy * dgy + z * z);
+          const float X = dsx * rds + dgx * rdg, Y = dsy * rds + dgy * rdg, Z = z * (rds + rdg);
+          if (X <= tx1c * Z || tx2c * Z <= X || Y <= ty1c * Z || ty2c * Z <= Y) continue;
           ++count;
         }
       }

@@ -44,17 +44,28 @@ __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float 
   const float x = (float)cx * P.dx, y = (float)cy * P.dy;
   const size_t col = ((size_t)(valid ? cy : 0) * P.nx + (valid ? cx : 0)) * P.nz + z0;
 
+  // per depth point (wave-uniform): z, z^2, table depth node and weight
+  float zz[ZP], z2[ZP], fz[ZP];
+  int wz[ZP];
+#pragma unroll
+  for (int i = 0; i < ZP; ++i) {
+    zz[i] = (float)(z0 + i) * P.dz;
+    z2[i] = zz[i] * zz[i];
+    wz[i] = qk::depth_node(P, zz[i], &fz[i]);
+  }
+
   float acc[ZP];
 #pragma unroll
   for (int i = 0; i < ZP; ++i) acc[i] = (valid && i < nzv) ? img[col + i] : 0.f;
 
   // Source leg (t, t*, 1/t, 1/ds, ds) of every tile point: computed for the
-  // first trace and again only when the source moves (a shot gather computes
-  // it once; a common-offset gather per trace).  A trace whose geometry is
-  // NaN (rejected outside the model extent) is skipped.
+  // first trace and again only when the source moves (a shot gather pays it
+  // once; a common-offset gather per trace).  A trace whose geometry is NaN
+  // (rejected outside the model extent) is skipped.
   float ts[ZP], tss[ZP], rts[ZP], rds[ZP], ds[ZP];
   float sx = NAN, sy = NAN, dsx = 0.f, dsy = 0.f;
   const qk::TileBox box = qk::tile_box(P, tx0, ty0, z0, ZP);
+  const unsigned bank_bytes = (unsigned)(P.K * P.nt * sizeof(BankT));
 
   for (int t = P.tr0; t < P.tr0 + P.ntr; ++t) {
     const float4 g = geom[t];
@@ -66,64 +77,57 @@ __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float 
       dsx = sx - x;
       dsy = sy - y;
       const float hs2 = dsx * dsx + dsy * dsy;
-      const qk::Column c = qk::column_setup(P, sx, sy, x, y);
-      float fr;
-      int cur = qk::depth_node(P, (float)z0 * P.dz, &fr);
-      float2 lo = qk::column_at(tab, c, cur), hi = qk::column_at(tab, c, cur + 1);
+      const auto c = qk::column_setup(P, tab, sx, sy, x, y);
+      int cur = wz[0];
+      float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
 #pragma unroll
       for (int i = 0; i < ZP; ++i) {
-        const float z = (float)(z0 + i) * P.dz;
-        const int w = qk::depth_node(P, z, &fr);
-        while (cur < w) {
+        while (cur < wz[i]) {
           ++cur;
           lo = hi;
-          hi = qk::column_at(tab, c, cur + 1);
+          hi = tab.at(c, cur + 1);
         }
-        ts[i] = fmaxf(lo.x + fr * (hi.x - lo.x), P.tmin);
-        tss[i] = lo.y + fr * (hi.y - lo.y);
+        ts[i] = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
+        tss[i] = lo.y + fz[i] * (hi.y - lo.y);
         rts[i] = __builtin_amdgcn_rcpf(ts[i]);
-        const float d2 = hs2 + z * z;
+        const float d2 = hs2 + z2[i];
         rds[i] = __builtin_amdgcn_rsqf(d2);
         ds[i] = d2 * rds[i];
       }
     }
     const float dgx = g.z - x, dgy = g.w - y;
     const float hg2 = dgx * dgx + dgy * dgy;
-    const qk::Column c = qk::column_setup(P, g.z, g.w, x, y);
-    const BankT *bt = bank + (size_t)(t - P.tr0) * P.K * P.nt;
-    float fr;
-    int cur = qk::depth_node(P, (float)z0 * P.dz, &fr);
-    float2 lo = qk::column_at(tab, c, cur), hi = qk::column_at(tab, c, cur + 1);
+    const auto c = qk::column_setup(P, tab, g.z, g.w, x, y);
+    const __amdgpu_buffer_rsrc_t br =
+        qk::make_rsrc(bank + (size_t)(t - P.tr0) * P.K * P.nt, bank_bytes);
+    int cur = wz[0];
+    float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
 #pragma unroll
     for (int i = 0; i < ZP; ++i) {
       if (i >= nzv) continue;
-      const int iz = z0 + i;
-      const float z = (float)iz * P.dz;
-      const int w = qk::depth_node(P, z, &fr);
-      while (cur < w) {
+      while (cur < wz[i]) {
         ++cur;
         lo = hi;
-        hi = qk::column_at(tab, c, cur + 1);
+        hi = tab.at(c, cur + 1);
       }
-      if (!valid || iz == 0) continue;
-      const float rdg = __builtin_amdgcn_rsqf(hg2 + z * z);
+      if (!valid || z0 + i == 0) continue;
+      const float rdg = __builtin_amdgcn_rsqf(hg2 + z2[i]);
       float wt;
-      if (!qk::dip_weight(P, dsx * rds[i] + dgx * rdg, dsy * rds[i] + dgy * rdg, z * (rds[i] + rdg),
-                          &wt))
+      if (!qk::dip_weight(P, dsx * rds[i] + dgx * rdg, dsy * rds[i] + dgy * rdg,
+                          zz[i] * (rds[i] + rdg), &wt))
         continue;
-      const float tg = fmaxf(lo.x + fr * (hi.x - lo.x), P.tmin);
-      const float tsg = lo.y + fr * (hi.y - lo.y);
+      const float tg = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
+      const float tsg = lo.y + fz[i] * (hi.y - lo.y);
       int j = (int)((tss[i] + tsg) * P.inv_dtstar + 0.5f);
       j = j < 0 ? 0 : (j > P.K - 1 ? P.K - 1 : j);
       const float f = (ts[i] + tg) * P.inv_dt;
       const int i0 = (int)f;
       if (!(i0 > 0 && i0 + 1 < P.nt)) continue;
       const float frt = f - (float)i0;
-      const BankT *b = bt + j * P.nt + i0;
-      const float b0 = qk::ld(b), b1 = qk::ld(b + 1);
+      const float2 b = qk::bank_pair(br, j * P.nt + i0, bank);
       const float rr = ds[i] * rdg;
       const float tr = rr * rr * rr * tg * rts[i];
-      acc[i] += wt * (tr + __builtin_amdgcn_rcpf(tr)) * (b0 + frt * (b1 - b0));
+      acc[i] += wt * (tr + __builtin_amdgcn_rcpf(tr)) * (b.x + frt * (b.y - b.x));
     }
   }
   if (valid) {
@@ -220,7 +224,13 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
     if (bank_dtype == QPSDM_BF16)
       return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
   } else if (table_dtype == QPSDM_F16B) {
-    qk::TabF16B T = qk::make_tab_f16b(p->tg, tab);
+    auto T = qk::make_tab_b16<false>(p->tg, tab);
+    if (bank_dtype == QPSDM_F32)
+      return launch_tz(P, T, img, g4, (const float *)bank, s);
+    if (bank_dtype == QPSDM_BF16)
+      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
+  } else if (table_dtype == QPSDM_U16B) {
+    auto T = qk::make_tab_b16<true>(p->tg, tab);
     if (bank_dtype == QPSDM_F32)
       return launch_tz(P, T, img, g4, (const float *)bank, s);
     if (bank_dtype == QPSDM_BF16)

@@ -43,6 +43,9 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   QPSDM_REQUIRE(g.nnx >= 1 && g.nny >= 1 && g.nu >= 2 && g.nv >= 2 && g.nw >= 2,
                 "image: table dims must be >= 2 (nodes >= 1)");
   QPSDM_REQUIRE(g.du > 0 && g.dv > 0 && g.dw > 0 && g.ndx > 0 && g.ndy > 0, "image: bad table spacing");
+  QPSDM_REQUIRE((size_t)g.nv * g.nu * g.nw * 8 < (1ull << 31),
+                "image: one node's table must stay below 2 GiB (32-bit buffer offsets)");
+  QPSDM_REQUIRE((size_t)p->K * p->nt * 4 < (1ull << 31), "image: one trace's bank must stay below 2 GiB");
   o->nx = p->nx;
   o->ny = p->ny;
   o->nz = p->nz;
@@ -101,12 +104,46 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   return QPSDM_OK;
 }
 
+
+// Buffer resource (base, size) for raw buffer loads: per-lane 32-bit byte
+// offsets plus a wave-uniform scalar offset; offsets past `bytes` read 0.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ inline float2 ld_f2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// FP32 tables: float2 (t, t*) [node][nv][nu][nw].
 struct TabF32 {
   const float2 *p;
-  size_t nstride;
+  size_t nstride;  // float2 per node
   int nu, nw;
-  __device__ inline float2 fetch(int node, int iv, int iu, int iw) const {
-    return p[(size_t)node * nstride + ((size_t)iv * nu + iu) * nw + iw];
+  // The four corner columns of one image column under one surface position.
+  struct Col {
+    __amdgpu_buffer_rsrc_t r;
+    int o00, o01, o10, o11;  // byte offsets of the corner columns
+    float a, b;
+  };
+  __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
+    Col c;
+    c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)(nstride * sizeof(float2)));
+    c.o00 = ((iv * nu + iu) * nw) * 8;
+    c.o01 = c.o00 + nw * 8;
+    c.o10 = c.o00 + nu * nw * 8;
+    c.o11 = c.o10 + nw * 8;
+    c.a = a;
+    c.b = b;
+    return c;
+  }
+  __device__ inline float2 at(const Col &c, int w) const {
+    const int so = w * 8;
+    const float2 v00 = ld_f2(c.r, c.o00, so), v01 = ld_f2(c.r, c.o01, so);
+    const float2 v10 = ld_f2(c.r, c.o10, so), v11 = ld_f2(c.r, c.o11, so);
+    const float lx = v00.x + c.a * (v01.x - v00.x), ly = v00.y + c.a * (v01.y - v00.y);
+    const float hx = v10.x + c.a * (v11.x - v10.x), hy = v10.y + c.a * (v11.y - v10.y);
+    return make_float2(lx + c.b * (hx - lx), ly + c.b * (hy - ly));
   }
 };
 
@@ -119,45 +156,71 @@ inline TabF32 make_tab_f32(const qpsdm_tgeom &g, const void *p) {
   return t;
 }
 
-// FP16 bricks: per node [bv][bu][bw] bricks of 8x8x8; each brick a float4
-// header {t_off, t_scale, s_off, s_scale} then 512 half2 (t, t*) [v][u][w].
+// 16-bit bricks: per node [bv][bu][bw] bricks of 8x8x8; each brick a float4
+// header {t_off, t_scale, s_off, s_scale} then 512 (t, t*) code pairs
+// [v][u][w], value = off + scale * h with h in [0, 1] stored as __half
+// (QPSDM_F16B) or as unsigned 16-bit fixed point h = code / 65535
+// (QPSDM_U16B).
 constexpr int kBrickBytes = 16 + 512 * 4;
 
-struct TabF16B {
+template <bool UNORM>
+struct TabB16 {
   const unsigned char *p;
   size_t nstride;  // bytes per node
   int nbu, nbw;
-  __device__ inline float2 fetch(int node, int iv, int iu, int iw) const {
-    const size_t brick = ((size_t)(iv >> 3) * nbu + (iu >> 3)) * nbw + (iw >> 3);
-    const unsigned char *b = p + (size_t)node * nstride + brick * kBrickBytes;
-    const float4 h = *(const float4 *)b;
-    const int inner = (((iv & 7) << 3) + (iu & 7)) * 8 + (iw & 7);
-    const __half2 v = *(const __half2 *)(b + 16 + inner * 4);
-    const float2 f = __half22float2(v);
+  struct Col {
+    __amdgpu_buffer_rsrc_t r;
+    int hdr[4], val[4];  // per corner: header and code offsets in depth brick 0
+    float a, b;
+  };
+  __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
+    Col c;
+    c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)nstride);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int v = iv + (k >> 1), u = iu + (k & 1);
+      c.hdr[k] = ((v >> 3) * nbu + (u >> 3)) * nbw * kBrickBytes;
+      c.val[k] = c.hdr[k] + 16 + (((v & 7) << 3) + (u & 7)) * 32;
+    }
+    c.a = a;
+    c.b = b;
+    return c;
+  }
+  __device__ inline float2 corner(const Col &c, int k, int w) const {
+    const int sb = (w >> 3) * kBrickBytes;
+    const float4 h = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.hdr[k], sb, 0));
+    const int code = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.val[k], sb + (w & 7) * 4, 0);
+    float2 f;
+    if (UNORM) {
+      f = make_float2((float)(code & 0xffff) * (1.0f / 65535.0f),
+                      (float)((unsigned)code >> 16) * (1.0f / 65535.0f));
+    } else {
+      f = __half22float2(__builtin_bit_cast(__half2, code));
+    }
     return make_float2(h.x + h.y * f.x, h.z + h.w * f.y);
+  }
+  __device__ inline float2 at(const Col &c, int w) const {
+    const float2 v00 = corner(c, 0, w), v01 = corner(c, 1, w);
+    const float2 v10 = corner(c, 2, w), v11 = corner(c, 3, w);
+    const float lx = v00.x + c.a * (v01.x - v00.x), ly = v00.y + c.a * (v01.y - v00.y);
+    const float hx = v10.x + c.a * (v11.x - v10.x), hy = v10.y + c.a * (v11.y - v10.y);
+    return make_float2(lx + c.b * (hx - lx), ly + c.b * (hy - ly));
   }
 };
 
-inline size_t f16b_node_bytes(const qpsdm_tgeom &g) {
+inline size_t b16_node_bytes(const qpsdm_tgeom &g) {
   return (size_t)((g.nv + 7) / 8) * ((g.nu + 7) / 8) * ((g.nw + 7) / 8) * kBrickBytes;
 }
 
-inline TabF16B make_tab_f16b(const qpsdm_tgeom &g, const void *p) {
-  TabF16B t;
+template <bool UNORM>
+inline TabB16<UNORM> make_tab_b16(const qpsdm_tgeom &g, const void *p) {
+  TabB16<UNORM> t;
   t.p = (const unsigned char *)p;
-  t.nstride = f16b_node_bytes(g);
+  t.nstride = b16_node_bytes(g);
   t.nbu = (g.nu + 7) / 8;
   t.nbw = (g.nw + 7) / 8;
   return t;
 }
-
-// Bilinear (x, y) interpolation weights of the nearest node's table at the
-// relative position (x - px, y - py): the table column of one surface
-// position under one image column.
-struct Column {
-  int node, iv, iu;
-  float a, b;
-};
 
 __device__ inline void frac_index(float q, int n, int *i0, float *a) {
   q = fminf(fmaxf(q, 0.0f), (float)(n - 1));
@@ -166,25 +229,19 @@ __device__ inline void frac_index(float q, int n, int *i0, float *a) {
   *a = q - (float)i;
 }
 
-__device__ inline Column column_setup(const ImgDev &P, float px, float py, float x, float y) {
-  Column c;
+// Table column of the surface position (px, py) under the image column
+// (x, y): nearest node, then bilinear weights at the relative position
+// (x - px, y - py).
+template <typename Tab>
+__device__ inline typename Tab::Col column_setup(const ImgDev &P, const Tab &tab, float px, float py,
+                                                 float x, float y) {
   const int in = min(max((int)floorf((px - P.n0x) * P.inv_ndx + 0.5f), 0), P.nnx - 1);
   const int jn = min(max((int)floorf((py - P.n0y) * P.inv_ndy + 0.5f), 0), P.nny - 1);
-  c.node = jn * P.nnx + in;
-  frac_index((x - px - P.u0) * P.inv_du, P.nu, &c.iu, &c.a);
-  frac_index((y - py - P.v0) * P.inv_dv, P.nv, &c.iv, &c.b);
-  return c;
-}
-
-// (t, t*) of the column at table depth node w.
-template <typename Tab>
-__device__ inline float2 column_at(const Tab &tab, const Column &c, int w) {
-  const float2 v00 = tab.fetch(c.node, c.iv, c.iu, w), v01 = tab.fetch(c.node, c.iv, c.iu + 1, w);
-  const float2 v10 = tab.fetch(c.node, c.iv + 1, c.iu, w);
-  const float2 v11 = tab.fetch(c.node, c.iv + 1, c.iu + 1, w);
-  const float lx = v00.x + c.a * (v01.x - v00.x), ly = v00.y + c.a * (v01.y - v00.y);
-  const float hx = v10.x + c.a * (v11.x - v10.x), hy = v10.y + c.a * (v11.y - v10.y);
-  return make_float2(lx + c.b * (hx - lx), ly + c.b * (hy - ly));
+  int iu, iv;
+  float a, b;
+  frac_index((x - px - P.u0) * P.inv_du, P.nu, &iu, &a);
+  frac_index((y - py - P.v0) * P.inv_dv, P.nv, &iv, &b);
+  return tab.column(jn * P.nnx + in, iv, iu, a, b);
 }
 
 // Depth node and weight of depth z: w = min(floor(z/dw), nw-2), frac clamped.
@@ -221,22 +278,34 @@ __device__ inline bool tile_culled(const ImgDev &P, const TileBox &b, float sx, 
 }
 
 // Dip-bound taper of the reference (src/q-psdm.cu:2322-2340) on the bisector
-// (X, Y, Z); false when the point is outside the widened bounds.
+// (X, Y, Z), Z > 0: tan = X/Z is compared as X against bound * Z, and 1/Z
+// is formed only for points inside a taper ramp.  false when outside.
 __device__ inline bool dip_weight(const ImgDev &P, float X, float Y, float Z, float *w) {
-  const float rz = __builtin_amdgcn_rcpf(Z);
-  const float tanx = X * rz, tany = Y * rz;
-  if (tanx <= P.tx1c || P.tx2c <= tanx || tany <= P.ty1c || P.ty2c <= tany) return false;
+  if (X <= P.tx1c * Z || P.tx2c * Z <= X || Y <= P.ty1c * Z || P.ty2c * Z <= Y) return false;
   float v = 1.0f;
-  if (tanx < P.tx1) v *= (tanx - P.tx1c) * P.rx1;
-  if (tanx > P.tx2) v *= (P.tx2c - tanx) * P.rx2;
-  if (tany < P.ty1) v *= (tany - P.ty1c) * P.ry1;
-  if (tany > P.ty2) v *= (P.ty2c - tany) * P.ry2;
+  if (X < P.tx1 * Z || X > P.tx2 * Z || Y < P.ty1 * Z || Y > P.ty2 * Z) {
+    const float rz = __builtin_amdgcn_rcpf(Z);
+    const float tanx = X * rz, tany = Y * rz;
+    if (tanx < P.tx1) v *= (tanx - P.tx1c) * P.rx1;
+    if (tanx > P.tx2) v *= (P.tx2c - tanx) * P.rx2;
+    if (tany < P.ty1) v *= (tany - P.ty1c) * P.ry1;
+    if (tany > P.ty2) v *= (P.ty2c - tany) * P.ry2;
+  }
   *w = v;
   return true;
 }
 
-__device__ inline float ld(const float *p) { return *p; }
-__device__ inline float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+// Samples b[idx], b[idx + 1] of a trace's bank row (resource over K*nt).
+__device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const float *) {
+  const float b0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+  const float b1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4 + 4, 0, 0));
+  return make_float2(b0, b1);
+}
+__device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
+  const unsigned short h0 = __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2, 0, 0);
+  const unsigned short h1 = __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2 + 2, 0, 0);
+  return make_float2(__uint_as_float((unsigned)h0 << 16), __uint_as_float((unsigned)h1 << 16));
+}
 
 }  // namespace qpsdm_dev
 

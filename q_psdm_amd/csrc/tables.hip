@@ -93,7 +93,9 @@ __global__ void build_kernel(ModelDev m, qpsdm_tgeom g, float2 *tab) {
 }
 
 // One work-group per brick: min/max of t and t* over the brick's valid
-// cells, then value = off + scale * h with h in [0, 1] stored as FP16.
+// cells, then value = off + scale * h with h in [0, 1] stored as FP16 or as
+// a 16-bit unsigned fixed-point code.
+template <bool UNORM>
 __global__ __launch_bounds__(512) void quant_kernel(qpsdm_tgeom g, const float2 *tab,
                                                     unsigned char *out) {
   const int nbv = (g.nv + 7) / 8, nbu = (g.nu + 7) / 8, nbw = (g.nw + 7) / 8;
@@ -128,14 +130,21 @@ __global__ __launch_bounds__(512) void quant_kernel(qpsdm_tgeom g, const float2 
   const float s0 = red[2][0], ss = fmaxf(red[3][0] - s0, 1e-30f);
   unsigned char *o = out + b * qpsdm_dev::kBrickBytes;
   if (tid == 0) *(float4 *)o = make_float4(t0, ts, s0, ss);
-  const __half2 h = in ? __floats2half2_rn((v.x - t0) / ts, (v.y - s0) / ss) : __floats2half2_rn(0.f, 0.f);
-  *(__half2 *)(o + 16 + tid * 4) = h;
+  const float ht = in ? fminf(fmaxf((v.x - t0) / ts, 0.f), 1.f) : 0.f;
+  const float hs = in ? fminf(fmaxf((v.y - s0) / ss, 0.f), 1.f) : 0.f;
+  unsigned code;
+  if (UNORM) {
+    code = (unsigned)__float2uint_rn(ht * 65535.0f) | ((unsigned)__float2uint_rn(hs * 65535.0f) << 16);
+  } else {
+    code = __builtin_bit_cast(unsigned, __floats2half2_rn(ht, hs));
+  }
+  *(unsigned *)(o + 16 + tid * 4) = code;
 }
 
 __global__ void blend_kernel(const float *a, const float *b, size_t n, float c1, float c2, float *o) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
-    o[i] = a[i] * c1 + b[i] * c2;
+    o[i] = __fadd_rn(__fmul_rn(a[i], c1), __fmul_rn(b[i], c2));  // no contraction: as the host blend
 }
 
 int check_geom(const qpsdm_tgeom *g) {
@@ -175,10 +184,10 @@ extern "C" int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, fl
 
 extern "C" size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g) {
   if (check_geom(g)) return 0;
-  return qpsdm_dev::f16b_node_bytes(*g) * g->nnx * g->nny;
+  return qpsdm_dev::b16_node_bytes(*g) * g->nnx * g->nny;
 }
 
-extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, void *out,
+extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, void *out, int code,
                                      void *stream) {
   int rc = check_geom(g);
   if (rc) return rc;
@@ -186,8 +195,13 @@ extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, voi
   const size_t nb = (size_t)((g->nv + 7) / 8) * ((g->nu + 7) / 8) * ((g->nw + 7) / 8) * g->nnx *
                     g->nny;
   QPSDM_REQUIRE(nb < (1ull << 31), "quantize: too many bricks for one launch");
-  hipLaunchKernelGGL(quant_kernel, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
-                     (const float2 *)tab, (unsigned char *)out);
+  QPSDM_REQUIRE(code == QPSDM_F16B || code == QPSDM_U16B, "quantize: code must be F16B or U16B");
+  if (code == QPSDM_U16B)
+    hipLaunchKernelGGL(quant_kernel<true>, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
+                       (const float2 *)tab, (unsigned char *)out);
+  else
+    hipLaunchKernelGGL(quant_kernel<false>, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
+                       (const float2 *)tab, (unsigned char *)out);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }

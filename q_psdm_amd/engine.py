@@ -106,10 +106,14 @@ class Survey:
                                          _ptr(tab), _stream(stream)))
         return tab
 
-    def quantize_tables(self, tab, stream=None):
+    def quantize_tables(self, tab, code="f16", stream=None):
+        """16-bit brick tables; code "f16" (FP16 codes) or "u16" (fixed point).
+        Returned as a uint8 tensor tagged with .qpsdm_code."""
         nbytes = L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
         out = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-        check(L.lib().qpsdm_tables_quantize(C.byref(self.tg), _ptr(tab), _ptr(out), _stream(stream)))
+        c = L.QPSDM_F16B if code == "f16" else L.QPSDM_U16B
+        check(L.lib().qpsdm_tables_quantize(C.byref(self.tg), _ptr(tab), _ptr(out), c, _stream(stream)))
+        out.qpsdm_code = c
         return out
 
     # -- kernel (a) --------------------------------------------------------
@@ -148,7 +152,7 @@ class Survey:
         else:
             assert tab.numel() == L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
         bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
-        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else L.QPSDM_F16B
+        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else getattr(tab, "qpsdm_code", L.QPSDM_F16B)
         lp = None
         if launch is not None:
             lp = L.Launch(*launch)

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqpsdm.so")
 
-QPSDM_F32, QPSDM_BF16, QPSDM_F16B = 0, 1, 2
+QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B = 0, 1, 2, 3
 
 
 class Window(C.Structure):
@@ -125,7 +125,7 @@ def lib():
         L.qpsdm_tables_build.argtypes = [C.POINTER(Model), C.POINTER(TGeom), vp, vp]
         L.qpsdm_tables_f16b_bytes.restype = C.c_size_t
         L.qpsdm_tables_f16b_bytes.argtypes = [C.POINTER(TGeom)]
-        L.qpsdm_tables_quantize.argtypes = [C.POINTER(TGeom), vp, vp, vp]
+        L.qpsdm_tables_quantize.argtypes = [C.POINTER(TGeom), vp, vp, C.c_int, vp]
         L.qpsdm_tables_blend.argtypes = [vp, vp, C.c_size_t, C.c_float, vp, vp]
         L.qpsdm_read_tidx.argtypes = [C.c_char_p, C.POINTER(TidxHeader), vp, C.c_int]
         L.qpsdm_write_tidx.argtypes = [C.c_char_p, C.POINTER(TidxHeader), vp]

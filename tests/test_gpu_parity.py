@@ -123,14 +123,18 @@ def test_end_to_end_config1_subset_vs_fp64(gpu):
     assert e64 < 1e-4
 
 
-def _dequant(sv, q):
-    """numpy decode of the FP16 offset+scale bricks back to [node][v][u][w][2]."""
+def _dequant(sv, q, code="f16"):
+    """numpy decode of the 16-bit offset+scale bricks back to [node][v][u][w][2]."""
     g = sv.cfg.tgeom
     nbv, nbu, nbw = (g["nv"] + 7) // 8, (g["nu"] + 7) // 8, (g["nw"] + 7) // 8
     nn = g["nnx"] * g["nny"]
     raw = q.cpu().numpy().reshape(nn, nbv, nbu, nbw, 2064)
     hdr = raw[..., :16].copy().view(np.float32)            # [..., 4]
-    val = raw[..., 16:].copy().view(np.float16).astype(np.float32).reshape(nn, nbv, nbu, nbw, 8, 8, 8, 2)
+    if code == "f16":
+        val = raw[..., 16:].copy().view(np.float16).astype(np.float32)
+    else:
+        val = raw[..., 16:].copy().view(np.uint16).astype(np.float32) / 65535.0
+    val = val.reshape(nn, nbv, nbu, nbw, 8, 8, 8, 2)
     out = np.empty((nn, nbv * 8, nbu * 8, nbw * 8, 2), np.float32)
     t = hdr[..., 0, None, None, None] + hdr[..., 1, None, None, None] * val[..., 0]
     s = hdr[..., 2, None, None, None] + hdr[..., 3, None, None, None] * val[..., 1]
@@ -140,17 +144,20 @@ def _dequant(sv, q):
 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
-def test_f16_brick_tables_error_bound(gpu, mk):
+@pytest.mark.parametrize("code,ulp", [("f16", 2.0 ** -11), ("u16", 2.0 ** -16)])
+def test_16bit_brick_tables_error_bound(gpu, mk, code, ulp):
     cfg = mk()
     sv, tab = _survey(cfg)
-    q = sv.quantize_tables(tab)
-    deq, hdr = _dequant(sv, q)
+    q = sv.quantize_tables(tab, code=code)
+    deq, hdr = _dequant(sv, q, code)
     ref = tab.cpu().numpy()
-    # FP16 of a value in [0, 1] is within 2^-11 relative; per brick the error
-    # is at most scale * 2^-11 (plus the FP32 rounding of off + scale * h)
+    # a code in [0, 1] is within one half-spacing (2^-12 for FP16 in [0.5,1),
+    # 2^-17 for 16-bit fixed point) of h, so the error per brick is at most
+    # scale * ulp (plus the FP32 rounding of off + scale * h)
+    err = np.abs(deq[..., 0] - ref[..., 0]).max()
     smax = max(hdr[..., 1].max(), 1e-9)
-    assert np.abs(deq[..., 0] - ref[..., 0]).max() <= smax * 2.0 ** -11 * 1.01 + 1e-6
-    assert np.abs(deq[..., 0] - ref[..., 0]).max() < 0.5e-3  # well inside the 0.5 ms budget
+    assert err <= smax * ulp * 1.01 + 2e-7
+    assert err < 0.5e-3  # inside the 0.5 ms budget
 
 
 def test_mixed_precision_image_vs_fp32_and_fp64(gpu):
@@ -167,9 +174,11 @@ def test_mixed_precision_image_vs_fp32_and_fp64(gpu):
     b32 = sv.qbank(tr_d)
     b16 = sv.qbank(tr_d, dtype=torch.bfloat16)
     q = sv.quantize_tables(tab)
+    qu = sv.quantize_tables(tab, code="u16")
     half = n // 2
     imgs = {}
-    for name, bank, tb in (("fp32", b32, tab), ("bf16bank", b16, tab), ("f16tab", b32, q), ("mixed", b16, q)):
+    for name, bank, tb in (("fp32", b32, tab), ("bf16bank", b16, tab), ("f16tab", b32, q), ("u16tab", b32, qu),
+                           ("mixed", b16, q), ("mixed_u16", b16, qu)):
         img = sv.new_image()
         for tr0, m in ((0, half), (half, n - half)):
             sv.image_shot(img, g_d, tr0, m, bank[tr0:tr0 + m], tb)
@@ -180,3 +189,5 @@ def test_mixed_precision_image_vs_fp32_and_fp64(gpu):
     print("rel-L2 vs FP64 / vs FP32: " + ", ".join(f"{k} {a:.2e}/{b:.2e}" for k, (a, b) in err.items()))
     assert err["fp32"][0] < 1e-4
     assert err["mixed"][0] < 5e-3 and err["mixed"][1] < 5e-3
+    assert err["u16tab"][0] < 2e-4
+    assert err["mixed_u16"][0] < 1e-3      # the mixed form's stated bound vs FP64
