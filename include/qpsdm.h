@@ -215,6 +215,57 @@ int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom_dev, int tr0
 /* Output normalisation img /= nsb (src/q-psdm.cu:1760-1766). */
 int qpsdm_image_scale(float *img_dev, size_t n, float s, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* Pipeline context.  Replaces qpsdm()'s producer/consumer threads and      */
+/* queue (src/q-psdm.cu:1693-1738, 1862-2061; src/queue.h:11-72) and       */
+/* psdm_kernel's per-trace synchronous copies and launch                   */
+/* (src/q-psdm.cu:2218-2243): two pinned staging slots, a copy stream and  */
+/* a compute stream (SU decode on the GPU, Q bank, imaging).  Owns the     */
+/* device image, the tables and every staging buffer.                      */
+/* ---------------------------------------------------------------------- */
+typedef struct qpsdm_ctx qpsdm_ctx;
+
+typedef struct {
+  qpsdm_image_params image;  /* grid, traces, dip taper, table geometry       */
+  qpsdm_bank_params bank;    /* windows and Q bank (nt, K, dtstar as image)   */
+  qpsdm_geom_params geom;    /* SU coordinate scale, model extent, origin     */
+  int bank_dtype;            /* QPSDM_F32 or QPSDM_BF16                       */
+  int table_dtype;           /* QPSDM_F32, QPSDM_F16B or QPSDM_U16B           */
+  int batch_traces;          /* traces per staging slot                       */
+  qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */
+} qpsdm_ctx_params;
+
+int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out);
+int qpsdm_ctx_destroy(qpsdm_ctx *c);
+/* Tables in the FP32 layout, host (on_device = 0) or device memory;
+ * quantised on the device for 16-bit table dtypes. */
+int qpsdm_ctx_set_tables(qpsdm_ctx *c, const float *tab, int on_device);
+int qpsdm_ctx_build_tables(qpsdm_ctx *c, const qpsdm_model *m);
+/* Big-endian SU records with ns = nt (whole records only). */
+int qpsdm_ctx_push_su(qpsdm_ctx *c, const void *bytes, size_t nbytes);
+/* Decoded host traces [ntr][nt] with image-relative geometry [ntr][4]. */
+int qpsdm_ctx_push_traces(qpsdm_ctx *c, const float *samples, const float *geom, int ntr);
+/* One decoded trace with its header, as psdm_kernel(segy tr, ...) takes it
+ * (src/q-psdm.cu:252-283); traces outside the model extent are dropped. */
+struct segy;
+int qpsdm_ctx_psdm_kernel(qpsdm_ctx *c, const struct segy *tr);
+int qpsdm_ctx_flush(qpsdm_ctx *c);
+/* Image [ny][nx][nz] to host memory; divided by nsb when normalize != 0
+ * (src/q-psdm.cu:1760-1766). */
+int qpsdm_ctx_image(qpsdm_ctx *c, float *host_img, int normalize);
+int qpsdm_ctx_reset_image(qpsdm_ctx *c);
+long long qpsdm_ctx_traces_pushed(const qpsdm_ctx *c);
+
+/* Image writer of qpsdm() (src/q-psdm.cu:1769-1800): for each CDP in cdps
+ * that lies in [fci, fci+nx), append one big-endian SU trace of nz samples
+ * (img_line [nx][nz], one inline) with ns = nz, dt = (int)(dzi*1e3),
+ * fldr = line, offset = (int)offset, trid = 1, counit = 1, nvs = 1,
+ * tracl = running index, cdp, and sx = gx, sy = gy, scalco from the
+ * velocity file's coordinates vel_xy [nx][2] of that CDP. */
+int qpsdm_write_image_su(const char *path, int append, const float *img_line, int nx, int nz, float dzi,
+                         int line, float offset, int fci, const int *cdps, int ncdps,
+                         const int32_t *vel_xy, short vel_scalco);
+
 const char *qpsdm_last_error(void);
 const char *qpsdm_version(void);
 

@@ -11,6 +11,7 @@
 #include <unistd.h>
 
 #include "common.h"
+#include "segy.h"
 
 namespace qpsdm {
 
@@ -258,4 +259,37 @@ extern "C" long qpsdm_load_line_table(const char *ttpath, const char *suffix, in
   for (long i = 0; i < n; ++i) out[i] = out[i] * c1 + b[i] * c2;
   free(b);
   return n;
+}
+
+// The offset-class image writer of qpsdm() (src/q-psdm.cu:1769-1800).
+extern "C" int qpsdm_write_image_su(const char *path, int append, const float *img, int nx, int nz,
+                                    float dzi, int line, float offset, int fci, const int *cdps,
+                                    int ncdps, const int32_t *vel_xy, short vel_scalco) {
+  QPSDM_REQUIRE(path && img && cdps && vel_xy && nx > 0 && nz > 0 && nz <= SU_NFLTS,
+                "write_image_su: bad args");
+  FILE *fp = fopen(path, append ? "ab" : "wb");
+  if (!fp) return qpsdm::set_error(QPSDM_EIO, "Error: File [%s] Open Error", path);
+  segy *tr = (segy *)calloc(1, sizeof(segy));
+  tr->ns = nz;
+  tr->dt = (int)(dzi * 1.0E3);
+  tr->fldr = line;
+  tr->offset = (int)offset;
+  tr->trid = 1;
+  tr->counit = 1;
+  tr->nvs = 1;
+  int k = 0;
+  for (int i = 0; i < ncdps; ++i) {
+    const int ixi = cdps[i] - fci;
+    if (ixi < 0 || ixi >= nx) continue;
+    tr->tracl = ++k;
+    tr->cdp = fci + ixi;
+    tr->sx = tr->gx = vel_xy[2 * ixi];
+    tr->sy = tr->gy = vel_xy[2 * ixi + 1];
+    tr->scalco = vel_scalco;
+    memcpy(tr->data, img + (size_t)ixi * nz, sizeof(float) * nz);
+    fputtr(fp, tr);
+  }
+  free(tr);
+  fclose(fp);
+  return QPSDM_OK;
 }

@@ -242,7 +242,7 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
 
 extern "C" int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom, int tr0, int ntr,
                                 unsigned long long *count, void *stream) {
-  qpsdm_launch sh = {8, 8, 8, 1};
+  qpsdm_launch sh = {8, 8, 8, 1};  // count kernel shape
   qk::ImgDev P;
   int rc = qk::make_dev(p, &sh, tr0, ntr, &P);
   if (rc) return rc;

@@ -89,7 +89,7 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->inv_du = 1.0f / g.du;
   o->inv_dv = 1.0f / g.dv;
   o->inv_dw = 1.0f / g.dw;
-  int tx = 8, ty = 8, tzp = 8;
+  int tx = 8, ty = 8, tzp = 4;
   if (sh) {
     if (sh->tile_x > 0) tx = sh->tile_x;
     if (sh->tile_y > 0) ty = sh->tile_y;

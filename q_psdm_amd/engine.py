@@ -176,3 +176,102 @@ class Survey:
 def tstar_max_from_tables(tab):
     """Largest two-leg t* the tables allow (2 x the largest table t*)."""
     return 2.0 * float(tab[..., 1].max())
+
+
+SEGY_BYTES = 240 + 4 * 65535
+
+
+def su_encode(samples, geom_xy, scalco=0, fldr=1, cdp0=1):
+    """Big-endian SU records (the reference's fputtr byte layout) for traces
+    samples [ntr][ns] with integer header coordinates geom_xy [ntr][4]
+    (sx, sy, gx, gy), through the library's own codec."""
+    samples = np.ascontiguousarray(samples, np.float32)
+    ntr, ns = samples.shape
+    rec = 240 + 4 * ns
+    out = bytearray(ntr * rec)
+    tr = (C.c_ubyte * SEGY_BYTES)()
+    hdr = np.zeros(60, np.int32)
+    buf = (C.c_ubyte * rec)()
+    for i in range(ntr):
+        C.memset(tr, 0, 240)
+        h = np.frombuffer(tr, dtype=np.uint8, count=240)
+        hv = h.view(np.int32)
+        hv[0] = i + 1                                      # tracl
+        hv[2] = fldr                                       # fldr
+        hv[5] = cdp0 + i                                   # cdp
+        h[70:72] = np.array([scalco], np.int16).view(np.uint8)
+        hv[18:22] = geom_xy[i]                             # sx sy gx gy
+        h[114:116] = np.array([ns], np.uint16).view(np.uint8)
+        h[116:118] = np.array([2000], np.uint16).view(np.uint8)
+        C.memmove(C.addressof(tr) + 240, samples[i].ctypes.data, 4 * ns)
+        L.lib().qpsdm_su_sputtr(buf, tr)
+        out[i * rec:(i + 1) * rec] = bytes(buf)
+    del hdr
+    return bytes(out)
+
+
+class Pipeline:
+    """The qpsdm_ctx pipeline: pinned staging slots, copy + compute streams,
+    SU decode on the GPU, Q bank and imaging into a context-owned image."""
+
+    def __init__(self, survey, geom_params=None, bank_dtype="f32", table_dtype="f32",
+                 batch_traces=1024, launch=None):
+        p = L.CtxParams()
+        p.image, p.bank = survey.ip, survey.bp
+        if geom_params is None:
+            geom_params = L.GeomParams(1.0, -1e30, 1e30, -1e30, 1e30, 0.0, 0.0, 0.0, 0.0,
+                                       survey.cfg.dx, survey.cfg.dy)
+        p.geom = geom_params
+        p.bank_dtype = L.QPSDM_F32 if bank_dtype == "f32" else L.QPSDM_BF16
+        p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B}[table_dtype]
+        p.batch_traces = batch_traces
+        if launch is not None:
+            p.shape = L.Launch(*launch)
+        self.survey = survey
+        self.params = p
+        self.ctx = C.c_void_p()
+        check(L.lib().qpsdm_ctx_create(C.byref(p), C.byref(self.ctx)))
+
+    def close(self):
+        if self.ctx:
+            L.lib().qpsdm_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def set_tables(self, tab):
+        on_dev = isinstance(tab, torch.Tensor) and tab.is_cuda
+        ptr = C.c_void_p(tab.data_ptr()) if isinstance(tab, torch.Tensor) else \
+            np.ascontiguousarray(tab, np.float32).ctypes.data_as(C.c_void_p)
+        check(L.lib().qpsdm_ctx_set_tables(self.ctx, ptr, 1 if on_dev else 0))
+
+    def build_tables(self):
+        check(L.lib().qpsdm_ctx_build_tables(self.ctx, C.byref(model_struct(self.survey.cfg.model))))
+
+    def push_su(self, data):
+        check(L.lib().qpsdm_ctx_push_su(self.ctx, data, len(data)))
+
+    def push_traces(self, samples, geom):
+        samples = np.ascontiguousarray(samples, np.float32)
+        geom = np.ascontiguousarray(geom, np.float32)
+        check(L.lib().qpsdm_ctx_push_traces(self.ctx, samples.ctypes.data_as(C.c_void_p),
+                                            geom.ctypes.data_as(C.c_void_p), samples.shape[0]))
+
+    def psdm_kernel(self, segy_record):
+        """One decoded trace given as SU record bytes (header + samples)."""
+        tr = (C.c_ubyte * SEGY_BYTES)()
+        L.lib().qpsdm_su_sgettr(segy_record, tr)
+        check(L.lib().qpsdm_ctx_psdm_kernel(self.ctx, tr))
+
+    def image(self, normalize=False):
+        c = self.survey.cfg
+        out = np.zeros((c.ny, c.nx, c.nz), np.float32)
+        check(L.lib().qpsdm_ctx_image(self.ctx, out.ctypes.data_as(C.c_void_p), 1 if normalize else 0))
+        return out
+
+    def reset_image(self):
+        check(L.lib().qpsdm_ctx_reset_image(self.ctx))
+
+    def traces_pushed(self):
+        return L.lib().qpsdm_ctx_traces_pushed(self.ctx)

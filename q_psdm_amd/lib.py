@@ -70,6 +70,12 @@ class Launch(C.Structure):
                 ("trace_chunk", C.c_int)]
 
 
+class CtxParams(C.Structure):
+    _fields_ = [("image", ImageParams), ("bank", BankParams), ("geom", GeomParams),
+                ("bank_dtype", C.c_int), ("table_dtype", C.c_int), ("batch_traces", C.c_int),
+                ("shape", Launch)]
+
+
 class TrHdr(C.Structure):
     _fields_ = [("sx", C.c_int32), ("sy", C.c_int32), ("gx", C.c_int32), ("gy", C.c_int32),
                 ("fldr", C.c_int32), ("cdp", C.c_int32), ("offset", C.c_int32),
@@ -95,7 +101,10 @@ EXPORTS = [
     "qpsdm_pkg_readparfile_ii", "qpsdm_pkg_initargs", "qpsdm_pkg_getparint",
     "qpsdm_pkg_getparfloat", "qpsdm_pkg_getpardouble", "qpsdm_pkg_getparstr",
     "qpsdm_pkg_time_offset_mute_santity", "qpsdm_pkg_parsepath", "qpsdm_pkg_strrtrim",
-    "qpsdm_pkg_free",
+    "qpsdm_pkg_free", "qpsdm_ctx_create", "qpsdm_ctx_destroy", "qpsdm_ctx_set_tables",
+    "qpsdm_ctx_build_tables", "qpsdm_ctx_push_su", "qpsdm_ctx_push_traces", "qpsdm_ctx_psdm_kernel",
+    "qpsdm_ctx_flush", "qpsdm_ctx_image", "qpsdm_ctx_reset_image", "qpsdm_ctx_traces_pushed",
+    "qpsdm_write_image_su",
 ]
 
 _lib = None
@@ -155,6 +164,20 @@ def lib():
         L.qpsdm_pkg_parsetuple.argtypes = [C.c_char_p, ip, ip]
         L.qpsdm_pkg_strrtrim.restype = C.c_char_p
         L.qpsdm_pkg_free.argtypes = [vp]
+        L.qpsdm_ctx_create.argtypes = [C.POINTER(CtxParams), C.POINTER(vp)]
+        L.qpsdm_ctx_destroy.argtypes = [vp]
+        L.qpsdm_ctx_set_tables.argtypes = [vp, vp, C.c_int]
+        L.qpsdm_ctx_build_tables.argtypes = [vp, C.POINTER(Model)]
+        L.qpsdm_ctx_push_su.argtypes = [vp, vp, C.c_size_t]
+        L.qpsdm_ctx_push_traces.argtypes = [vp, vp, vp, C.c_int]
+        L.qpsdm_ctx_psdm_kernel.argtypes = [vp, vp]
+        L.qpsdm_ctx_flush.argtypes = [vp]
+        L.qpsdm_ctx_image.argtypes = [vp, vp, C.c_int]
+        L.qpsdm_ctx_reset_image.argtypes = [vp]
+        L.qpsdm_ctx_traces_pushed.restype = C.c_longlong
+        L.qpsdm_ctx_traces_pushed.argtypes = [vp]
+        L.qpsdm_write_image_su.argtypes = [C.c_char_p, C.c_int, vp, C.c_int, C.c_int, C.c_float, C.c_int,
+                                           C.c_float, C.c_int, vp, C.c_int, vp, C.c_short]
         _lib = L
     return _lib
 

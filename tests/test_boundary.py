@@ -255,3 +255,33 @@ def test_line_table_bracket_and_blend(tmp_path):
     n = L.lib().qpsdm_load_line_table(str(tmp_path).encode(), b"ttt", 106, 90, 110,
                                       out.ctypes.data_as(C.c_void_p), 12, C.byref(lo), C.byref(hi))
     assert n < 0 and b"no travel time table" in L.lib().qpsdm_last_error()
+
+
+def test_image_writer_headers_and_bytes(tmp_path):
+    """qpsdm_write_image_su keeps the reference writer's header fields
+    (src/q-psdm.cu:1769-1800) and big-endian samples."""
+    nx, nz = 5, 7
+    img = np.arange(nx * nz, dtype=np.float32).reshape(nx, nz) * 0.5
+    cdps = np.array([101, 103, 99, 104], np.int32)   # 99 lies outside [fci, fci+nx)
+    vel = np.array([[670110 + 25 * i, 5104080] for i in range(nx)], np.int32)
+    p = str(tmp_path / "node2_0.off0.L350.su").encode()
+    L.check(L.lib().qpsdm_write_image_su(p, 0, img.ctypes.data_as(C.c_void_p), nx, nz, 8.0, 350, 400.0,
+                                         100, cdps.ctypes.data_as(C.c_void_p), len(cdps),
+                                         vel.ctypes.data_as(C.c_void_p), 0))
+    raw = open(p, "rb").read()
+    rec = 240 + 4 * nz
+    assert len(raw) == 3 * rec
+    hdr = (L.TrHdr * 3)()
+    smp = np.zeros((3, nz), np.float32)
+    assert L.lib().qpsdm_su_decode(raw, len(raw), nz, smp.ctypes.data_as(C.c_void_p), hdr) == 3
+    for k, cdp in enumerate((101, 103, 104)):
+        h = hdr[k]
+        assert (h.tracl, h.fldr, h.cdp, h.offset, h.ns, h.dt) == (k + 1, 350, cdp, 400, nz, 8000)
+        assert (h.sx, h.gx, h.sy, h.gy) == (vel[cdp - 100, 0],) * 2 + (5104080,) * 2
+        np.testing.assert_array_equal(smp[k], img[cdp - 100])
+    # trid = 1, counit = 1, nvs = 1 as the reference sets them; other bytes 0
+    assert raw[28:30] == b"\x00\x01" and raw[88:90] == b"\x00\x01" and raw[30:32] == b"\x00\x01"
+    assert raw[210:240] == bytes(30)
+    # same header bytes as the reference's golden output file for its first trace
+    gold = open(os.path.join(GOLD, "L350_first3.su"), "rb").read()
+    assert gold[28:32] == raw[28:32] and gold[114:118][:2] != b""
