@@ -212,6 +212,40 @@ int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, flo
 int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom_dev, int tr0, int ntr,
                      unsigned long long *count_dev, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* Batched imaging (no counterpart in the reference, which launches once   */
+/* per trace): shot gathers of uneven sizes with per-shot dip apertures    */
+/* through one persistent kernel; tiles are owned by one work-group across */
+/* all shots (fixed summation order, no image atomics).                    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int tr0, ntr;                      /* geom/bank rows of the shot          */
+  float tanx1, tanx2, tany1, tany2;  /* the shot's dip bounds (tan)         */
+} qpsdm_shot;
+
+/* shots_dev: device array of nshot descriptors; geom/bank hold every trace
+ * of the batch (bank row = geom row); work_dev: 8 device uint32 counters
+ * (reset by the call); nwg persistent work-groups (0 = resident capacity).
+ * The image params' own dip bounds are not used; taperzone is. */
+int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
+                      const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
+                      int table_dtype, const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,
+                      void *stream);
+int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom_dev, const qpsdm_shot *shots_dev,
+                      int nshot, unsigned long long *count_dev, void *stream);
+
+/* A captured hipGraph of one batch: Q bank of traces_dev [ntr][nt] into
+ * bank_dev, then qpsdm_image_batch.  Replays read the buffers' contents at
+ * launch time, so the caller refills traces/geom/shots between replays. */
+typedef struct qpsdm_graph qpsdm_graph;
+int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float *traces_dev, int ntr, void *bank_dev,
+                             int bank_dtype, const qpsdm_image_params *ip, const qpsdm_launch *shape,
+                             float *img_dev, const float *geom_dev, const void *tab_dev, int table_dtype,
+                             const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,
+                             qpsdm_graph **out);
+int qpsdm_graph_launch(qpsdm_graph *g, void *stream);
+int qpsdm_graph_destroy(qpsdm_graph *g);
+
 /* Output normalisation img /= nsb (src/q-psdm.cu:1760-1766). */
 int qpsdm_image_scale(float *img_dev, size_t n, float s, void *stream);
 

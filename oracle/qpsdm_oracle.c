@@ -440,7 +440,8 @@ void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double
 /* bounds it1 > 0 && it2 < nt (src/q-psdm.cu:2354-2357); weight            */
 /* w*(tr + 1/tr), tr = (ds/dg)^3 tg/ts (src/q-psdm.cu:2372-2374); image    */
 /* z fastest, traces accumulated in order.                                  */
-/* geom: per trace (sx, sy, gx, gy) in image-relative metres.               */
+/* geom: per trace (sx, sy, gx, gy) in image-relative metres; a NaN sx     */
+/* marks a trace the extent check rejected (src/q-psdm.cu:1379-1383).       */
 /* Returns the update count (pairs inside the dip aperture).                */
 /* ---------------------------------------------------------------------- */
 #define IMAGE_BODY(REAL, SQRT)                                                             \
@@ -455,6 +456,7 @@ void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double
     for (int itr = 0; itr < ntr; ++itr) {                                                  \
       REAL sx = geom[4 * itr], sy = geom[4 * itr + 1];                                     \
       REAL gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];                                 \
+      if (sx != sx) continue; /* rejected by the extent check */                           \
       const BANK *b = bank + (size_t)itr * p->K * p->nt;                                   \
       REAL y = (REAL)iy * p->dy;                                                           \
       for (int ix = 0; ix < p->nx; ++ix) {                                                 \
@@ -559,6 +561,7 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
     for (int itr = 0; itr < ntr; ++itr) {
       const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
       const float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
+      if (sx != sx) continue;
       const float *bt = bank + (size_t)itr * p->K * p->nt;
       const float y = (float)iy * p->dy;
       for (int ix = 0; ix < p->nx; ++ix) {
@@ -617,6 +620,7 @@ int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
     for (int itr = 0; itr < ntr; ++itr) {
       float sx = geom[4 * itr], sy = geom[4 * itr + 1];
       float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
+      if (sx != sx) continue;
       float y = (float)iy * p->dy;
       for (int ix = 0; ix < p->nx; ++ix) {
         float x = (float)ix * p->dx;

@@ -1,0 +1,242 @@
+// Batched imaging: shot gathers of uneven trace counts and per-shot
+// apertures through one persistent kernel, and a captured hipGraph of
+// [Q bank of the batch -> counter reset -> persistent imaging].
+//
+// The persistent grid is sized to the resident capacity of the device.
+// Image tiles are split into eight contiguous ranges, one per group of
+// work-groups with the same blockIdx.x % 8 (the XCD a work-group lands on
+// under round-robin dispatch), so tiles processed together on one XCD are
+// spatial neighbours sharing table and bank lines in its L2; each group takes
+// tiles from its own atomic counter and then steals from the other ranges.
+// A tile is owned by one work-group for every shot of the batch, so the image
+// needs no atomics and the summation order (shots, then traces, in order) is
+// fixed.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "common.h"
+#include "image_tile.h"
+
+namespace qk = qpsdm_dev;
+
+// Prepares the Q-bank coefficients of a parameter set (defined in qbank.hip).
+int qpsdm_qbank_prepare(const qpsdm_bank_params *p);
+
+namespace {
+
+constexpr int kParts = 8;
+
+struct ShotRuns {
+  const qpsdm_shot *shots;
+  float tz;
+  __device__ qk::Run operator()(int r) const {
+    const qpsdm_shot s = shots[r];
+    qk::Run run;
+    run.tr0 = s.tr0;
+    run.ntr = s.ntr;
+    run.brow0 = s.tr0;
+    run.dip = qk::make_dip(s.tanx1, s.tanx2, s.tany1, s.tany2, tz);
+    return run;
+  }
+};
+
+__global__ void reset_kernel(unsigned *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+template <int ZP, typename BankT, typename Tab>
+__global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+                                                         const float4 *__restrict__ geom,
+                                                         const BankT *__restrict__ bank, ShotRuns runs,
+                                                         int nshot, unsigned *counters, int ncol, int nzg) {
+  const int total = ncol * nzg;
+  const int part = blockIdx.x % kParts;
+  for (int step = 0; step < kParts; ++step) {
+    const int q = (part + step) % kParts;
+    const int lo = (int)((long long)total * q / kParts), hi = (int)((long long)total * (q + 1) / kParts);
+    for (;;) {
+      unsigned k = 0;
+      if (threadIdx.x == 0) k = atomicAdd(&counters[q], 1u);
+      k = __shfl(k, 0, 64);
+      const int tile = lo + (int)k;
+      if (tile >= hi) break;
+      // depth group slowest: a range covers whole depth slabs of the image
+      qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+    }
+  }
+}
+
+template <int ZP>
+__global__ __launch_bounds__(64) void count_batch_kernel(qk::ImgDev P, const float4 *__restrict__ geom,
+                                                         ShotRuns runs, int nshot,
+                                                         unsigned long long *count) {
+  const unsigned n = qk::count_tile<ZP>(P, geom, blockIdx.x, blockIdx.y, runs, nshot);
+  unsigned long long v = n;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (threadIdx.x == 0 && v) atomicAdd(count, v);
+}
+
+// Launch arguments of one batch; with run == false only the resident grid
+// size is resolved (a query made before graph capture).
+struct Go {
+  float *img;
+  const float4 *geom;
+  ShotRuns runs;
+  int nshot;
+  unsigned *work;
+  int nwg;
+  hipStream_t s;
+  bool run;
+  int *nwg_out;
+};
+
+template <int ZP, typename BankT, typename Tab>
+int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
+  auto kern = image_batch_kernel<ZP, BankT, Tab>;
+  int nwg = go.nwg;
+  if (nwg <= 0) {
+    int per_cu = 0, dev = 0, ncu = 0;
+    QPSDM_HIP(hipGetDevice(&dev));
+    QPSDM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    QPSDM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    nwg = ncu * (per_cu > 0 ? per_cu : 1);
+  }
+  if (go.nwg_out) *go.nwg_out = nwg;
+  if (!go.run) return QPSDM_OK;
+  const hipStream_t s = go.s;
+  const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
+  const int nzg = (P.nz + ZP - 1) / ZP;
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, kParts);
+  hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, go.nshot, go.work,
+                     ncol, nzg);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+template <typename BankT, typename Tab>
+int launch_batch_tz(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
+  switch (P.tile_z) {
+    case 2: return launch_batch_t<2>(P, tab, bank, go);
+    case 3: return launch_batch_t<3>(P, tab, bank, go);
+    case 4: return launch_batch_t<4>(P, tab, bank, go);
+    case 6: return launch_batch_t<6>(P, tab, bank, go);
+    case 8: return launch_batch_t<8>(P, tab, bank, go);
+    default:
+      return qpsdm::set_error(QPSDM_EINVAL, "image_batch: unsupported tile_z=%d (2, 3, 4, 6, 8)", P.tile_z);
+  }
+}
+
+template <typename Tab>
+int launch_batch_bank(const qk::ImgDev &P, const Tab &tab, const void *bank, int bank_dtype, const Go &go) {
+  if (bank_dtype == QPSDM_F32) return launch_batch_tz(P, tab, (const float *)bank, go);
+  if (bank_dtype == QPSDM_BF16) return launch_batch_tz(P, tab, (const __hip_bfloat16 *)bank, go);
+  return qpsdm::set_error(QPSDM_EINVAL, "image_batch: bank dtype %d", bank_dtype);
+}
+
+int batch_dispatch(const qpsdm_image_params *p, const qpsdm_launch *shape, const void *bank, int bank_dtype,
+                   const void *tab, int table_dtype, const Go &go) {
+  qk::ImgDev P;
+  int rc = qk::make_dev(p, shape, 0, 0, &P);
+  if (rc) return rc;
+  switch (table_dtype) {
+    case QPSDM_F32: return launch_batch_bank(P, qk::make_tab_f32(p->tg, tab), bank, bank_dtype, go);
+    case QPSDM_F16B: return launch_batch_bank(P, qk::make_tab_b16<false>(p->tg, tab), bank, bank_dtype, go);
+    case QPSDM_U16B: return launch_batch_bank(P, qk::make_tab_b16<true>(p->tg, tab), bank, bank_dtype, go);
+    default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: table dtype %d", table_dtype);
+  }
+}
+
+}  // namespace
+
+extern "C" int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img,
+                                 const float *geom, const void *bank, int bank_dtype, const void *tab,
+                                 int table_dtype, const qpsdm_shot *shots, int nshot, unsigned *work,
+                                 int nwg, void *stream) {
+  QPSDM_REQUIRE(p, "image_batch: null params");
+  QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
+  if (nshot == 0) return QPSDM_OK;
+  const Go go{img, (const float4 *)geom, ShotRuns{shots, p->taperzone}, nshot, work, nwg, (hipStream_t)stream,
+              true, nullptr};
+  return batch_dispatch(p, shape, bank, bank_dtype, tab, table_dtype, go);
+}
+
+extern "C" int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom, const qpsdm_shot *shots,
+                                 int nshot, unsigned long long *count, void *stream) {
+  qpsdm_launch sh = {8, 8, 8, 1};  // count kernel shape
+  qk::ImgDev P;
+  int rc = qk::make_dev(p, &sh, 0, 0, &P);
+  if (rc) return rc;
+  QPSDM_REQUIRE(geom && shots && count && nshot >= 0, "count_batch: null buffer");
+  if (nshot == 0) return QPSDM_OK;
+  const int nxt = (P.nx + 7) / 8, nyt = (P.ny + 7) / 8;
+  hipLaunchKernelGGL(count_batch_kernel<8>, dim3(nxt * nyt, (P.nz + 7) / 8), dim3(64), 0, (hipStream_t)stream,
+                     P, (const float4 *)geom, ShotRuns{shots, p->taperzone}, nshot, count);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+// ---- captured graph of one batch ---------------------------------------
+struct qpsdm_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float *traces, int ntr, void *bank,
+                                        int bank_dtype, const qpsdm_image_params *ip, const qpsdm_launch *shape,
+                                        float *img, const float *geom, const void *tab, int table_dtype,
+                                        const qpsdm_shot *shots, int nshot, unsigned *work, int nwg,
+                                        qpsdm_graph **out) {
+  QPSDM_REQUIRE(out && bp && ip && traces && bank && img && geom && tab && shots && work,
+                "batch_graph_create: null args");
+  QPSDM_REQUIRE(ntr > 0 && nshot > 0, "batch_graph_create: empty batch");
+  // everything that allocates or copies synchronously happens before capture
+  int rc = qpsdm_qbank_prepare(bp);
+  if (rc) return rc;
+  if (nwg <= 0) {  // resolve the resident grid size outside the capture
+    const Go q{img, (const float4 *)geom, ShotRuns{shots, ip->taperzone}, nshot, work, 0, nullptr, false, &nwg};
+    rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
+    if (rc) return rc;
+  }
+  hipStream_t s;
+  QPSDM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  qpsdm_graph *g = new qpsdm_graph();
+  auto fail = [&](int code) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    (void)hipStreamDestroy(s);
+    return code;
+  };
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "batch_graph_create: begin capture failed"));
+  rc = qpsdm_qbank(bp, traces, ntr, bank, bank_dtype, s);
+  if (!rc)
+    rc = qpsdm_image_batch(ip, shape, img, geom, bank, bank_dtype, tab, table_dtype, shots, nshot, work, nwg, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  g->graph = graph;
+  if (rc) return fail(rc);
+  if (e != hipSuccess || !graph)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "batch_graph_create: end capture failed: %s", hipGetErrorString(e)));
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "batch_graph_create: instantiate failed"));
+  (void)hipStreamDestroy(s);
+  *out = g;
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_graph_launch(qpsdm_graph *g, void *stream) {
+  QPSDM_REQUIRE(g && g->exec, "graph_launch: null graph");
+  QPSDM_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_graph_destroy(qpsdm_graph *g) {
+  if (!g) return QPSDM_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return QPSDM_OK;
+}

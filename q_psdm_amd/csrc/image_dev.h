@@ -13,6 +13,39 @@
 
 namespace qpsdm_dev {
 
+// Dip-bound taper of one run of traces: bounds tan1 <= tan2 per axis, the
+// bounds widened by taperzone*(1 + tan^2) (src/q-psdm.cu:2324-2333), the
+// reciprocal ramp widths and the |tan| beyond which every point is rejected.
+struct Dip {
+  float tx1, tx2, ty1, ty2, tx1c, tx2c, ty1c, ty2c;
+  float rx1, rx2, ry1, ry2;
+  float Tx, Ty;
+};
+
+__host__ __device__ inline Dip make_dip(float tanx1, float tanx2, float tany1, float tany2, float tz) {
+  Dip d;
+  d.tx1 = tanx1;
+  d.tx2 = tanx2;
+  d.ty1 = tany1;
+  d.ty2 = tany2;
+  d.tx1c = tanx1 - tz * (1 + tanx1 * tanx1);
+  d.tx2c = tanx2 + tz * (1 + tanx2 * tanx2);
+  d.ty1c = tany1 - tz * (1 + tany1 * tany1);
+  d.ty2c = tany2 + tz * (1 + tany2 * tany2);
+  d.rx1 = 1.0f / (d.tx1 - d.tx1c);
+  d.rx2 = 1.0f / (d.tx2c - d.tx2);
+  d.ry1 = 1.0f / (d.ty1 - d.ty1c);
+  d.ry2 = 1.0f / (d.ty2c - d.ty2);
+  d.Tx = fmaxf(fabsf(d.tx1c), fabsf(d.tx2c));
+  d.Ty = fmaxf(fabsf(d.ty1c), fabsf(d.ty2c));
+  return d;
+}
+
+__host__ __device__ inline bool dip_ok(const Dip &d) {
+  return d.tx1c < d.tx1 && d.tx1 <= d.tx2 && d.tx2 < d.tx2c && d.ty1c < d.ty1 && d.ty1 <= d.ty2 &&
+         d.ty2 < d.ty2c;
+}
+
 struct ImgDev {
   int nx, ny, nz;
   float dx, dy, dz;
@@ -20,9 +53,8 @@ struct ImgDev {
   float inv_dt;
   int K;
   float inv_dtstar;
-  float tx1, tx2, ty1, ty2, tx1c, tx2c, ty1c, ty2c;
-  float rx1, rx2, ry1, ry2;  // reciprocal ramp widths
-  float Tx, Ty;              // |tan| beyond which a point is rejected
+  float taperzone;
+  Dip dip;  // of single-run launches
   float tmin;
   int nnx, nny;
   float n0x, n0y, inv_ndx, inv_ndy;
@@ -56,24 +88,9 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->inv_dt = 1.0f / p->dt;
   o->K = p->K;
   o->inv_dtstar = 1.0f / p->dtstar;
-  const float tz = p->taperzone;
-  o->tx1 = p->tanx1;
-  o->tx2 = p->tanx2;
-  o->ty1 = p->tany1;
-  o->ty2 = p->tany2;
-  o->tx1c = p->tanx1 - tz * (1 + p->tanx1 * p->tanx1);
-  o->tx2c = p->tanx2 + tz * (1 + p->tanx2 * p->tanx2);
-  o->ty1c = p->tany1 - tz * (1 + p->tany1 * p->tany1);
-  o->ty2c = p->tany2 + tz * (1 + p->tany2 * p->tany2);
-  QPSDM_REQUIRE(o->tx1c < o->tx1 && o->tx1 <= o->tx2 && o->tx2 < o->tx2c && o->ty1c < o->ty1 &&
-                    o->ty1 <= o->ty2 && o->ty2 < o->ty2c,
-                "image: dip bounds must satisfy tan1c < tan1 <= tan2 < tan2c");
-  o->rx1 = 1.0f / (o->tx1 - o->tx1c);
-  o->rx2 = 1.0f / (o->tx2c - o->tx2);
-  o->ry1 = 1.0f / (o->ty1 - o->ty1c);
-  o->ry2 = 1.0f / (o->ty2c - o->ty2);
-  o->Tx = fmaxf(fabsf(o->tx1c), fabsf(o->tx2c));
-  o->Ty = fmaxf(fabsf(o->ty1c), fabsf(o->ty2c));
+  o->taperzone = p->taperzone;
+  o->dip = make_dip(p->tanx1, p->tanx2, p->tany1, p->tany2, p->taperzone);
+  QPSDM_REQUIRE(dip_ok(o->dip), "image: dip bounds must satisfy tan1c < tan1 <= tan2 < tan2c");
   o->tmin = p->tmin;
   o->nnx = g.nnx;
   o->nny = g.nny;
@@ -256,23 +273,22 @@ struct TileBox {
   float xa, xb, ya, yb, limx, limy;
 };
 
-__device__ inline TileBox tile_box(const ImgDev &P, int tx0, int ty0, int z0, int tz) {
+__device__ inline TileBox tile_box(const ImgDev &P, const Dip &D, int tx0, int ty0, int z0, int tz) {
   TileBox b;
   b.xa = (float)tx0 * P.dx;
   b.xb = (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx;
   b.ya = (float)ty0 * P.dy;
   b.yb = (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy;
   const float zmax = (float)(min(z0 + tz, P.nz) - 1) * P.dz;
-  b.limx = P.Tx * zmax * 1.0001f + 0.01f;
-  b.limy = P.Ty * zmax * 1.0001f + 0.01f;
+  b.limx = D.Tx * zmax * 1.0001f + 0.01f;
+  b.limy = D.Ty * zmax * 1.0001f + 0.01f;
   return b;
 }
 
 // True when every point of the tile lies on one side of both the source and
 // the receiver by more than Tx*z (or Ty*z): then the bisector's |tan| exceeds
 // Tx (Ty) at every point and the dip test rejects all of them.
-__device__ inline bool tile_culled(const ImgDev &P, const TileBox &b, float sx, float sy, float gx,
-                                   float gy) {
+__device__ inline bool tile_culled(const TileBox &b, float sx, float sy, float gx, float gy) {
   return (b.xa - fmaxf(sx, gx) > b.limx) || (fminf(sx, gx) - b.xb > b.limx) ||
          (b.ya - fmaxf(sy, gy) > b.limy) || (fminf(sy, gy) - b.yb > b.limy);
 }
@@ -280,7 +296,7 @@ __device__ inline bool tile_culled(const ImgDev &P, const TileBox &b, float sx, 
 // Dip-bound taper of the reference (src/q-psdm.cu:2322-2340) on the bisector
 // (X, Y, Z), Z > 0: tan = X/Z is compared as X against bound * Z, and 1/Z
 // is formed only for points inside a taper ramp.  false when outside.
-__device__ inline bool dip_weight(const ImgDev &P, float X, float Y, float Z, float *w) {
+__device__ inline bool dip_weight(const Dip &P, float X, float Y, float Z, float *w) {
   if (X <= P.tx1c * Z || P.tx2c * Z <= X || Y <= P.ty1c * Z || P.ty2c * Z <= Y) return false;
   float v = 1.0f;
   if (X < P.tx1 * Z || X > P.tx2 * Z || Y < P.ty1 * Z || Y > P.ty2 * Z) {

@@ -361,6 +361,15 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
 
 }  // namespace
 
+// Coefficients of a parameter set uploaded ahead of use (so that a later
+// launch can be captured into a graph without allocations or copies).
+int qpsdm_qbank_prepare(const qpsdm_bank_params *p) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  Coefs *c = nullptr;
+  return get_coefs(p, &c);
+}
+
 extern "C" int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces, int ntr,
                                    float *re, float *im, void *stream) {
   QPSDM_REQUIRE(re && im, "spectra: null outputs");

@@ -172,6 +172,98 @@ class Survey:
     def scale_image(self, img, s, stream=None):
         check(L.lib().qpsdm_image_scale(_ptr(img), img.numel(), s, _stream(stream)))
 
+    # -- batched imaging -----------------------------------------------------
+    def _codes(self, bank, tab):
+        if tab.dtype == torch.float32:
+            assert tuple(tab.shape) == self.table_shape()
+        else:
+            assert tab.numel() == L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
+        bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
+        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else getattr(tab, "qpsdm_code", L.QPSDM_F16B)
+        return bcode, tcode
+
+    def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, stream=None):
+        """Accumulate every shot of a batch into img with the persistent
+        kernel; shots is a ShotList (bank row = geom row)."""
+        c = self.cfg
+        assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
+        assert geom.dim() == 2 and geom.shape[1] == 4 and shots.rows <= geom.shape[0]
+        assert bank.dim() == 3 and bank.shape[0] >= shots.rows and bank.shape[1:] == (c.K, c.nt)
+        bcode, tcode = self._codes(bank, tab)
+        if work is None:
+            work = torch.empty(8, dtype=torch.int32, device="cuda")
+        lp = L.Launch(*launch) if launch is not None else None
+        check(L.lib().qpsdm_image_batch(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img), _ptr(geom),
+                                        _ptr(bank), bcode, _ptr(tab), tcode, _ptr(shots.dev), shots.n,
+                                        _ptr(work), nwg, _stream(stream)))
+
+    def count_batch(self, geom, shots, stream=None):
+        counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+        check(L.lib().qpsdm_count_batch(C.byref(self.ip), _ptr(geom), _ptr(shots.dev), shots.n, _ptr(counter),
+                                        _stream(stream)))
+        return int(counter.item())
+
+
+class ShotList:
+    """Device array of qpsdm_shot descriptors: (tr0, ntr) rows of geom/bank
+    and the shot's dip bounds as tangents."""
+
+    def __init__(self, shots, device="cuda"):
+        arr = (L.Shot * max(len(shots), 1))()
+        for i, s in enumerate(shots):
+            arr[i] = L.Shot(*s)
+        raw = bytes(arr)
+        self.n = len(shots)
+        self.rows = max((s[0] + s[1] for s in shots), default=0)
+        self.host = list(shots)
+        self.dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+    def update(self, shots):
+        """Overwrite the descriptors in place (same count), for graph replays."""
+        assert len(shots) == self.n
+        arr = (L.Shot * max(self.n, 1))()
+        for i, s in enumerate(shots):
+            arr[i] = L.Shot(*s)
+        self.dev.copy_(torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8))
+        self.rows = max((s[0] + s[1] for s in shots), default=0)
+        self.host = list(shots)
+
+
+class BatchGraph:
+    """A captured graph of [Q bank of traces -> persistent imaging of shots].
+    The buffers are bound at capture; replays read their current contents."""
+
+    def __init__(self, survey, traces, bank, img, geom, tab, shots, launch=None, nwg=0):
+        c = survey.cfg
+        assert traces.dtype == torch.float32 and traces.shape[1] == c.nt and traces.is_contiguous()
+        assert bank.shape == (traces.shape[0], c.K, c.nt) and img.shape == (c.ny, c.nx, c.nz)
+        assert shots.rows <= traces.shape[0] <= geom.shape[0]
+        bcode, tcode = survey._codes(bank, tab)
+        self.work = torch.empty(8, dtype=torch.int32, device="cuda")
+        self._keep = (traces, bank, img, geom, tab, shots)
+        lp = L.Launch(*launch) if launch is not None else None
+        h = C.c_void_p()
+        torch.cuda.synchronize()
+        check(L.lib().qpsdm_batch_graph_create(
+            C.byref(survey.bp), _ptr(traces), traces.shape[0], _ptr(bank), bcode, C.byref(survey.ip),
+            C.byref(lp) if lp else None, _ptr(img), _ptr(geom), _ptr(tab), tcode, _ptr(shots.dev), shots.n,
+            _ptr(self.work), nwg, C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        check(L.lib().qpsdm_graph_launch(self.h, _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().qpsdm_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def tstar_max_from_tables(tab):
     """Largest two-leg t* the tables allow (2 x the largest table t*)."""

@@ -83,6 +83,11 @@ class TrHdr(C.Structure):
                 ("ns", C.c_uint16), ("dt", C.c_uint16)]
 
 
+class Shot(C.Structure):
+    _fields_ = [("tr0", C.c_int), ("ntr", C.c_int), ("tanx1", C.c_float), ("tanx2", C.c_float),
+                ("tany1", C.c_float), ("tany2", C.c_float)]
+
+
 class CoorT(C.Structure):
     _fields_ = [("sx", C.c_int), ("sy", C.c_int)]
 
@@ -104,7 +109,8 @@ EXPORTS = [
     "qpsdm_pkg_free", "qpsdm_ctx_create", "qpsdm_ctx_destroy", "qpsdm_ctx_set_tables",
     "qpsdm_ctx_build_tables", "qpsdm_ctx_push_su", "qpsdm_ctx_push_traces", "qpsdm_ctx_psdm_kernel",
     "qpsdm_ctx_flush", "qpsdm_ctx_image", "qpsdm_ctx_reset_image", "qpsdm_ctx_traces_pushed",
-    "qpsdm_write_image_su",
+    "qpsdm_write_image_su", "qpsdm_image_batch", "qpsdm_count_batch", "qpsdm_batch_graph_create",
+    "qpsdm_graph_launch", "qpsdm_graph_destroy",
 ]
 
 _lib = None
@@ -178,6 +184,14 @@ def lib():
         L.qpsdm_ctx_traces_pushed.argtypes = [vp]
         L.qpsdm_write_image_su.argtypes = [C.c_char_p, C.c_int, vp, C.c_int, C.c_int, C.c_float, C.c_int,
                                            C.c_float, C.c_int, vp, C.c_int, vp, C.c_short]
+        L.qpsdm_image_batch.argtypes = [C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp, C.c_int,
+                                        vp, C.c_int, vp, C.c_int, vp, C.c_int, vp]
+        L.qpsdm_count_batch.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp]
+        L.qpsdm_batch_graph_create.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, C.c_int,
+                                               C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp,
+                                               C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
+        L.qpsdm_graph_launch.argtypes = [vp, vp]
+        L.qpsdm_graph_destroy.argtypes = [vp]
         _lib = L
     return _lib
 

@@ -144,10 +144,32 @@ def shots_tiny(nshots=2, nrec=6, seed=0x5EED0000, extent=(240.0, 200.0)):
     return out
 
 
+def dip_tans(aperture_deg, contract_deg=3.0):
+    """Dip bounds (tanx1, tanx2, tany1, tany2) of a symmetric opening angle."""
+    t = math.tan(math.radians(aperture_deg - contract_deg))
+    return (-t, t, -t, t)
+
+
+def shots_config4(nshots=500, seed=0x5EED0004, extent=(4000.0, 4000.0)):
+    """Uneven shot gathers: trace counts floor(exp(U[ln 200, ln 4000])) and a
+    per-shot opening angle U[25, 60] deg; shots and receivers uniformly
+    random over the 4 x 4 km patch.  Returns [(sx, sy, g[n, 2], aperture)]."""
+    rng = np.random.default_rng(seed)
+    n = np.floor(np.exp(rng.uniform(math.log(200.0), math.log(4000.0), nshots))).astype(int)
+    ap = rng.uniform(25.0, 60.0, nshots)
+    out = []
+    for s in range(nshots):
+        r = np.random.default_rng((seed, s))
+        sx, sy = r.uniform(0, extent[0]), r.uniform(0, extent[1])
+        g = np.stack([r.uniform(0, extent[0], n[s]), r.uniform(0, extent[1], n[s])], 1)
+        out.append((float(sx), float(sy), g, float(ap[s])))
+    return out
+
+
 def geometry(shots):
     """Per-trace (sx, sy, gx, gy) float32 [ntr, 4] and shot ranges [(tr0, n)]."""
     rows, ranges, t0 = [], [], 0
-    for sx, sy, g in shots:
+    for sx, sy, g, *_ in shots:
         n = g.shape[0]
         a = np.empty((n, 4), np.float32)
         a[:, 0], a[:, 1], a[:, 2], a[:, 3] = sx, sy, g[:, 0], g[:, 1]

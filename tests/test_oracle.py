@@ -318,6 +318,23 @@ def test_image_zero_linear_reciprocal():
     assert n3 == n and np.allclose(i3, i1, rtol=1e-9, atol=1e-9 * np.abs(i1).max())
 
 
+def test_image_skips_rejected_traces():
+    """A NaN-geometry row (a trace the extent check rejected) adds nothing,
+    in the FP64 and FP32 sums and the count."""
+    cfg = synth.tiny()
+    tab, ip = _tiny_case(cfg)
+    geom, _ = synth.geometry(synth.shots_tiny(1, 5))
+    bank = np.random.default_rng(6).standard_normal((geom.shape[0], cfg.K, cfg.nt))
+    bad = geom.copy()
+    bad[2] = np.nan
+    keep = np.array([i for i in range(geom.shape[0]) if i != 2])
+    for fp32 in (False, True):
+        a, na = O.image(bank, bad, tab, ip, fp32=fp32)
+        b, nb = O.image(bank[keep], geom[keep], tab, ip, fp32=fp32)
+        assert na == nb and np.array_equal(a, b)
+    assert O.count(bad, ip) == O.count(geom[keep], ip)
+
+
 def test_count_matches_vectorised_dip_test():
     cfg = synth.tiny()
     tab, ip = _tiny_case(cfg)
