@@ -234,15 +234,77 @@ int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, fl
 int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom_dev, const qpsdm_shot *shots_dev,
                       int nshot, unsigned long long *count_dev, void *stream);
 
+/* As qpsdm_image_batch, with each shot's source leg read from its own
+ * source table (src_dev as written by qpsdm_surrogate_src_tables for the
+ * same shots); the receiver legs use tab_dev.  src_dev == NULL is
+ * qpsdm_image_batch. */
+int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
+                          const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
+                          int table_dtype, const qpsdm_shot *shots_dev, int nshot, const void *src_dev,
+                          unsigned *work_dev, int nwg, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Learned source-leg surrogate (no counterpart in the reference, whose    */
+/* traveltimes come only from the table files, src/q-psdm.cu:2382-2622).  */
+/* MLP 5-128-128-128-128-3 with SiLU, BF16 MFMA operands, FP32 accumulate; */
+/* inputs (sx, sy, x, y, z) mapped from box [lo, hi] to [-1, 1]; outputs    */
+/* (t, t*, log A), A = 1 / int v ds, as residuals (scaled by out_scale)     */
+/* over the model without its anomaly, evaluated in closed form.  Trained  */
+/* on the device by its own forward/backward/Adam kernels.                 */
+/* ---------------------------------------------------------------------- */
+typedef struct qpsdm_surrogate qpsdm_surrogate;
+
+int qpsdm_surrogate_nparams(void);
+int qpsdm_surrogate_create(const qpsdm_model *m, const float box_lo[5], const float box_hi[5],
+                           const float out_scale[3], unsigned long long seed, qpsdm_surrogate **out);
+int qpsdm_surrogate_destroy(qpsdm_surrogate *s);
+/* FP32 master parameters, flattened W1[128][5], b1, W2, b2, W3, b3, W4, b4,
+ * W5[3][128], b5 (host memory). */
+int qpsdm_surrogate_get_params(const qpsdm_surrogate *s, float *host);
+int qpsdm_surrogate_set_params(qpsdm_surrogate *s, const float *host);
+/* n training points uniform in the input box, X [n][5], with exact targets
+ * T [n][3] from the FP64 straight-ray integrals of the full model. */
+int qpsdm_surrogate_samples(const qpsdm_surrogate *s, unsigned long long seed, long long offset, int n,
+                            float *X_dev, float *T_dev, void *stream);
+/* Outputs [n][3] (t, t*, log A) at X [n][5]. */
+int qpsdm_surrogate_eval(const qpsdm_surrogate *s, const float *X_dev, int n, float *out_dev, void *stream);
+/* Loss (mean weighted squared residual error) and its gradient [nparams]
+ * on n points (n a multiple of 64); no update. */
+int qpsdm_surrogate_grad(qpsdm_surrogate *s, const float *X_dev, const float *T_dev, int n, float *grad_dev,
+                         float *loss_host, void *stream);
+/* Adam over `steps` mini-batches of `batch` points taken in order from the
+ * nsamples points (X, T), learning rate cosine-decayed lr0 -> lr1; the loss
+ * of every step into loss_host[steps] when given. */
+int qpsdm_surrogate_train(qpsdm_surrogate *s, const float *X_dev, const float *T_dev, long long nsamples,
+                          int steps, int batch, float lr0, float lr1, float *loss_host, void *stream);
+/* Judge: the surrogate at each node's own source position against that
+ * node's FP32 table, at the table points inside the input box (the only
+ * points source tables query); mask[brick] = 1 where max |dt| <= tol over
+ * the 8^3 brick (node-major, [node][bv][bu][bw]); err_dev [bricks] gets the
+ * max. */
+long long qpsdm_surrogate_mask_bricks(const qpsdm_tgeom *g);
+int qpsdm_surrogate_mask(const qpsdm_surrogate *s, const qpsdm_tgeom *g, const float *tab_dev, float tol,
+                         unsigned char *mask_dev, float *err_dev, void *stream);
+/* Per-shot source tables on the relative sub-grid that reaches the image:
+ * header float4 {sx, sy, u0, v0} per shot, then [nshot][nvs][nus][nw]
+ * float2 (t, t*) from the surrogate, or, where the mask (optional) marks the
+ * nearest node's brick invalid, from that node's table tab_dev (FP32). */
+size_t qpsdm_surrogate_src_bytes(const qpsdm_image_params *ip, int nshot);
+int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_image_params *ip, const float *geom_dev,
+                               const qpsdm_shot *shots_dev, int nshot, const float *tab_dev,
+                               const unsigned char *mask_dev, void *src_dev, void *stream);
+
 /* A captured hipGraph of one batch: Q bank of traces_dev [ntr][nt] into
- * bank_dev, then qpsdm_image_batch.  Replays read the buffers' contents at
+ * bank_dev, then (with a surrogate) the shots' source tables into src_dev,
+ * then the persistent imaging.  Replays read the buffers' contents at
  * launch time, so the caller refills traces/geom/shots between replays. */
 typedef struct qpsdm_graph qpsdm_graph;
 int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float *traces_dev, int ntr, void *bank_dev,
                              int bank_dtype, const qpsdm_image_params *ip, const qpsdm_launch *shape,
                              float *img_dev, const float *geom_dev, const void *tab_dev, int table_dtype,
-                             const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,
-                             qpsdm_graph **out);
+                             const qpsdm_shot *shots_dev, int nshot, const qpsdm_surrogate *sur,
+                             const float *tab_f32_dev, const unsigned char *mask_dev, void *src_dev,
+                             unsigned *work_dev, int nwg, qpsdm_graph **out);
 int qpsdm_graph_launch(qpsdm_graph *g, void *stream);
 int qpsdm_graph_destroy(qpsdm_graph *g);
 

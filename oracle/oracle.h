@@ -92,6 +92,7 @@ void or_trace_spectra(const float *trace, int nt, float dt, const or_window *w,
                       float *re, float *im);
 void or_qbank_f64(const float *traces, int ntr, const or_bparams *p, double *bank);
 void or_qbank_f32(const float *traces, int ntr, const or_bparams *p, float *bank);
+double or_ray_sigma(const or_model *m, double sx, double sy, double x, double y, double z);
 void or_ray_times(const or_model *m, double sx, double sy, double x, double y,
                   double z, double *t, double *tstar);
 void or_build_tables(const or_model *m, const or_tgeom *g, float *tab);

@@ -75,6 +75,8 @@ def lib():
         L.or_qbank_f64.argtypes = [fp, C.c_int, C.POINTER(BParams), dp]
         L.or_qbank_f32.argtypes = [fp, C.c_int, C.POINTER(BParams), fp]
         L.or_ray_times.argtypes = [C.POINTER(Model)] + [C.c_double] * 5 + [dp, dp]
+        L.or_ray_sigma.restype = C.c_double
+        L.or_ray_sigma.argtypes = [C.POINTER(Model)] + [C.c_double] * 5
         L.or_build_tables.argtypes = [C.POINTER(Model), C.POINTER(TGeom), fp]
         L.or_lookup.argtypes = [C.POINTER(TGeom), fp] + [C.c_double] * 5 + [dp, dp]
         L.or_image_f64.restype = C.c_int64
@@ -204,6 +206,10 @@ def ray_times(m, sx, sy, x, y, z):
     t, ts = C.c_double(), C.c_double()
     lib().or_ray_times(C.byref(m), sx, sy, x, y, z, C.byref(t), C.byref(ts))
     return t.value, ts.value
+
+
+def ray_sigma(m, sx, sy, x, y, z):
+    return lib().or_ray_sigma(C.byref(m), sx, sy, x, y, z)
 
 
 def tgeom(**kw):

@@ -368,6 +368,41 @@ void or_ray_times(const or_model *m, double sx, double sy, double x, double y, d
   *tstar = acc_q * L;
 }
 
+/* Spreading integral sigma = int v ds on the same straight segment (the
+ * amplitude target A = 1/sigma of the learned surrogate); closed form
+ * L (v0 + k z / 2) for linear v(z), the same Gauss-Legendre pieces otherwise. */
+double or_ray_sigma(const or_model *m, double sx, double sy, double x, double y, double z) {
+  double hx = x - sx, hy = y - sy;
+  double L = sqrt(hx * hx + hy * hy + z * z);
+  if (m->kind == 0) return L * (m->v0 + 0.5 * m->kgrad * z);
+  double acc = 0;
+  int nb = 0;
+  double lam[20];
+  lam[nb++] = 0.0;
+  for (int l = 1; l < m->nlayer; ++l) {
+    double zb = l * m->layer_dz;
+    if (zb < z) lam[nb++] = zb / z;
+  }
+  lam[nb++] = 1.0;
+  double is2 = 1.0 / (2.0 * m->asig * m->asig);
+  for (int sgi = 0; sgi + 1 < nb; ++sgi) {
+    double a = lam[sgi], b = lam[sgi + 1];
+    if (b <= a) continue;
+    double zm = 0.5 * (a + b) * z;
+    int l = (int)(zm / m->layer_dz);
+    if (l >= m->nlayer) l = m->nlayer - 1;
+    if (l < 0) l = 0;
+    double half = 0.5 * (b - a), mid = 0.5 * (a + b);
+    for (int q = 0; q < 8; ++q) {
+      double la = mid + half * GLX[q];
+      double dx = sx + la * hx - m->ax, dy = sy + la * hy - m->ay, dz = la * z - m->az;
+      double G = exp(-(dx * dx + dy * dy + dz * dz) * is2);
+      acc += GLW[q] * half * m->vl[l] * (1.0 - m->av * G);
+    }
+  }
+  return acc * L;
+}
+
 /* Table set: node (jn, in) sits at (n0x + in*ndx, n0y + jn*ndy); its table
  * holds (t, t*) at relative offsets (u, v) = (u0 + iu*du, v0 + iv*dv) and
  * depth w = iw*dw, layout [node][iv][iu][iw][2] (z fastest, as the

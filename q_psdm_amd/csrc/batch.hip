@@ -28,9 +28,15 @@ namespace {
 
 constexpr int kParts = 8;
 
+// Per-shot runs; with src_hdr set, each shot's source leg comes from its own
+// source table (header {sx, sy, su0, sv0} per shot, then the tables).
 struct ShotRuns {
   const qpsdm_shot *shots;
   float tz;
+  const float4 *src_hdr;
+  const float2 *src_tab;
+  long long src_stride;
+  int nus, nvs;
   __device__ qk::Run operator()(int r) const {
     const qpsdm_shot s = shots[r];
     qk::Run run;
@@ -38,9 +44,30 @@ struct ShotRuns {
     run.ntr = s.ntr;
     run.brow0 = s.tr0;
     run.dip = qk::make_dip(s.tanx1, s.tanx2, s.tany1, s.tany2, tz);
+    run.src = nullptr;
+    run.su0 = run.sv0 = 0.f;
+    run.snu = nus;
+    run.snv = nvs;
+    if (src_hdr) {
+      const float4 h = src_hdr[r];
+      run.src = src_tab + (size_t)r * src_stride;
+      run.su0 = h.z;
+      run.sv0 = h.w;
+    }
     return run;
   }
 };
+
+ShotRuns shot_runs(const qpsdm_image_params *p, const qpsdm_shot *shots, int nshot, const void *src) {
+  ShotRuns R{shots, p->taperzone, nullptr, nullptr, 0, 0, 0};
+  if (src) {
+    qk::src_subgrid(p, &R.nus, &R.nvs);
+    R.src_stride = (long long)R.nvs * R.nus * p->tg.nw;
+    R.src_hdr = (const float4 *)src;
+    R.src_tab = (const float2 *)((const char *)src + (size_t)nshot * 16);
+  }
+  return R;
+}
 
 __global__ void reset_kernel(unsigned *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
@@ -150,16 +177,24 @@ int batch_dispatch(const qpsdm_image_params *p, const qpsdm_launch *shape, const
 
 }  // namespace
 
+extern "C" int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img,
+                                     const float *geom, const void *bank, int bank_dtype, const void *tab,
+                                     int table_dtype, const qpsdm_shot *shots, int nshot, const void *src,
+                                     unsigned *work, int nwg, void *stream) {
+  QPSDM_REQUIRE(p, "image_batch: null params");
+  QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
+  if (nshot == 0) return QPSDM_OK;
+  const Go go{img, (const float4 *)geom, shot_runs(p, shots, nshot, src), nshot, work, nwg, (hipStream_t)stream,
+              true, nullptr};
+  return batch_dispatch(p, shape, bank, bank_dtype, tab, table_dtype, go);
+}
+
 extern "C" int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img,
                                  const float *geom, const void *bank, int bank_dtype, const void *tab,
                                  int table_dtype, const qpsdm_shot *shots, int nshot, unsigned *work,
                                  int nwg, void *stream) {
-  QPSDM_REQUIRE(p, "image_batch: null params");
-  QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
-  if (nshot == 0) return QPSDM_OK;
-  const Go go{img, (const float4 *)geom, ShotRuns{shots, p->taperzone}, nshot, work, nwg, (hipStream_t)stream,
-              true, nullptr};
-  return batch_dispatch(p, shape, bank, bank_dtype, tab, table_dtype, go);
+  return qpsdm_image_batch_src(p, shape, img, geom, bank, bank_dtype, tab, table_dtype, shots, nshot, nullptr,
+                               work, nwg, stream);
 }
 
 extern "C" int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom, const qpsdm_shot *shots,
@@ -172,7 +207,7 @@ extern "C" int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom,
   if (nshot == 0) return QPSDM_OK;
   const int nxt = (P.nx + 7) / 8, nyt = (P.ny + 7) / 8;
   hipLaunchKernelGGL(count_batch_kernel<8>, dim3(nxt * nyt, (P.nz + 7) / 8), dim3(64), 0, (hipStream_t)stream,
-                     P, (const float4 *)geom, ShotRuns{shots, p->taperzone}, nshot, count);
+                     P, (const float4 *)geom, shot_runs(p, shots, nshot, nullptr), nshot, count);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }
@@ -186,8 +221,10 @@ struct qpsdm_graph {
 extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float *traces, int ntr, void *bank,
                                         int bank_dtype, const qpsdm_image_params *ip, const qpsdm_launch *shape,
                                         float *img, const float *geom, const void *tab, int table_dtype,
-                                        const qpsdm_shot *shots, int nshot, unsigned *work, int nwg,
-                                        qpsdm_graph **out) {
+                                        const qpsdm_shot *shots, int nshot, const qpsdm_surrogate *sur,
+                                        const float *tab_f32, const unsigned char *mask, void *src,
+                                        unsigned *work, int nwg, qpsdm_graph **out) {
+  QPSDM_REQUIRE(!sur || src, "batch_graph_create: a surrogate needs the source-table buffer");
   QPSDM_REQUIRE(out && bp && ip && traces && bank && img && geom && tab && shots && work,
                 "batch_graph_create: null args");
   QPSDM_REQUIRE(ntr > 0 && nshot > 0, "batch_graph_create: empty batch");
@@ -195,7 +232,7 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   int rc = qpsdm_qbank_prepare(bp);
   if (rc) return rc;
   if (nwg <= 0) {  // resolve the resident grid size outside the capture
-    const Go q{img, (const float4 *)geom, ShotRuns{shots, ip->taperzone}, nshot, work, 0, nullptr, false, &nwg};
+    const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg};
     rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
     if (rc) return rc;
   }
@@ -212,8 +249,10 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(qpsdm::set_error(QPSDM_EHIP, "batch_graph_create: begin capture failed"));
   rc = qpsdm_qbank(bp, traces, ntr, bank, bank_dtype, s);
+  if (!rc && sur) rc = qpsdm_surrogate_src_tables(sur, ip, geom, shots, nshot, tab_f32, mask, src, s);
   if (!rc)
-    rc = qpsdm_image_batch(ip, shape, img, geom, bank, bank_dtype, tab, table_dtype, shots, nshot, work, nwg, s);
+    rc = qpsdm_image_batch_src(ip, shape, img, geom, bank, bank_dtype, tab, table_dtype, shots, nshot,
+                               sur ? src : nullptr, work, nwg, s);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &graph);
   g->graph = graph;

@@ -25,6 +25,9 @@ inline OneRun one_run(const qk::ImgDev &P) {
   r.run.ntr = P.ntr;
   r.run.brow0 = 0;
   r.run.dip = P.dip;
+  r.run.src = nullptr;
+  r.run.su0 = r.run.sv0 = 0.f;
+  r.run.snu = r.run.snv = 0;
   return r;
 }
 

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace qpsdm_dev {
@@ -267,6 +269,14 @@ __device__ inline int depth_node(const ImgDev &P, float z, float *frac) {
   const int w = min((int)fw, P.nw - 2);
   *frac = fw - (float)w;
   return w;
+}
+
+// Per-shot source table sub-grid: the relative nodes that can reach the
+// image from any source position (nodes at the node tables' spacing).
+inline void src_subgrid(const qpsdm_image_params *p, int *nus, int *nvs) {
+  const qpsdm_tgeom &g = p->tg;
+  *nus = std::min(g.nu, (int)ceil((p->nx - 1) * (double)p->dx / g.du) + 2);
+  *nvs = std::min(g.nv, (int)ceil((p->ny - 1) * (double)p->dy / g.dv) + 2);
 }
 
 struct TileBox {

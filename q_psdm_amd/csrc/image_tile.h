@@ -22,11 +22,41 @@
 namespace qpsdm_dev {
 
 // A run of traces [tr0, tr0 + ntr) of geom; bank row of trace t is
-// t - tr0 + brow0; dip bounds of the run.
+// t - tr0 + brow0; dip bounds of the run.  When src is set, the run's
+// source leg comes from its own table (one node at the source, relative
+// origin (su0, sv0), snv x snu x nw nodes at the node tables' spacing)
+// instead of the nearest node's table.
 struct Run {
   int tr0, ntr, brow0;
   Dip dip;
+  const float2 *src;
+  float su0, sv0;
+  int snu, snv;
 };
+
+// Source leg of the tile points: walk the column c of table tab down the
+// depth nodes; t (floored at tmin), t*, 1/t, 1/ds, ds per point.
+template <int ZP, typename Tab>
+__device__ inline void source_leg(const ImgDev &P, const Tab &tab, const typename Tab::Col &c, const int (&wz)[ZP],
+                                  const float (&fz)[ZP], const float (&z2)[ZP], float hs2, float (&ts)[ZP],
+                                  float (&tss)[ZP], float (&rts)[ZP], float (&rds)[ZP], float (&ds)[ZP]) {
+  int cur = wz[0];
+  float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
+#pragma unroll
+  for (int i = 0; i < ZP; ++i) {
+    while (cur < wz[i]) {
+      ++cur;
+      lo = hi;
+      hi = tab.at(c, cur + 1);
+    }
+    ts[i] = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
+    tss[i] = lo.y + fz[i] * (hi.y - lo.y);
+    rts[i] = __builtin_amdgcn_rcpf(ts[i]);
+    const float d2 = hs2 + z2[i];
+    rds[i] = __builtin_amdgcn_rsqf(d2);
+    ds[i] = d2 * rds[i];
+  }
+}
 
 template <int ZP, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__restrict__ img,
@@ -59,6 +89,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
 
   float ts[ZP], tss[ZP], rts[ZP], rds[ZP], ds[ZP];
   float sx = NAN, sy = NAN, dsx = 0.f, dsy = 0.f;
+  int cache_run = -1;
   const unsigned bank_bytes = (unsigned)(P.K * P.nt * sizeof(BankT));
 
   for (int r = 0; r < nrun; ++r) {
@@ -69,28 +100,26 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       const float4 g = geom[t];
       if (!(g.x == g.x)) continue;  // trace rejected upstream (NaN geometry)
       if (tile_culled(box, g.x, g.y, g.z, g.w)) continue;
-      if (g.x != sx || g.y != sy) {
+      if (g.x != sx || g.y != sy || r != cache_run) {
+        cache_run = r;
         sx = g.x;
         sy = g.y;
         dsx = sx - x;
         dsy = sy - y;
         const float hs2 = dsx * dsx + dsy * dsy;
-        const auto c = column_setup(P, tab, sx, sy, x, y);
-        int cur = wz[0];
-        float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
-#pragma unroll
-        for (int i = 0; i < ZP; ++i) {
-          while (cur < wz[i]) {
-            ++cur;
-            lo = hi;
-            hi = tab.at(c, cur + 1);
-          }
-          ts[i] = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
-          tss[i] = lo.y + fz[i] * (hi.y - lo.y);
-          rts[i] = __builtin_amdgcn_rcpf(ts[i]);
-          const float d2 = hs2 + z2[i];
-          rds[i] = __builtin_amdgcn_rsqf(d2);
-          ds[i] = d2 * rds[i];
+        if (run.src) {
+          TabF32 st;
+          st.p = run.src;
+          st.nstride = (size_t)run.snv * run.snu * P.nw;
+          st.nu = run.snu;
+          st.nw = P.nw;
+          int iu, iv;
+          float a, b;
+          frac_index((x - sx - run.su0) * P.inv_du, run.snu, &iu, &a);
+          frac_index((y - sy - run.sv0) * P.inv_dv, run.snv, &iv, &b);
+          source_leg<ZP>(P, st, st.column(0, iv, iu, a, b), wz, fz, z2, hs2, ts, tss, rts, rds, ds);
+        } else {
+          source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, z2, hs2, ts, tss, rts, rds, ds);
         }
       }
       const float dgx = g.z - x, dgy = g.w - y;

@@ -6,73 +6,11 @@
 
 #include "common.h"
 #include "image_dev.h"
+#include "ray_dev.h"
 
 namespace {
 
-struct ModelDev {
-  int kind;
-  double v0, kgrad, q0;
-  int nlayer;
-  double layer_dz;
-  double vl[16], ql[16];
-  double ax, ay, az, is2, av, aq;
-};
-
-__constant__ double kGLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
-                               -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
-                               0.7966664774136267,  0.9602898564975363};
-__constant__ double kGLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
-                               0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
-                               0.2223810344533745, 0.1012285362903763};
-
-// t = int slowness dl, t* = int slowness/Q dl on the straight segment from
-// (sx, sy, 0) to (x, y, z): closed form for linear v(z), else 8-point
-// Gauss-Legendre on each layer piece.
-__device__ void ray_times(const ModelDev &m, double sx, double sy, double x, double y, double z,
-                          double *t, double *tstar) {
-  const double hx = x - sx, hy = y - sy;
-  const double L = sqrt(hx * hx + hy * hy + z * z);
-  if (m.kind == 0) {
-    double tt;
-    if (m.kgrad * z > 1e-9 * m.v0)
-      tt = L / (m.kgrad * z) * log((m.v0 + m.kgrad * z) / m.v0);
-    else
-      tt = L / m.v0;
-    *t = tt;
-    *tstar = tt / m.q0;
-    return;
-  }
-  double at = 0, aq = 0;
-  double prev = 0.0;
-  for (int l = 0; l < m.nlayer; ++l) {
-    double nxt = 1.0;
-    if (l + 1 < m.nlayer) {
-      const double zb = (l + 1) * m.layer_dz;
-      if (zb < z) nxt = zb / z;
-    }
-    if (nxt > prev) {
-      const double zm = 0.5 * (prev + nxt) * z;
-      int li = (int)(zm / m.layer_dz);
-      li = li >= m.nlayer ? m.nlayer - 1 : (li < 0 ? 0 : li);
-      const double half = 0.5 * (nxt - prev), mid = 0.5 * (prev + nxt);
-      for (int q = 0; q < 8; ++q) {
-        const double la = mid + half * kGLX[q];
-        const double dx = sx + la * hx - m.ax, dy = sy + la * hy - m.ay, dz = la * z - m.az;
-        const double G = exp(-(dx * dx + dy * dy + dz * dz) * m.is2);
-        const double v = m.vl[li] * (1.0 - m.av * G);
-        const double qq = m.ql[li] * (1.0 - m.aq * G);
-        at += kGLW[q] * half / v;
-        aq += kGLW[q] * half / (v * qq);
-      }
-    }
-    prev = nxt;
-    if (nxt >= 1.0) break;
-  }
-  *t = at * L;
-  *tstar = aq * L;
-}
-
-__global__ void build_kernel(ModelDev m, qpsdm_tgeom g, float2 *tab) {
+__global__ void build_kernel(qpsdm_dev::ModelDev m, qpsdm_tgeom g, float2 *tab) {
   const size_t per = (size_t)g.nv * g.nu * g.nw;
   const size_t total = per * g.nnx * g.nny;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -86,8 +24,8 @@ __global__ void build_kernel(ModelDev m, qpsdm_tgeom g, float2 *tab) {
     const double px = g.n0x + (double)(node % g.nnx) * g.ndx;
     const double py = g.n0y + (double)(node / g.nnx) * g.ndy;
     double t, ts;
-    ray_times(m, px, py, px + g.u0 + (double)iu * g.du, py + g.v0 + (double)iv * g.dv,
-              (double)iw * g.dw, &t, &ts);
+    qpsdm_dev::ray_integrals(m, px, py, px + g.u0 + (double)iu * g.du, py + g.v0 + (double)iv * g.dv,
+              (double)iw * g.dw, &t, &ts, nullptr);
     tab[i] = make_float2((float)t, (float)ts);
   }
 }
@@ -160,22 +98,8 @@ extern "C" int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, fl
   int rc = check_geom(g);
   if (rc) return rc;
   QPSDM_REQUIRE(m && tab, "tables_build: null args");
-  QPSDM_REQUIRE(m->kind == 0 || (m->kind == 1 && m->nlayer >= 1 && m->nlayer <= 16 && m->layer_dz > 0),
-                "tables_build: bad model");
-  ModelDev d;
-  d.kind = m->kind;
-  d.v0 = m->v0;
-  d.kgrad = m->kgrad;
-  d.q0 = m->q0;
-  d.nlayer = m->nlayer;
-  d.layer_dz = m->layer_dz;
-  for (int i = 0; i < 16; ++i) d.vl[i] = m->vl[i], d.ql[i] = m->ql[i];
-  d.ax = m->ax;
-  d.ay = m->ay;
-  d.az = m->az;
-  d.is2 = m->asig > 0 ? 1.0 / (2.0 * m->asig * m->asig) : 0.0;
-  d.av = m->av;
-  d.aq = m->aq;
+  QPSDM_REQUIRE(qpsdm_dev::model_ok(m), "tables_build: bad model");
+  const qpsdm_dev::ModelDev d = qpsdm_dev::make_model(*m);
   hipLaunchKernelGGL(build_kernel, dim3(4096), dim3(256), 0, (hipStream_t)stream, d, *g,
                      (float2 *)tab);
   QPSDM_HIP(hipGetLastError());

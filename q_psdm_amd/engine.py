@@ -182,9 +182,10 @@ class Survey:
         tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else getattr(tab, "qpsdm_code", L.QPSDM_F16B)
         return bcode, tcode
 
-    def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, stream=None):
+    def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, src=None, stream=None):
         """Accumulate every shot of a batch into img with the persistent
-        kernel; shots is a ShotList (bank row = geom row)."""
+        kernel; shots is a ShotList (bank row = geom row); src: the shots'
+        source tables from Surrogate.src_tables (None: node tables)."""
         c = self.cfg
         assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
         assert geom.dim() == 2 and geom.shape[1] == 4 and shots.rows <= geom.shape[0]
@@ -193,9 +194,12 @@ class Survey:
         if work is None:
             work = torch.empty(8, dtype=torch.int32, device="cuda")
         lp = L.Launch(*launch) if launch is not None else None
-        check(L.lib().qpsdm_image_batch(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img), _ptr(geom),
-                                        _ptr(bank), bcode, _ptr(tab), tcode, _ptr(shots.dev), shots.n,
-                                        _ptr(work), nwg, _stream(stream)))
+        if src is not None:
+            assert src.numel() >= L.lib().qpsdm_surrogate_src_bytes(C.byref(self.ip), shots.n)
+        check(L.lib().qpsdm_image_batch_src(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img), _ptr(geom),
+                                            _ptr(bank), bcode, _ptr(tab), tcode, _ptr(shots.dev), shots.n,
+                                            _ptr(src) if src is not None else None, _ptr(work), nwg,
+                                            _stream(stream)))
 
     def count_batch(self, geom, shots, stream=None):
         counter = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -230,23 +234,30 @@ class ShotList:
 
 
 class BatchGraph:
-    """A captured graph of [Q bank of traces -> persistent imaging of shots].
-    The buffers are bound at capture; replays read their current contents."""
+    """A captured graph of [Q bank of traces -> (surrogate source tables) ->
+    persistent imaging of shots].  The buffers are bound at capture; replays
+    read their current contents."""
 
-    def __init__(self, survey, traces, bank, img, geom, tab, shots, launch=None, nwg=0):
+    def __init__(self, survey, traces, bank, img, geom, tab, shots, launch=None, nwg=0, surrogate=None,
+                 tab_f32=None, mask=None, src=None):
         c = survey.cfg
         assert traces.dtype == torch.float32 and traces.shape[1] == c.nt and traces.is_contiguous()
         assert bank.shape == (traces.shape[0], c.K, c.nt) and img.shape == (c.ny, c.nx, c.nz)
         assert shots.rows <= traces.shape[0] <= geom.shape[0]
         bcode, tcode = survey._codes(bank, tab)
         self.work = torch.empty(8, dtype=torch.int32, device="cuda")
-        self._keep = (traces, bank, img, geom, tab, shots)
+        if surrogate is not None and src is None:
+            src = surrogate.new_src(survey, shots.n)
+        self.src = src
+        self._keep = (traces, bank, img, geom, tab, shots, surrogate, tab_f32, mask, src)
         lp = L.Launch(*launch) if launch is not None else None
         h = C.c_void_p()
+        opt = lambda t: _ptr(t) if t is not None else None
         torch.cuda.synchronize()
         check(L.lib().qpsdm_batch_graph_create(
             C.byref(survey.bp), _ptr(traces), traces.shape[0], _ptr(bank), bcode, C.byref(survey.ip),
             C.byref(lp) if lp else None, _ptr(img), _ptr(geom), _ptr(tab), tcode, _ptr(shots.dev), shots.n,
+            surrogate.h if surrogate is not None else None, opt(tab_f32), opt(mask), opt(src),
             _ptr(self.work), nwg, C.byref(h)))
         self.h = h
 
@@ -263,6 +274,15 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+def build_tables_into(model, tgeom, out, stream=None):
+    """Straight-ray tables of tgeom (dict) into the float32 tensor out."""
+    g = tgeom_struct(tgeom)
+    n = g.nnx * g.nny * g.nv * g.nu * g.nw * 2
+    assert out.dtype == torch.float32 and out.numel() >= n
+    check(L.lib().qpsdm_tables_build(C.byref(model_struct(model)), C.byref(g), _ptr(out), _stream(stream)))
+    return out
 
 
 def tstar_max_from_tables(tab):
@@ -367,3 +387,116 @@ class Pipeline:
 
     def traces_pushed(self):
         return L.lib().qpsdm_ctx_traces_pushed(self.ctx)
+
+
+class Surrogate:
+    """The learned source leg: MLP 5-128-128-128-128-3 on the device.
+
+    box_lo/box_hi: input box of (sx, sy, x, y, z); out_scale: residual scale
+    of (t, t*, log A) over the anomaly-free closed form."""
+
+    def __init__(self, model, box_lo, box_hi, out_scale, seed=0x5EED0005):
+        self.nparams = L.lib().qpsdm_surrogate_nparams()
+        h = C.c_void_p()
+        check(L.lib().qpsdm_surrogate_create(C.byref(model_struct(model)), (C.c_float * 5)(*box_lo),
+                                             (C.c_float * 5)(*box_hi), (C.c_float * 3)(*out_scale), seed,
+                                             C.byref(h)))
+        self.h = h
+        self.box_lo, self.box_hi, self.out_scale = list(box_lo), list(box_hi), list(out_scale)
+
+    @staticmethod
+    def for_survey(cfg, out_scale=(0.02, 0.004, 0.05), margin=100.0, seed=0x5EED0005):
+        """Input box: sources over the image's surface and the table nodes,
+        points over the image grid widened by margin, depths over the table
+        depth range."""
+        g = cfg.tgeom
+        # the source tables reach up to two table nodes past the image
+        margin = max(margin, 2.5 * max(g["du"], g["dv"]))
+        ex, ey = (cfg.nx - 1) * cfg.dx, (cfg.ny - 1) * cfg.dy
+        zmax = (g["nw"] - 1) * g["dw"]
+        sxl, sxh = min(0.0, g["n0x"]), max(ex, g["n0x"] + (g["nnx"] - 1) * g["ndx"])
+        syl, syh = min(0.0, g["n0y"]), max(ey, g["n0y"] + (g["nny"] - 1) * g["ndy"])
+        lo = (sxl, syl, -margin, -margin, 0.0)
+        hi = (sxh, syh, ex + margin, ey + margin, zmax)
+        return Surrogate(cfg.model, lo, hi, out_scale, seed)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().qpsdm_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        check(L.lib().qpsdm_surrogate_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(L.lib().qpsdm_surrogate_set_params(self.h, p.ctypes.data))
+
+    def samples(self, n, seed, offset=0, stream=None):
+        X = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+        T = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        check(L.lib().qpsdm_surrogate_samples(self.h, seed, offset, n, _ptr(X), _ptr(T), _stream(stream)))
+        return X, T
+
+    def eval(self, X, stream=None):
+        assert X.dtype == torch.float32 and X.dim() == 2 and X.shape[1] == 5
+        out = torch.empty((X.shape[0], 3), dtype=torch.float32, device="cuda")
+        check(L.lib().qpsdm_surrogate_eval(self.h, _ptr(X), X.shape[0], _ptr(out), _stream(stream)))
+        return out
+
+    def grad(self, X, T, stream=None):
+        g = torch.empty(self.nparams, dtype=torch.float32, device="cuda")
+        loss = C.c_float()
+        check(L.lib().qpsdm_surrogate_grad(self.h, _ptr(X), _ptr(T), X.shape[0], _ptr(g), C.byref(loss),
+                                           _stream(stream)))
+        return g, loss.value
+
+    def train(self, X, T, steps, batch, lr0=2e-3, lr1=2e-5, stream=None):
+        hist = np.zeros(max(steps, 1), np.float32)
+        check(L.lib().qpsdm_surrogate_train(self.h, _ptr(X), _ptr(T), X.shape[0], steps, batch, lr0, lr1,
+                                            hist.ctypes.data, _stream(stream)))
+        return hist[:steps]
+
+    def mask(self, tg, tab, tol=0.5e-3, stream=None):
+        """Validity per (node, 8^3 brick) against the FP32 node tables:
+        (mask uint8, max |dt| per brick float32), both on the device."""
+        nb = L.lib().qpsdm_surrogate_mask_bricks(C.byref(tg))
+        m = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        e = torch.empty(nb, dtype=torch.float32, device="cuda")
+        check(L.lib().qpsdm_surrogate_mask(self.h, C.byref(tg), _ptr(tab), tol, _ptr(m), _ptr(e),
+                                           _stream(stream)))
+        return m, e
+
+    def new_src(self, survey, nshot):
+        nb = L.lib().qpsdm_surrogate_src_bytes(C.byref(survey.ip), nshot)
+        return torch.empty(nb, dtype=torch.uint8, device="cuda")
+
+    def src_tables(self, survey, geom, shots, tab=None, mask=None, out=None, stream=None):
+        """Source tables of the shots (header + tables, one uint8 buffer)."""
+        if out is None:
+            out = self.new_src(survey, shots.n)
+        assert tab is None or tab.dtype == torch.float32
+        assert geom.dim() == 2 and geom.shape[1] == 4 and shots.rows <= geom.shape[0]
+        assert out.numel() >= L.lib().qpsdm_surrogate_src_bytes(C.byref(survey.ip), shots.n)
+        check(L.lib().qpsdm_surrogate_src_tables(self.h, C.byref(survey.ip), _ptr(geom), _ptr(shots.dev),
+                                                 shots.n, _ptr(tab) if tab is not None else None,
+                                                 _ptr(mask) if mask is not None else None, _ptr(out),
+                                                 _stream(stream)))
+        return out
+
+    def src_grid(self, survey):
+        """(nus, nvs, nw) of one shot's source table."""
+        c = survey.cfg
+        g = c.tgeom
+        nus = min(g["nu"], math.ceil((c.nx - 1) * c.dx / g["du"]) + 2)
+        nvs = min(g["nv"], math.ceil((c.ny - 1) * c.dy / g["dv"]) + 2)
+        return nus, nvs, g["nw"]

@@ -110,7 +110,11 @@ EXPORTS = [
     "qpsdm_ctx_build_tables", "qpsdm_ctx_push_su", "qpsdm_ctx_push_traces", "qpsdm_ctx_psdm_kernel",
     "qpsdm_ctx_flush", "qpsdm_ctx_image", "qpsdm_ctx_reset_image", "qpsdm_ctx_traces_pushed",
     "qpsdm_write_image_su", "qpsdm_image_batch", "qpsdm_count_batch", "qpsdm_batch_graph_create",
-    "qpsdm_graph_launch", "qpsdm_graph_destroy",
+    "qpsdm_graph_launch", "qpsdm_graph_destroy", "qpsdm_image_batch_src",
+    "qpsdm_surrogate_nparams", "qpsdm_surrogate_create", "qpsdm_surrogate_destroy",
+    "qpsdm_surrogate_get_params", "qpsdm_surrogate_set_params", "qpsdm_surrogate_samples",
+    "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
+    "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables",
 ]
 
 _lib = None
@@ -189,7 +193,26 @@ def lib():
         L.qpsdm_count_batch.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp]
         L.qpsdm_batch_graph_create.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, C.c_int,
                                                C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp,
-                                               C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
+                                               C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.POINTER(vp)]
+        L.qpsdm_image_batch_src.argtypes = [C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp, C.c_int,
+                                            vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]
+        f5 = C.c_float * 5
+        L.qpsdm_surrogate_create.argtypes = [C.POINTER(Model), f5, f5, C.c_float * 3, C.c_ulonglong,
+                                             C.POINTER(vp)]
+        L.qpsdm_surrogate_destroy.argtypes = [vp]
+        L.qpsdm_surrogate_get_params.argtypes = [vp, vp]
+        L.qpsdm_surrogate_set_params.argtypes = [vp, vp]
+        L.qpsdm_surrogate_samples.argtypes = [vp, C.c_ulonglong, C.c_longlong, C.c_int, vp, vp, vp]
+        L.qpsdm_surrogate_eval.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.qpsdm_surrogate_grad.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_float), vp]
+        L.qpsdm_surrogate_train.argtypes = [vp, vp, vp, C.c_longlong, C.c_int, C.c_int, C.c_float, C.c_float,
+                                            vp, vp]
+        L.qpsdm_surrogate_mask_bricks.restype = C.c_longlong
+        L.qpsdm_surrogate_mask_bricks.argtypes = [C.POINTER(TGeom)]
+        L.qpsdm_surrogate_mask.argtypes = [vp, C.POINTER(TGeom), vp, C.c_float, vp, vp, vp]
+        L.qpsdm_surrogate_src_bytes.restype = C.c_size_t
+        L.qpsdm_surrogate_src_bytes.argtypes = [C.POINTER(ImageParams), C.c_int]
+        L.qpsdm_surrogate_src_tables.argtypes = [vp, C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp, vp]
         L.qpsdm_graph_launch.argtypes = [vp, vp]
         L.qpsdm_graph_destroy.argtypes = [vp]
         _lib = L

@@ -84,6 +84,17 @@ def config2():
     return c
 
 
+def config5(seed=0x5EED0005):
+    """Config 2's layers and anomaly with each layer's Q drawn from U[30, 200]
+    (seeded): the model the learned surrogate is trained on and images with."""
+    c = config2()
+    c.name = "config5"
+    c.seed = seed
+    q = np.random.default_rng(seed).uniform(30.0, 200.0, len(c.model["vl"]))
+    c.model = dict(c.model, ql=[float(v) for v in q], q0=float(q[0]))
+    return c
+
+
 def tiny():
     """Small v(z) case that the FP64 oracle images in well under a second."""
     c = Config("tiny", 24, 20, 30, 10.0, 10.0, 10.0, 501, 0.002, 8, f1=3.0, seed=0x5EED0000)
