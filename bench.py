@@ -1,20 +1,39 @@
 """Benchmark of the Q-compensated Kirchhoff PSDM imaging path on one MI355X.
 
-Workload: the 401 x 401 x 301 (10 m) layered + anomaly survey, 200 shots x
-2000 jittered receivers, 2001 samples at 2 ms, K = 32 t* levels, 25 m
-tables.  One step = one shot: the Q filter bank of its 2000 traces plus
-the Kirchhoff sum of those traces into the full image, with traces, tables
-and image already resident in HBM.  value = image-point x trace updates/s
-(updates = pairs inside the dip aperture, counted once per shot outside the
-timed region by the count kernel).
+Workload: the 401 x 401 x 301 image at 10 m, 2000 jittered receivers per
+shot over 4 x 4 km, 2001 samples at 2 ms, K = 32 t* levels, 25 m tables
+(5 x 5 surface nodes), on the layered + Gaussian-anomaly model with each
+layer's Q drawn from U[30, 200] (the surrogate variant of that survey).
+A step images `--shots-per-step` shots (Q bank of their traces + the
+Kirchhoff sum into the full image), inputs resident in HBM.  Three forms
+run the same shots:
 
-The cpu_baseline leg times the CPU oracle's FP32 path (Q bank + imaging
-sum, OpenMP) on a bounded sample of the same workload on this host.
+  fp32   FP32 bank, FP32 nearest-node tables, one launch per shot
+  mixed  BF16 bank, 16-bit fixed-point brick tables, the persistent batched
+         kernel under a captured hipGraph
+  fast   mixed + the learned source leg: per-shot source tables from the
+         BF16-MFMA surrogate (trained on the device first, table fallback
+         where it misses by more than 0.5 ms), inside the same graph
+
+value = the fast form's image-point x trace updates/s (updates = pairs
+inside the dip aperture, counted by the count kernel outside the timed
+region).  The fp32 and mixed forms, their kernel rooflines, the image
+differences between forms, the surrogate's training and accuracy, and a
+CPU baseline (the oracle's FP32 path on a bounded sample, OpenMP) are
+reported in the same JSON line.  HBM traffic of the imaging kernel comes
+from two rocprofv3 counter runs of this script (FETCH_SIZE, WRITE_SIZE),
+scaled by what the same counters read for a copy kernel of known size.
 """
 import argparse
+import csv
+import glob
 import json
+import math
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -23,155 +42,418 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"fp32": 72.0, "mixed": 36.0}
-BANK_DTYPE = {"fp32": "f32", "mixed": "bf16"}
+# algorithmic bytes per update: the bank sample pair + the receiver leg's
+# eight trilinear (t, t*) corners
+BYTES_PER_UPDATE = {"fp32": 2 * 4 + 8 * 2 * 4, "mixed": 2 * 2 + 8 * 2 * 2, "fast": 2 * 2 + 8 * 2 * 2}
+KERNEL_NAME = {"fp32": "image_kernel", "mixed": "image_batch_kernel", "fast": "image_batch_kernel"}
+CALIB_MIB = 512
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", choices=["fp32", "mixed"], default="fp32")
-    ap.add_argument("--ztile", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-traces", type=int, default=256)
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="device copy of this many MiB before the steps (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
-                          "in DESIGN.md and not run", "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import torch
-
-    from q_psdm_amd import engine, synth
-
-    cfg = synth.config2()
-    nshot = args.warmup + args.steps
-    t0 = time.time()
-    probe = engine.Survey(cfg, 0.05)
-    tab = probe.build_tables()
-    torch.cuda.synchronize()
-    sv = engine.Survey(cfg, engine.tstar_max_from_tables(tab))
-    log(f"tables built: {tuple(tab.shape)} {tab.numel() * 4 / 1e9:.2f} GB in {time.time() - t0:.1f}s; "
-        f"dtstar={sv.dtstar * 1e3:.3f} ms nsb={sv.win.nsb} nww={sv.win.nww}")
-    if args.form == "mixed":
-        tab_use = sv.quantize_tables(tab)
-        bank_dtype = torch.bfloat16
-    else:
-        tab_use = tab
-        bank_dtype = torch.float32
-
-    shots = synth.shots_config2(nshots=nshot)
-    geom, ranges = synth.geometry(shots)
-    g_d = torch.from_numpy(geom).cuda()
-    t0 = time.time()
-    traces = [torch.from_numpy(synth.reflectivity_traces(cfg, n, seed=(cfg.seed, s))).cuda()
-              for s, (_, n) in enumerate(ranges)]
-    log(f"traces generated for {nshot} shots in {time.time() - t0:.1f}s")
-    counts = [sv.count_shot(g_d, tr0, n) for tr0, n in ranges]
-    torch.cuda.synchronize()
-    log(f"updates per shot: {counts}")
-
-    if args.calib_copy_mib:
-        a = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
-        a.fill_(1.0)
-        c = torch.empty_like(a)
+class Workload:
+    def __init__(self, args, torch, engine, synth):
+        self.cfg = cfg = synth.config5()
+        self.S = args.shots_per_step
+        self.nstep = args.warmup + args.steps
+        nshot = self.nstep * self.S
+        t0 = time.time()
+        probe = engine.Survey(cfg, 0.05)
+        self.tab = probe.build_tables()
         torch.cuda.synchronize()
-        c.copy_(a)
+        self.sv = engine.Survey(cfg, engine.tstar_max_from_tables(self.tab))
+        log(f"tables {tuple(self.tab.shape)} {self.tab.numel() * 4 / 1e9:.2f} GB in {time.time() - t0:.1f}s; "
+            f"dtstar={self.sv.dtstar * 1e3:.3f} ms nsb={self.sv.win.nsb} nww={self.sv.win.nww}")
+        self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16")
+        shots = synth.shots_config2(nshots=nshot)
+        self.geom, self.ranges = synth.geometry(shots)
+        self.g_d = torch.from_numpy(self.geom).cuda()
+        t0 = time.time()
+        self.traces = [torch.from_numpy(synth.reflectivity_traces(cfg, n, seed=(cfg.seed, s))).cuda()
+                       for s, (_, n) in enumerate(self.ranges)]
+        log(f"traces for {nshot} shots in {time.time() - t0:.1f}s")
+        self.tans = cfg.dip_bounds()[:4]
+        self.counts = [self.sv.count_shot(self.g_d, tr0, n) for tr0, n in self.ranges]
         torch.cuda.synchronize()
-        del a, c
+        log(f"updates per shot: {self.counts}")
+        # per step: concatenated traces, geometry rows re-based to 0, shot list
+        self.steps = []
+        for k in range(self.nstep):
+            idx = list(range(k * self.S, (k + 1) * self.S))
+            tr = torch.cat([self.traces[i] for i in idx], 0).contiguous()
+            rows = np.concatenate([self.geom[self.ranges[i][0]:self.ranges[i][0] + self.ranges[i][1]] for i in idx])
+            desc, t = [], 0
+            for i in idx:
+                desc.append((t, self.ranges[i][1]) + tuple(self.tans))
+                t += self.ranges[i][1]
+            self.steps.append(dict(idx=idx, traces=tr, geom=torch.from_numpy(np.ascontiguousarray(rows)).cuda(),
+                                   shots=engine.ShotList(desc), updates=sum(self.counts[i] for i in idx)))
+
+
+def time_form(form, W, args, torch, engine, sur=None, mask=None):
+    """Runs warmup + timed steps of one form; returns (image, stats)."""
+    sv, cfg = W.sv, W.cfg
     img = sv.new_image()
-    bank = torch.empty((max(n for _, n in ranges), cfg.K, cfg.nt), dtype=bank_dtype, device="cuda")
-    launch = (8, 8, args.ztile, 1)
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nshot)]
+    launch = (args.tile_x, args.tile_y, args.ztile, 0)
+    ev = []
+    if form == "fp32":
+        bank = torch.empty((max(n for _, n in W.ranges), cfg.K, cfg.nt), dtype=torch.float32, device="cuda")
 
-    def step(s):
-        tr0, n = ranges[s]
-        e = ev[s]
-        e[0].record()
-        sv.qbank(traces[s], out=bank[:n])
-        e[1].record()
-        sv.image_shot(img, g_d, tr0, n, bank, tab_use, launch=launch)
-        e[2].record()
+        def run(k):
+            es = []
+            for i in W.steps[k]["idx"]:
+                tr0, n = W.ranges[i]
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                e[0].record()
+                sv.qbank(W.traces[i], out=bank[:n])
+                e[1].record()
+                sv.image_shot(img, W.g_d, tr0, n, bank, W.tab, launch=launch)
+                e[2].record()
+                es.append(e)
+            ev.append(es)
+    else:
+        # per step two captured graphs on one stream: [Q bank -> source tables]
+        # and [counter reset -> persistent imaging], HIP events around the second
+        graphs = []
+        for k in range(W.nstep):
+            st = W.steps[k]
+            bank = torch.empty((st["traces"].shape[0], cfg.K, cfg.nt), dtype=torch.bfloat16, device="cuda")
+            su = sur if form == "fast" else None
+            g_pre = engine.BatchGraph(sv, st["traces"], bank, img, st["geom"], W.tab_u16, st["shots"], launch=launch,
+                                      surrogate=su, tab_f32=W.tab, mask=mask, stages=3)
+            g_img = engine.BatchGraph(sv, st["traces"], bank, img, st["geom"], W.tab_u16, st["shots"], launch=launch,
+                                      surrogate=su, tab_f32=W.tab, mask=mask, src=g_pre.src, stages=4)
+            graphs.append((g_pre, g_img, bank))
 
-    for s in range(args.warmup):
-        step(s)
+        def run(k):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record()
+            graphs[k][0].launch()
+            e[1].record()
+            graphs[k][1].launch()
+            e[2].record()
+            ev.append([e])
+
+    for k in range(args.warmup):
+        run(k)
     torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for s in range(args.warmup, nshot):
-        step(s)
+    ev.clear()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, W.nstep):
+        run(k)
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    upd = sum(counts[args.warmup:])
-    value = upd / elapsed
-    bank_ms = [ev[s][0].elapsed_time(ev[s][1]) for s in range(args.warmup, nshot)]
-    img_ms = [ev[s][1].elapsed_time(ev[s][2]) for s in range(args.warmup, nshot)]
-    avg_img_s = float(np.mean(img_ms)) / 1e3
-    upd_per_launch = upd / args.steps
-    achieved = upd_per_launch * BYTES_PER_UPDATE[args.form] / avg_img_s / 1e9
-    bank_bytes = [ranges[s][1] * (4 * cfg.nt + cfg.K * cfg.nt * (4 if args.form == "fp32" else 2))
-                  for s in range(args.warmup, nshot)]
-    bank_gbs = float(np.mean([b / (m / 1e3) / 1e9 for b, m in zip(bank_bytes, bank_ms)]))
-    log(f"bank ms {['%.2f' % x for x in bank_ms]}  image ms {['%.1f' % x for x in img_ms]}")
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(cfg, sv, tab, geom, ranges, traces, args.cpu_traces)
-
-    out = {
-        "metric": "image-point x trace updates/s @1 GPU",
-        "value": value,
-        "unit": "updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if args.form == "fp32" else "f32-accum/bf16-bank/f16-table",
-        "data": "synthetic (seeded layered+anomaly model, jittered geometry, reflectivity traces)",
-        "config": {"workload": "config2: image 401x401x301 @10 m, 200 shots x 2000 receivers, "
-                               "nt=2001 @2 ms, K=32, tables 25 m; step = one shot",
-                   "form": args.form, "tile": list(launch), "updates_per_step": upd_per_launch},
-        "roofline": {"bound": "hbm", "kernel": "image_kernel", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "algorithmic_bytes_per_update": BYTES_PER_UPDATE[args.form],
-                     "avg_launch_ms": avg_img_s * 1e3},
-        "qbank": {"avg_launch_ms": float(np.mean(bank_ms)), "achieved_GBs": bank_gbs,
-                  "frac": bank_gbs / HBM_PEAK_GBS, "bytes_per_trace": bank_bytes[0] / ranges[-1][1]},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    elapsed = time.perf_counter() - t0
+    upd = sum(W.steps[k]["updates"] for k in range(args.warmup, W.nstep))
+    if form == "fp32":
+        img_ms = [e[1].elapsed_time(e[2]) for es in ev for e in es]
+        bank_ms = [e[0].elapsed_time(e[1]) for es in ev for e in es]
+        launches = len(img_ms)
+    else:
+        img_ms = [es[0][1].elapsed_time(es[0][2]) for es in ev]
+        pre_ms = [es[0][0].elapsed_time(es[0][1]) for es in ev]
+        bank_ms = None
+        launches = len(img_ms)
+    avg_img = float(np.mean(img_ms)) / 1e3
+    achieved = upd / launches * BYTES_PER_UPDATE[form] / avg_img / 1e9
+    stats = {"value": upd / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+             "imaging_kernel": KERNEL_NAME[form], "imaging_avg_launch_ms": avg_img * 1e3,
+             "updates_per_launch": upd / launches, "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
+             "imaging_share_of_step": sum(img_ms) / (elapsed * 1e3)}
+    if form != "fp32":
+        stats["bank_and_source_tables_ms"] = float(np.mean(pre_ms))
+    if bank_ms:
+        nb = W.ranges[0][1]
+        stats["qbank_avg_launch_ms"] = float(np.mean(bank_ms))
+        stats["qbank_GBs"] = nb * (4 * cfg.nt + cfg.K * cfg.nt * 4) / (float(np.mean(bank_ms)) / 1e3) / 1e9
+    log(f"[{form}] {stats['value']:.3e} updates/s, {stats['ms_per_step']:.1f} ms/step, imaging "
+        f"{avg_img * 1e3:.1f} ms/launch ({stats['imaging_share_of_step'] * 100:.0f}% of the step)")
+    if form != "fp32":
+        for g_pre, g_img, _ in graphs:
+            g_img.close()
+            g_pre.close()
+    return img, stats
 
 
-def cpu_baseline(cfg, sv, tab, geom, ranges, traces, ntr):
-    """Oracle FP32 path (bank + sum) on the first ntr traces of shot 0."""
+def train_surrogate(W, args, torch, engine):
+    cfg = W.cfg
+    sur = engine.Surrogate.for_survey(cfg, out_scale=(0.02, 0.004, 0.05))
+    t0 = time.time()
+    X, T = sur.samples(args.train_samples, seed=0x5EED5)
+    hist = sur.train(X, T, steps=args.train_steps, batch=args.train_batch, lr0=2e-3, lr1=2e-5)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    del X, T
+    Xv, Tv = sur.samples(1 << 18, seed=0x5EED6, offset=1 << 40)
+    d = (sur.eval(Xv) - Tv).abs()
+    mask, err = sur.mask(W.sv.tg, W.tab, tol=0.5e-3)
+    torch.cuda.synchronize()
+    info = {"train_seconds": t_train, "samples": args.train_samples, "steps": args.train_steps,
+            "batch": args.train_batch, "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-20:].mean()),
+            "heldout_dt_rms_ms": float((d[:, 0] ** 2).mean().sqrt()) * 1e3,
+            "heldout_dt_max_ms": float(d[:, 0].max()) * 1e3,
+            "heldout_dtstar_max_ms": float(d[:, 1].max()) * 1e3,
+            "fallback_brick_fraction": 1.0 - float(mask.float().mean())}
+    del Xv, Tv, d, err
+    log(f"surrogate: trained {args.train_steps} steps in {t_train:.1f}s, loss {info['loss_first']:.3e} -> "
+        f"{info['loss_last']:.3e}; held-out |dt| rms {info['heldout_dt_rms_ms']:.3f} ms max "
+        f"{info['heldout_dt_max_ms']:.2f} ms; {info['fallback_brick_fraction'] * 100:.1f}% bricks fall back")
+    return sur, mask, info
+
+
+def source_leg_gain(W, sur, mask, torch, engine):
+    """Source tables of the first step's shots: surrogate (with fallback),
+    nearest node, and exact (FP64 ray integrals at the true source)."""
+    st = W.steps[0]
+    sv, cfg = W.sv, W.cfg
+    nb = mask.numel()
+    src_s = sur.src_tables(sv, st["geom"], st["shots"], tab=W.tab, mask=mask)
+    src_n = sur.src_tables(sv, st["geom"], st["shots"], tab=W.tab, mask=torch.zeros(nb, dtype=torch.uint8,
+                                                                                   device="cuda"))
+    n = st["shots"].n
+    nus, nvs, nw = sur.src_grid(sv)
+    hdr = src_s[:16 * n].view(torch.float32).view(n, 4).cpu().numpy()
+    exact = src_s.clone()
+    body = exact[16 * n:].view(torch.float32)
+    per = nvs * nus * nw * 2
+    for k in range(n):
+        tg = dict(cfg.tgeom, nnx=1, nny=1, n0x=float(hdr[k, 0]), n0y=float(hdr[k, 1]), nu=nus, nv=nvs,
+                  u0=float(hdr[k, 2]), v0=float(hdr[k, 3]))
+        engine.build_tables_into(cfg.model, tg, body[k * per:(k + 1) * per])
+    view = lambda b: b[16 * n:].view(torch.float32).view(n, nvs, nus, nw, 2)
+    es = (view(src_s) - view(exact))[..., 0].abs()
+    en = (view(src_n) - view(exact))[..., 0].abs()
+    return exact, {"src_dt_mean_ms_surrogate": float(es.mean()) * 1e3, "src_dt_max_ms_surrogate": float(es.max()) * 1e3,
+                   "src_dt_mean_ms_nearest_node": float(en.mean()) * 1e3,
+                   "src_dt_max_ms_nearest_node": float(en.max()) * 1e3}
+
+
+def image_exact_source(W, exact_src0, sur, torch, engine):
+    """FP32 image of the first step's shots with exact source tables (the
+    reference the surrogate approximates)."""
+    st = W.steps[0]
+    sv, cfg = W.sv, W.cfg
+    img = sv.new_image()
+    bank = sv.qbank(st["traces"])
+    sv.image_batch(img, st["geom"], bank, W.tab, st["shots"], src=exact_src0)
+    return img
+
+
+def first_step_image(W, form, sur, mask, torch, engine):
+    st = W.steps[0]
+    sv = W.sv
+    img = sv.new_image()
+    if form == "fp32":
+        bank = sv.qbank(st["traces"])
+        sv.image_batch(img, st["geom"], bank, W.tab, st["shots"])
+    else:
+        bank = sv.qbank(st["traces"], dtype=torch.bfloat16)
+        src = sur.src_tables(sv, st["geom"], st["shots"], tab=W.tab, mask=mask) if form == "fast" else None
+        sv.image_batch(img, st["geom"], bank, W.tab_u16, st["shots"], src=src)
+    return img
+
+
+def rel_l2(a, b):
+    a = a.double()
+    b = b.double()
+    return float((a - b).norm() / b.norm().clamp_min(1e-300))
+
+
+def cpu_baseline(W, ntr):
+    """The oracle's FP32 path (FP32 Q bank + FP32 sum, OpenMP over image
+    lines) on the first ntr traces of shot 0, on this host."""
     from oracle import oracle as O
     from tests.helpers import o_bparams, o_iparams
 
-    tr0, _ = ranges[0]
-    g = np.ascontiguousarray(geom[tr0:tr0 + ntr])
-    x = traces[0][:ntr].cpu().numpy()
-    t_host = tab.cpu().numpy()
+    sv = W.sv
+    tr0, _ = W.ranges[0]
+    g = np.ascontiguousarray(W.geom[tr0:tr0 + ntr])
+    x = W.traces[0][:ntr].cpu().numpy()
+    t_host = W.tab.cpu().numpy()
     t0 = time.perf_counter()
     b = O.qbank(x, o_bparams(sv), dtype=np.float32)
     t1 = time.perf_counter()
     _, n = O.image(b, g, t_host, o_iparams(sv), fp32=True)
     t2 = time.perf_counter()
     log(f"cpu baseline: bank {t1 - t0:.1f}s image {t2 - t1:.1f}s updates {n}")
-    return {"value": n / (t2 - t0), "unit": "updates/s", "cores": O.num_threads(),
-            "kind": "port", "sample": f"shot 0, first {ntr} traces, full 401x401x301 image "
-                                       f"(FP32 Q bank + FP32 sum, OpenMP)",
-            "seconds": t2 - t0, "image_seconds": t2 - t1}
+    return {"value": n / (t2 - t0), "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"shot 0, first {ntr} of its traces, full 401x401x301 image (FP32 Q bank + FP32 "
+                      f"Kirchhoff sum, OpenMP over image lines)", "seconds": t2 - t0}
+
+
+# ---- HBM traffic from counter runs --------------------------------------------
+
+def probe(args):
+    """Child of a rocprofv3 counter run: one step of the fast form with an
+    untrained surrogate (the imaging kernel's traffic does not depend on the
+    weights) and a calibration copy of CALIB_MIB MiB."""
+    import torch
+
+    from q_psdm_amd import engine, synth
+    args.warmup, args.steps = 0, 1
+    W = Workload(args, torch, engine, synth)
+    a = torch.ones(CALIB_MIB << 18, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    engine.copy_f32(a, c)
+    torch.cuda.synchronize()
+    del a, c
+    sur = engine.Surrogate.for_survey(W.cfg, out_scale=(0.02, 0.004, 0.05))
+    mask = torch.zeros(engine.L.lib().qpsdm_surrogate_mask_bricks(engine.C.byref(W.sv.tg)), dtype=torch.uint8,
+                       device="cuda")
+    time_form(args.probe, W, args, torch, engine, sur=sur, mask=mask)
+
+
+def counter_run(counter, form, args, tmp):
+    rp = shutil.which("rocprofv3")
+    if not rp:
+        return None, "rocprofv3 not found"
+    d = os.path.join(tmp, counter)
+    cmd = [rp, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "probe", "--", sys.executable,
+           os.path.join(ROOT, "bench.py"), "--probe", form, "--shots-per-step", str(args.shots_per_step),
+           "--ztile", str(args.ztile), "--tile-x", str(args.tile_x), "--tile-y", str(args.tile_y)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        # the child's progress goes to this process's stderr (a long run stays visibly alive)
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=sys.stderr, stderr=sys.stderr,
+                           timeout=args.probe_timeout)
+    except subprocess.TimeoutExpired:
+        return None, f"{counter} run timed out"
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        return None, f"{counter} run rc={r.returncode}, no counter file"
+    rows = list(csv.DictReader(open(files[0])))
+    kern = [float(x["Counter_Value"]) for x in rows
+            if KERNEL_NAME[form] in x["Kernel_Name"] and "count" not in x["Kernel_Name"]]
+    copy = [float(x["Counter_Value"]) for x in rows
+            if x["Kernel_Name"].startswith("(anonymous namespace)::copy_kernel")]
+    if not kern or not copy:
+        return None, f"{counter}: kernel rows {len(kern)}, copy rows {len(copy)}"
+    return (float(np.mean(kern)), copy[0]), None
+
+
+def measure_traffic(form, args):
+    tmp = tempfile.mkdtemp(prefix="qpsdm_pmc_", dir="/tmp")
+    try:
+        out = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            res, err = counter_run(counter, form, args, tmp)
+            if err:
+                log(f"traffic: {err}")
+                return None, err
+            kib, copy_kib = res
+            # the copy kernel reads and writes CALIB_MIB MiB exactly
+            factor = copy_kib / (CALIB_MIB * 1024.0)
+            out[counter] = {"kernel_KiB": kib, "copy_KiB": copy_kib, "factor": factor,
+                            "bytes": kib * 1024.0 / factor}
+            log(f"traffic: {counter} kernel {kib:.0f} KiB, copy {copy_kib:.0f} KiB (factor {factor:.3f})")
+        return out, None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots-per-step", type=int, default=2)
+    ap.add_argument("--forms", default="fp32,mixed,fast")
+    ap.add_argument("--ztile", type=int, default=4)
+    ap.add_argument("--tile-x", type=int, default=8)
+    ap.add_argument("--tile-y", type=int, default=8)
+    ap.add_argument("--train-samples", type=int, default=20_000_000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-batch", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-traces", type=int, default=256)
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--probe", choices=["fp32", "mixed", "fast"], default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-timeout", type=int, default=240)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; how the path would shard is described in "
+                          "DESIGN.md and not run", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.probe:
+        probe(args)
+        return
+
+    import torch
+
+    from q_psdm_amd import engine, synth
+    forms = [f for f in args.forms.split(",") if f]
+    W = Workload(args, torch, engine, synth)
+    sur = mask = sinfo = None
+    if "fast" in forms:
+        sur, mask, sinfo = train_surrogate(W, args, torch, engine)
+    results, images = {}, {}
+    for f in forms:
+        images[f], results[f] = time_form(f, W, args, torch, engine, sur=sur, mask=mask)
+    torch.cuda.synchronize()
+    errors = {}
+    if "fp32" in images and "mixed" in images:
+        errors["mixed_vs_fp32"] = rel_l2(images["mixed"], images["fp32"])
+    if "fast" in images and "mixed" in images:
+        errors["fast_vs_mixed"] = rel_l2(images["fast"], images["mixed"])
+    if "fast" in images and "fp32" in images:
+        errors["fast_vs_fp32"] = rel_l2(images["fast"], images["fp32"])
+    if sur is not None:
+        exact0, gain = source_leg_gain(W, sur, mask, torch, engine)
+        sinfo.update(gain)
+        ref = image_exact_source(W, exact0, sur, torch, engine)
+        for f in ("fp32", "fast"):
+            if f in forms:
+                errors[f"{f}_vs_exact_source_first_step"] = rel_l2(first_step_image(W, f, sur, mask, torch, engine),
+                                                                   ref)
+        del exact0, ref
+    log(f"image differences (rel-L2): {errors}")
+    head = "fast" if "fast" in results else forms[-1]
+    traffic = None
+    if not args.no_traffic:
+        del images
+        torch.cuda.empty_cache()
+        t, err = measure_traffic(head, args)
+        if t:
+            traffic = t["FETCH_SIZE"]["bytes"] + t["WRITE_SIZE"]["bytes"]
+            results[head]["traffic_detail"] = t
+        else:
+            results[head]["traffic_note"] = err
+    cpu = None if args.no_cpu else cpu_baseline(W, args.cpu_traces)
+    R = results[head]
+    out = {
+        "metric": "image-point x trace updates/s @1 GPU (FP32 vs mixed+surrogate+batched)",
+        "value": R["value"],
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fast": "f32 accumulate; bf16 bank; u16 offset+scale tables; bf16-mfma surrogate",
+                  "mixed": "f32 accumulate; bf16 bank; u16 offset+scale tables", "fp32": "f32"}[head],
+        "data": "synthetic: seeded layered + anomaly model (layer Q ~ U[30,200]), jittered geometry, "
+                "band-limited reflectivity traces with 5% noise",
+        "config": {"workload": "401x401x301 image @10 m; 2000 receivers/shot over 4x4 km; nt=2001 @2 ms; K=32; "
+                               "25 m tables on 5x5 nodes; layered+anomaly model with randomized Q",
+                   "form": head, "shots_per_step": args.shots_per_step,
+                   "tile": [args.tile_x, args.tile_y, args.ztile],
+                   "updates_per_step": R["updates_per_launch"] * (1 if head != "fp32" else args.shots_per_step)},
+        "roofline": {"bound": "hbm", "kernel": R["imaging_kernel"], "achieved": R["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": R["frac"], "traffic": traffic,
+                     "algorithmic_bytes_per_update": BYTES_PER_UPDATE[head],
+                     "avg_launch_ms": R["imaging_avg_launch_ms"],
+                     "updates_per_launch": R["updates_per_launch"]},
+        "forms": results,
+        "speedup_vs_fp32": (R["value"] / results["fp32"]["value"]) if "fp32" in results else None,
+        "image_rel_l2": errors,
+        "surrogate": sinfo,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -34,6 +34,9 @@ extern "C" {
 #define QPSDM_BF16 1 /* bank only                         */
 #define QPSDM_F16B 2 /* tables: FP16 codes, offset+scale per 8^3 brick            */
 #define QPSDM_U16B 3 /* tables: unsigned 16-bit codes, offset+scale per 8^3 brick */
+#define QPSDM_U16X 4 /* tables: unsigned 16-bit codes, offset+step per 9x9x8 brick whose
+                        v and u edges repeat the neighbour's first nodes (one brick holds
+                        every bilinear corner of its cells) */
 
 /* Two-window spectral set-up of one offset class; replaces the window block
  * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
@@ -113,9 +116,14 @@ int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab_de
  * QPSDM_U16B have the same size). */
 size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g);
 
+/* Size in bytes of a table set in storage type dtype (F32, F16B, U16B, U16X). */
+size_t qpsdm_tables_bytes(const qpsdm_tgeom *g, int dtype);
+
 /* Quantise an FP32 table set into 16-bit bricks: per 8^3 brick of (t, t*)
  * an offset and a scale per quantity, codes h = (v - off)/scale in [0, 1]
- * stored as FP16 (QPSDM_F16B) or as round(h * 65535) (QPSDM_U16B). */
+ * stored as FP16 (QPSDM_F16B) or as round(h * 65535) (QPSDM_U16B); or per
+ * 9x9x8 edge-sharing brick an offset and a step, codes round((v - off)/step)
+ * (QPSDM_U16X). */
 int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, int code,
                           void *stream);
 
@@ -193,8 +201,17 @@ typedef struct {
 /* Launch shape picked by the autotuner (0 = default). */
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
-  int trace_chunk;            /* traces evaluated together per depth point            */
+  int trace_chunk;            /* reserved: traces per tile work item (0 = all)         */
+  int parts;                  /* persistent kernel: tile ranges with their own work
+                                 counters, homed on blockIdx.x % parts (0 = 8, 1..8)   */
 } qpsdm_launch;
+
+/* Table lookup as the imaging kernel does it (nearest node of the surface
+ * position (px, py), bilinear in the relative (x, y), linear in z): out
+ * [n][2] = (t, t*) at xyz [n][3], for any table storage type.  Replaces the
+ * table indexing of image_depth_gpu (src/q-psdm.cu:2303-2318). */
+int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab_dev, int table_dtype, float px, float py,
+                        const float *xyz_dev, int n, float *out_dev, void *stream);
 
 /* A run of traces: geom_dev holds (sx, sy, gx, gy) per trace, image-relative,
  * and the run is geom rows [tr0, tr0+ntr); bank_dev holds the run's ntr
@@ -304,9 +321,16 @@ int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float *traces_de
                              float *img_dev, const float *geom_dev, const void *tab_dev, int table_dtype,
                              const qpsdm_shot *shots_dev, int nshot, const qpsdm_surrogate *sur,
                              const float *tab_f32_dev, const unsigned char *mask_dev, void *src_dev,
-                             unsigned *work_dev, int nwg, qpsdm_graph **out);
+                             unsigned *work_dev, int nwg, int stages, qpsdm_graph **out);
+/* stages: bit mask of what the graph holds, 1 = Q bank, 2 = source tables,
+ * 4 = imaging (7 = all); a caller that times the imaging alone captures
+ * 1|2 and 4 as two graphs and launches them in order on one stream. */
 int qpsdm_graph_launch(qpsdm_graph *g, void *stream);
 int qpsdm_graph_destroy(qpsdm_graph *g);
+
+/* Plain device copy of n floats (n % 4 == 0, 16-byte aligned): a kernel of
+ * exactly known traffic, for calibrating hardware byte counters. */
+int qpsdm_copy_f32(const float *src_dev, float *dst_dev, size_t n, void *stream);
 
 /* Output normalisation img /= nsb (src/q-psdm.cu:1760-1766). */
 int qpsdm_image_scale(float *img_dev, size_t n, float s, void *stream);
@@ -326,7 +350,7 @@ typedef struct {
   qpsdm_bank_params bank;    /* windows and Q bank (nt, K, dtstar as image)   */
   qpsdm_geom_params geom;    /* SU coordinate scale, model extent, origin     */
   int bank_dtype;            /* QPSDM_F32 or QPSDM_BF16                       */
-  int table_dtype;           /* QPSDM_F32, QPSDM_F16B or QPSDM_U16B           */
+  int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B or QPSDM_U16X */
   int batch_traces;          /* traces per staging slot                       */
   qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */
 } qpsdm_ctx_params;

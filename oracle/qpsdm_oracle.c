@@ -570,15 +570,17 @@ static inline void lookup_f32(const or_tgeom *g, const float *tab, float px, flo
   for (int k = 0; k < 2; ++k) {
 #define TV(dv_, du_, comp_) T[2 * (((size_t)(iv + dv_) * g->nu + (iu + du_)) * g->nw + (w + k)) + comp_]
     for (int comp = 0; comp < 2; ++comp) {
-      const float lx = TV(0, 0, comp) + a * (TV(0, 1, comp) - TV(0, 0, comp));
-      const float hx = TV(1, 0, comp) + a * (TV(1, 1, comp) - TV(1, 0, comp));
-      col[k][comp] = lx + b * (hx - lx);
+      const float lx = fmaf(a, TV(0, 1, comp) - TV(0, 0, comp), TV(0, 0, comp));
+      const float hx = fmaf(a, TV(1, 1, comp) - TV(1, 0, comp), TV(1, 0, comp));
+      col[k][comp] = fmaf(b, hx - lx, lx);
     }
 #undef TV
   }
-  *t = col[0][0] + c * (col[1][0] - col[0][0]);
-  *ts = col[0][1] + c * (col[1][1] - col[0][1]);
+  *t = fmaf(c, col[1][0] - col[0][0], col[0][0]);
+  *ts = fmaf(c, col[1][1] - col[0][1], col[0][1]);
 }
+
+static inline float sat01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
 
 int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float *tab,
                      const or_iparams *p, float *img) {
@@ -590,6 +592,7 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
   const float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
   const float rx1 = 1.0f / (tx1 - tx1c), rx2 = 1.0f / (tx2c - tx2);
   const float ry1 = 1.0f / (ty1 - ty1c), ry2 = 1.0f / (ty2c - ty2);
+  const float cx1 = -tx1c * rx1, cx2 = tx2c * rx2, cy1 = -ty1c * ry1, cy2 = ty2c * ry2;
   const float inv_dt = 1.0f / p->dt, inv_dts = 1.0f / p->dtstar;
 #pragma omp parallel for schedule(dynamic) reduction(+ : count)
   for (int iy = 0; iy < p->ny; ++iy) {
@@ -605,36 +608,36 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
         const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
         for (int iz = 1; iz < p->nz; ++iz) {
           const float z = (float)iz * p->dz;
-          const float d2 = hs2 + z * z;
-          const float rds = 1.0f / sqrtf(d2), rdg = 1.0f / sqrtf(hg2 + z * z);
-          const float X = dsx * rds + dgx * rdg, Y = dsy * rds + dgy * rdg, Z = z * (rds + rdg);
-          if (X <= tx1c * Z || tx2c * Z <= X || Y <= ty1c * Z || ty2c * Z <= Y) continue;
-          ++count;
-          float wt = 1.0f;
-          if (X < tx1 * Z || X > tx2 * Z || Y < ty1 * Z || Y > ty2 * Z) {
-            const float rz = 1.0f / Z;
-            const float tanx = X * rz, tany = Y * rz;
-            if (tanx < tx1) wt *= (tanx - tx1c) * rx1;
-            if (tanx > tx2) wt *= (tx2c - tanx) * rx2;
-            if (tany < ty1) wt *= (tany - ty1c) * ry1;
-            if (tany > ty2) wt *= (ty2c - tany) * ry2;
-          }
-          float ts, tss, tg, tsg;
+          /* source leg: t in samples, t* in levels + 1/2, bisector terms, ds^3/t */
+          float ts, tss;
           lookup_f32(g, tab, sx, sy, x, y, z, &ts, &tss);
-          lookup_f32(g, tab, gx, gy, x, y, z, &tg, &tsg);
           ts = fmaxf(ts, p->tmin);
+          const float d2s = hs2 + z * z;
+          const float rds = 1.0f / sqrtf(d2s), ds = d2s * rds;
+          const float tsn = ts * inv_dt, tsl = fmaf(tss, inv_dts, 0.5f);
+          const float A = ds * ds * ds * (1.0f / ts);
+          /* receiver leg and bisector; dip taper as clamped ramps */
+          const float rdg = 1.0f / sqrtf(hg2 + z * z);
+          const float X = fmaf(dgx, rdg, dsx * rds), Y = fmaf(dgy, rdg, dsy * rds), Z = fmaf(z, rdg, z * rds);
+          const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
+          const float wx = fminf(sat01(fmaf(tanx, rx1, cx1)), sat01(fmaf(tanx, -rx2, cx2)));
+          const float wy = fminf(sat01(fmaf(tany, ry1, cy1)), sat01(fmaf(tany, -ry2, cy2)));
+          const float wt = wx * wy;
+          if (!(wt > 0.0f)) continue;
+          ++count;
+          float tg, tsg;
+          lookup_f32(g, tab, gx, gy, x, y, z, &tg, &tsg);
           tg = fmaxf(tg, p->tmin);
-          int j = (int)((tss + tsg) * inv_dts + 0.5f);
-          j = j < 0 ? 0 : (j > p->K - 1 ? p->K - 1 : j);
-          const float f = (ts + tg) * inv_dt;
+          const float f = fmaf(tg, inv_dt, tsn);
           const int i0 = (int)f;
           if (!(i0 > 0 && i0 + 1 < p->nt)) continue;
+          int j = (int)fmaf(tsg, inv_dts, tsl);
+          j = j < 0 ? 0 : (j > p->K - 1 ? p->K - 1 : j);
           const float fr = f - (float)i0;
           const float *b = bt + (size_t)j * p->nt + i0;
-          const float ds = d2 * rds;
-          const float rr = ds * rdg;
-          const float tr = rr * rr * rr * tg * (1.0f / ts);
-          img[((size_t)iy * p->nx + ix) * p->nz + iz] += wt * (tr + 1.0f / tr) * (b[0] + fr * (b[1] - b[0]));
+          const float tr = A * (rdg * rdg * rdg) * tg;
+          float *o = &img[((size_t)iy * p->nx + ix) * p->nz + iz];
+          *o = fmaf(wt * (tr + 1.0f / tr), fmaf(fr, b[1] - b[0], b[0]), *o);
         }
       }
     }

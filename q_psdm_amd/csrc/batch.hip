@@ -43,7 +43,7 @@ struct ShotRuns {
     run.tr0 = s.tr0;
     run.ntr = s.ntr;
     run.brow0 = s.tr0;
-    run.dip = qk::make_dip(s.tanx1, s.tanx2, s.tany1, s.tany2, tz);
+    run.dip = qk::uniform_dip(qk::make_dip(s.tanx1, s.tanx2, s.tany1, s.tany2, tz));
     run.src = nullptr;
     run.su0 = run.sv0 = 0.f;
     run.snu = nus;
@@ -77,16 +77,19 @@ template <int ZP, typename BankT, typename Tab>
 __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                          const float4 *__restrict__ geom,
                                                          const BankT *__restrict__ bank, ShotRuns runs,
-                                                         int nshot, unsigned *counters, int ncol, int nzg) {
+                                                         int nshot, unsigned *counters, int ncol, int nzg,
+                                                         int parts) {
   const int total = ncol * nzg;
-  const int part = blockIdx.x % kParts;
-  for (int step = 0; step < kParts; ++step) {
-    const int q = (part + step) % kParts;
-    const int lo = (int)((long long)total * q / kParts), hi = (int)((long long)total * (q + 1) / kParts);
+  const int part = blockIdx.x % parts;
+  for (int step = 0; step < parts; ++step) {
+    const int q = (part + step) % parts;
+    const int lo = (int)((long long)total * q / parts), hi = (int)((long long)total * (q + 1) / parts);
     for (;;) {
       unsigned k = 0;
       if (threadIdx.x == 0) k = atomicAdd(&counters[q], 1u);
-      k = __shfl(k, 0, 64);
+      // broadcast as a scalar: the tile and everything derived from it stay
+      // wave-uniform (scalar depth walk, scalar buffer offsets)
+      k = __builtin_amdgcn_readfirstlane(k);
       const int tile = lo + (int)k;
       if (tile >= hi) break;
       // depth group slowest: a range covers whole depth slabs of the image
@@ -117,6 +120,7 @@ struct Go {
   hipStream_t s;
   bool run;
   int *nwg_out;
+  int parts;
 };
 
 template <int ZP, typename BankT, typename Tab>
@@ -137,7 +141,7 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   const int nzg = (P.nz + ZP - 1) / ZP;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, kParts);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, go.nshot, go.work,
-                     ncol, nzg);
+                     ncol, nzg, go.parts);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }
@@ -171,6 +175,7 @@ int batch_dispatch(const qpsdm_image_params *p, const qpsdm_launch *shape, const
     case QPSDM_F32: return launch_batch_bank(P, qk::make_tab_f32(p->tg, tab), bank, bank_dtype, go);
     case QPSDM_F16B: return launch_batch_bank(P, qk::make_tab_b16<false>(p->tg, tab), bank, bank_dtype, go);
     case QPSDM_U16B: return launch_batch_bank(P, qk::make_tab_b16<true>(p->tg, tab), bank, bank_dtype, go);
+    case QPSDM_U16X: return launch_batch_bank(P, qk::make_tab_u16x(p->tg, tab), bank, bank_dtype, go);
     default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: table dtype %d", table_dtype);
   }
 }
@@ -184,8 +189,10 @@ extern "C" int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_la
   QPSDM_REQUIRE(p, "image_batch: null params");
   QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
   if (nshot == 0) return QPSDM_OK;
+  const int parts = (shape && shape->parts > 0) ? shape->parts : kParts;
+  QPSDM_REQUIRE(parts >= 1 && parts <= kParts, "image_batch: parts must be in 1..%d", kParts);
   const Go go{img, (const float4 *)geom, shot_runs(p, shots, nshot, src), nshot, work, nwg, (hipStream_t)stream,
-              true, nullptr};
+              true, nullptr, parts};
   return batch_dispatch(p, shape, bank, bank_dtype, tab, table_dtype, go);
 }
 
@@ -223,7 +230,8 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
                                         float *img, const float *geom, const void *tab, int table_dtype,
                                         const qpsdm_shot *shots, int nshot, const qpsdm_surrogate *sur,
                                         const float *tab_f32, const unsigned char *mask, void *src,
-                                        unsigned *work, int nwg, qpsdm_graph **out) {
+                                        unsigned *work, int nwg, int stages, qpsdm_graph **out) {
+  QPSDM_REQUIRE(stages > 0 && (stages & ~7) == 0, "batch_graph_create: stages must be a non-empty mask of 1|2|4");
   QPSDM_REQUIRE(!sur || src, "batch_graph_create: a surrogate needs the source-table buffer");
   QPSDM_REQUIRE(out && bp && ip && traces && bank && img && geom && tab && shots && work,
                 "batch_graph_create: null args");
@@ -232,7 +240,8 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   int rc = qpsdm_qbank_prepare(bp);
   if (rc) return rc;
   if (nwg <= 0) {  // resolve the resident grid size outside the capture
-    const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg};
+    const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg,
+               kParts};
     rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
     if (rc) return rc;
   }
@@ -248,9 +257,9 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   };
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(qpsdm::set_error(QPSDM_EHIP, "batch_graph_create: begin capture failed"));
-  rc = qpsdm_qbank(bp, traces, ntr, bank, bank_dtype, s);
-  if (!rc && sur) rc = qpsdm_surrogate_src_tables(sur, ip, geom, shots, nshot, tab_f32, mask, src, s);
-  if (!rc)
+  if (stages & 1) rc = qpsdm_qbank(bp, traces, ntr, bank, bank_dtype, s);
+  if (!rc && sur && (stages & 2)) rc = qpsdm_surrogate_src_tables(sur, ip, geom, shots, nshot, tab_f32, mask, src, s);
+  if (!rc && (stages & 4))
     rc = qpsdm_image_batch_src(ip, shape, img, geom, bank, bank_dtype, tab, table_dtype, shots, nshot,
                                sur ? src : nullptr, work, nwg, s);
   hipGraph_t graph = nullptr;

@@ -112,9 +112,55 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
       return launch_tz(P, T, img, g4, (const float *)bank, s);
     if (bank_dtype == QPSDM_BF16)
       return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
+  } else if (table_dtype == QPSDM_U16X) {
+    auto T = qk::make_tab_u16x(p->tg, tab);
+    if (bank_dtype == QPSDM_F32)
+      return launch_tz(P, T, img, g4, (const float *)bank, s);
+    if (bank_dtype == QPSDM_BF16)
+      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
   }
   return qpsdm::set_error(QPSDM_EINVAL, "image: bad dtypes bank=%d table=%d", bank_dtype,
                           table_dtype);
+}
+
+// ---- table lookup operator -------------------------------------------------
+template <typename Tab>
+__global__ void lookup_kernel(qk::ImgDev P, Tab tab, float px, float py, const float *__restrict__ xyz, int n,
+                              float2 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  auto c = qk::column_setup(P, tab, px, py, x, y);
+  float fz;
+  const int w = qk::depth_node(P, z, &fz);
+  const float2 lo = tab.at(c, w), hi = tab.at(c, w + 1);
+  out[i] = make_float2(fmaf(fz, hi.x - lo.x, lo.x), fmaf(fz, hi.y - lo.y, lo.y));
+}
+
+template <typename Tab>
+int launch_lookup(const qk::ImgDev &P, const Tab &tab, float px, float py, const float *xyz, int n, float *out,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(lookup_kernel<Tab>, dim3((n + 255) / 256), dim3(256), 0, s, P, tab, px, py, xyz, n,
+                     (float2 *)out);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab, int table_dtype, float px,
+                                   float py, const float *xyz, int n, float *out, void *stream) {
+  qk::ImgDev P;
+  int rc = qk::make_dev(p, nullptr, 0, 0, &P);
+  if (rc) return rc;
+  QPSDM_REQUIRE(tab && xyz && out && n >= 0, "tables_lookup: null args");
+  if (n == 0) return QPSDM_OK;
+  const hipStream_t s = (hipStream_t)stream;
+  switch (table_dtype) {
+    case QPSDM_F32: return launch_lookup(P, qk::make_tab_f32(p->tg, tab), px, py, xyz, n, out, s);
+    case QPSDM_F16B: return launch_lookup(P, qk::make_tab_b16<false>(p->tg, tab), px, py, xyz, n, out, s);
+    case QPSDM_U16B: return launch_lookup(P, qk::make_tab_b16<true>(p->tg, tab), px, py, xyz, n, out, s);
+    case QPSDM_U16X: return launch_lookup(P, qk::make_tab_u16x(p->tg, tab), px, py, xyz, n, out, s);
+    default: return qpsdm::set_error(QPSDM_EINVAL, "tables_lookup: table dtype %d", table_dtype);
+  }
 }
 
 extern "C" int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom, int tr0, int ntr,

@@ -21,6 +21,7 @@ namespace qpsdm_dev {
 struct Dip {
   float tx1, tx2, ty1, ty2, tx1c, tx2c, ty1c, ty2c;
   float rx1, rx2, ry1, ry2;
+  float cx1, cx2, cy1, cy2;  // ramp offsets: ramp = tan * r + c
   float Tx, Ty;
 };
 
@@ -38,9 +39,25 @@ __host__ __device__ inline Dip make_dip(float tanx1, float tanx2, float tany1, f
   d.rx2 = 1.0f / (d.tx2c - d.tx2);
   d.ry1 = 1.0f / (d.ty1 - d.ty1c);
   d.ry2 = 1.0f / (d.ty2c - d.ty2);
+  d.cx1 = -d.tx1c * d.rx1;
+  d.cx2 = d.tx2c * d.rx2;
+  d.cy1 = -d.ty1c * d.ry1;
+  d.cy2 = d.ty2c * d.ry2;
   d.Tx = fmaxf(fabsf(d.tx1c), fabsf(d.tx2c));
   d.Ty = fmaxf(fabsf(d.ty1c), fabsf(d.ty2c));
   return d;
+}
+
+// A wave-uniform Dip computed in vector registers, moved to scalar ones.
+__device__ inline Dip uniform_dip(const Dip &d) {
+  auto u = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+  Dip o;
+  o.tx1 = u(d.tx1), o.tx2 = u(d.tx2), o.ty1 = u(d.ty1), o.ty2 = u(d.ty2);
+  o.tx1c = u(d.tx1c), o.tx2c = u(d.tx2c), o.ty1c = u(d.ty1c), o.ty2c = u(d.ty2c);
+  o.rx1 = u(d.rx1), o.rx2 = u(d.rx2), o.ry1 = u(d.ry1), o.ry2 = u(d.ry2);
+  o.cx1 = u(d.cx1), o.cx2 = u(d.cx2), o.cy1 = u(d.cy1), o.cy2 = u(d.cy2);
+  o.Tx = u(d.Tx), o.Ty = u(d.Ty);
+  return o;
 }
 
 __host__ __device__ inline bool dip_ok(const Dip &d) {
@@ -80,6 +97,8 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   QPSDM_REQUIRE((size_t)g.nv * g.nu * g.nw * 8 < (1ull << 31),
                 "image: one node's table must stay below 2 GiB (32-bit buffer offsets)");
   QPSDM_REQUIRE((size_t)p->K * p->nt * 4 < (1ull << 31), "image: one trace's bank must stay below 2 GiB");
+  QPSDM_REQUIRE((size_t)g.nv * g.nu < (1u << 24) && p->nt < (1 << 24),
+                "image: table columns per node and trace length must stay below 2^24 (24-bit index products)");
   o->nx = p->nx;
   o->ny = p->ny;
   o->nz = p->nz;
@@ -148,21 +167,22 @@ struct TabF32 {
   __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
     Col c;
     c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)(nstride * sizeof(float2)));
-    c.o00 = ((iv * nu + iu) * nw) * 8;
-    c.o01 = c.o00 + nw * 8;
-    c.o10 = c.o00 + nu * nw * 8;
-    c.o11 = c.o10 + nw * 8;
+    const unsigned nw8 = (unsigned)nw * 8u;
+    c.o00 = (int)__umul24(__umul24((unsigned)iv, (unsigned)nu) + (unsigned)iu, nw8);
+    c.o01 = c.o00 + (int)nw8;
+    c.o10 = c.o00 + (int)__umul24((unsigned)nu, nw8);
+    c.o11 = c.o10 + (int)nw8;
     c.a = a;
     c.b = b;
     return c;
   }
-  __device__ inline float2 at(const Col &c, int w) const {
+  __device__ inline float2 at(Col &c, int w) const {
     const int so = w * 8;
     const float2 v00 = ld_f2(c.r, c.o00, so), v01 = ld_f2(c.r, c.o01, so);
     const float2 v10 = ld_f2(c.r, c.o10, so), v11 = ld_f2(c.r, c.o11, so);
-    const float lx = v00.x + c.a * (v01.x - v00.x), ly = v00.y + c.a * (v01.y - v00.y);
-    const float hx = v10.x + c.a * (v11.x - v10.x), hy = v10.y + c.a * (v11.y - v10.y);
-    return make_float2(lx + c.b * (hx - lx), ly + c.b * (hy - ly));
+    const float lx = fmaf(c.a, v01.x - v00.x, v00.x), ly = fmaf(c.a, v01.y - v00.y, v00.y);
+    const float hx = fmaf(c.a, v11.x - v10.x, v10.x), hy = fmaf(c.a, v11.y - v10.y, v10.y);
+    return make_float2(fmaf(c.b, hx - lx, lx), fmaf(c.b, hy - ly, ly));
   }
 };
 
@@ -218,14 +238,90 @@ struct TabB16 {
     }
     return make_float2(h.x + h.y * f.x, h.z + h.w * f.y);
   }
-  __device__ inline float2 at(const Col &c, int w) const {
+  __device__ inline float2 at(Col &c, int w) const {
     const float2 v00 = corner(c, 0, w), v01 = corner(c, 1, w);
     const float2 v10 = corner(c, 2, w), v11 = corner(c, 3, w);
-    const float lx = v00.x + c.a * (v01.x - v00.x), ly = v00.y + c.a * (v01.y - v00.y);
-    const float hx = v10.x + c.a * (v11.x - v10.x), hy = v10.y + c.a * (v11.y - v10.y);
-    return make_float2(lx + c.b * (hx - lx), ly + c.b * (hy - ly));
+    const float lx = fmaf(c.a, v01.x - v00.x, v00.x), ly = fmaf(c.a, v01.y - v00.y, v00.y);
+    const float hx = fmaf(c.a, v11.x - v10.x, v10.x), hy = fmaf(c.a, v11.y - v10.y, v10.y);
+    return make_float2(fmaf(c.b, hx - lx, lx), fmaf(c.b, hy - ly, ly));
   }
 };
+
+// 16-bit fixed-point bricks sharing their v and u edges (QPSDM_U16X): brick
+// (bv, bu, bw) holds nodes v in [8 bv, 8 bv + 8], u in [8 bu, 8 bu + 8]
+// (edges clamped to the table) and w in [8 bw, 8 bw + 7], as a float4 header
+// {t_off, t_step, s_off, s_step} then 9 x 9 x 8 codes (t | t* << 16) in
+// [v][u][w] order; value = off + step * code.  The four bilinear corners of
+// any table cell lie in one brick, so a column reads one header per depth
+// brick and interpolates the raw codes before decoding once.
+constexpr int kXCodes = 9 * 9 * 8;
+constexpr int kXBrickBytes = 16 + kXCodes * 4;
+
+struct TabU16X {
+  const unsigned char *p;
+  size_t nstride;  // bytes per node
+  int nbu, nbw;
+  struct Col {
+    __amdgpu_buffer_rsrc_t r;
+    int base;   // header of the column's brick at depth brick 0
+    int o[4];   // code offsets of the four corners (w = 0 of the brick)
+    float a, b;
+    int hb;     // depth brick of the cached header
+    float4 h;
+  };
+  __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
+    Col c;
+    c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)nstride);
+    c.base = (int)__umul24(__umul24((unsigned)(iv >> 3), (unsigned)nbu) + (unsigned)(iu >> 3),
+                           (unsigned)(nbw * kXBrickBytes));
+    const int lv = iv & 7, lu = iu & 7;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c.o[k] = c.base + 16 + ((lv + (k >> 1)) * 9 + lu + (k & 1)) * 32;
+    c.a = a;
+    c.b = b;
+    c.hb = -1;
+    return c;
+  }
+  __device__ inline float2 at(Col &c, int w) const {
+    const int bw = w >> 3;
+    const int sb = bw * kXBrickBytes;
+    if (bw != c.hb) {
+      c.h = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.base, sb, 0));
+      c.hb = bw;
+    }
+    const int so = sb + (w & 7) * 4;
+    const unsigned q00 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[0], so, 0);
+    const unsigned q01 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[1], so, 0);
+    const unsigned q10 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[2], so, 0);
+    const unsigned q11 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[3], so, 0);
+    auto lo = [](unsigned q) { return (float)(q & 0xffffu); };
+    auto hi = [](unsigned q) { return (float)(q >> 16); };
+    const float t0 = fmaf(c.a, lo(q01) - lo(q00), lo(q00)), t1 = fmaf(c.a, lo(q11) - lo(q10), lo(q10));
+    const float s0 = fmaf(c.a, hi(q01) - hi(q00), hi(q00)), s1 = fmaf(c.a, hi(q11) - hi(q10), hi(q10));
+    return make_float2(fmaf(c.h.y, fmaf(c.b, t1 - t0, t0), c.h.x), fmaf(c.h.w, fmaf(c.b, s1 - s0, s0), c.h.z));
+  }
+};
+
+inline void u16x_bricks(const qpsdm_tgeom &g, int *nbv, int *nbu, int *nbw) {
+  *nbv = (g.nv - 2) / 8 + 1;
+  *nbu = (g.nu - 2) / 8 + 1;
+  *nbw = (g.nw + 7) / 8;
+}
+
+inline size_t u16x_node_bytes(const qpsdm_tgeom &g) {
+  int nbv, nbu, nbw;
+  u16x_bricks(g, &nbv, &nbu, &nbw);
+  return (size_t)nbv * nbu * nbw * kXBrickBytes;
+}
+
+inline TabU16X make_tab_u16x(const qpsdm_tgeom &g, const void *p) {
+  TabU16X t;
+  int nbv;
+  u16x_bricks(g, &nbv, &t.nbu, &t.nbw);
+  t.p = (const unsigned char *)p;
+  t.nstride = u16x_node_bytes(g);
+  return t;
+}
 
 inline size_t b16_node_bytes(const qpsdm_tgeom &g) {
   return (size_t)((g.nv + 7) / 8) * ((g.nu + 7) / 8) * ((g.nw + 7) / 8) * kBrickBytes;

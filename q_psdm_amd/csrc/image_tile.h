@@ -34,12 +34,22 @@ struct Run {
   int snu, snv;
 };
 
+// Source-leg cache of one tile point, per source position: t in samples,
+// t* in bank levels plus the rounding half, the source's unit-vector terms
+// of the bisector (dsx/ds, dsy/ds, z/ds) and ds^3 / t (the source part of
+// the (ds/dg)^3 tg/ts weight).
+template <int ZP>
+struct SrcCache {
+  float tsn[ZP], tsl[ZP], sxr[ZP], syr[ZP], zr[ZP], A[ZP];
+};
+
 // Source leg of the tile points: walk the column c of table tab down the
-// depth nodes; t (floored at tmin), t*, 1/t, 1/ds, ds per point.
+// depth nodes (t floored at tmin).
 template <int ZP, typename Tab>
-__device__ inline void source_leg(const ImgDev &P, const Tab &tab, const typename Tab::Col &c, const int (&wz)[ZP],
-                                  const float (&fz)[ZP], const float (&z2)[ZP], float hs2, float (&ts)[ZP],
-                                  float (&tss)[ZP], float (&rts)[ZP], float (&rds)[ZP], float (&ds)[ZP]) {
+__device__ inline void source_leg(const ImgDev &P, const Tab &tab, typename Tab::Col c, const int (&wz)[ZP],
+                                  const float (&fz)[ZP], const float (&zz)[ZP], const float (&z2)[ZP], float dsx,
+                                  float dsy, SrcCache<ZP> &S) {
+  const float hs2 = dsx * dsx + dsy * dsy;
   int cur = wz[0];
   float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
 #pragma unroll
@@ -49,13 +59,31 @@ __device__ inline void source_leg(const ImgDev &P, const Tab &tab, const typenam
       lo = hi;
       hi = tab.at(c, cur + 1);
     }
-    ts[i] = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
-    tss[i] = lo.y + fz[i] * (hi.y - lo.y);
-    rts[i] = __builtin_amdgcn_rcpf(ts[i]);
+    const float ts = fmaxf(fmaf(fz[i], hi.x - lo.x, lo.x), P.tmin);
+    const float tss = fmaf(fz[i], hi.y - lo.y, lo.y);
     const float d2 = hs2 + z2[i];
-    rds[i] = __builtin_amdgcn_rsqf(d2);
-    ds[i] = d2 * rds[i];
+    const float rds = __builtin_amdgcn_rsqf(d2);
+    const float ds = d2 * rds;
+    S.tsn[i] = ts * P.inv_dt;
+    S.tsl[i] = fmaf(tss, P.inv_dtstar, 0.5f);
+    S.sxr[i] = dsx * rds;
+    S.syr[i] = dsy * rds;
+    S.zr[i] = zz[i] * rds;
+    S.A[i] = ds * ds * ds * __builtin_amdgcn_rcpf(ts);
   }
+}
+
+__device__ inline float sat(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
+
+// Dip-bound taper weight of the reference (src/q-psdm.cu:2322-2340) as
+// clamped ramps on tan = X/Z: 1 inside [tan1, tan2], linear to 0 at the
+// widened bounds, 0 outside (the reference rejects those points).
+__device__ inline float dip_taper(const Dip &D, float X, float Y, float Z) {
+  const float rz = __builtin_amdgcn_rcpf(Z);
+  const float tx = X * rz, ty = Y * rz;
+  const float wx = fminf(sat(fmaf(tx, D.rx1, D.cx1)), sat(fmaf(tx, -D.rx2, D.cx2)));
+  const float wy = fminf(sat(fmaf(ty, D.ry1, D.cy1)), sat(fmaf(ty, -D.ry2, D.cy2)));
+  return wx * wy;
 }
 
 template <int ZP, typename BankT, typename Tab, typename Runs>
@@ -87,10 +115,13 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
 #pragma unroll
   for (int i = 0; i < ZP; ++i) acc[i] = (valid && i < nzv) ? img[col + i] : 0.f;
 
-  float ts[ZP], tss[ZP], rts[ZP], rds[ZP], ds[ZP];
-  float sx = NAN, sy = NAN, dsx = 0.f, dsy = 0.f;
+  SrcCache<ZP> S;
+  float sx = NAN, sy = NAN;
   int cache_run = -1;
   const unsigned bank_bytes = (unsigned)(P.K * P.nt * sizeof(BankT));
+  const unsigned nt = (unsigned)P.nt;
+  const float tmin = __builtin_canonicalizef(P.tmin);  // lets fmaxf skip re-quieting it per point
+  const float jmax = (float)(P.K - 1);
 
   for (int r = 0; r < nrun; ++r) {
     const Run run = runs(r);
@@ -104,9 +135,6 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
         cache_run = r;
         sx = g.x;
         sy = g.y;
-        dsx = sx - x;
-        dsy = sy - y;
-        const float hs2 = dsx * dsx + dsy * dsy;
         if (run.src) {
           TabF32 st;
           st.p = run.src;
@@ -117,18 +145,19 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           float a, b;
           frac_index((x - sx - run.su0) * P.inv_du, run.snu, &iu, &a);
           frac_index((y - sy - run.sv0) * P.inv_dv, run.snv, &iv, &b);
-          source_leg<ZP>(P, st, st.column(0, iv, iu, a, b), wz, fz, z2, hs2, ts, tss, rts, rds, ds);
+          source_leg<ZP>(P, st, st.column(0, iv, iu, a, b), wz, fz, zz, z2, sx - x, sy - y, S);
         } else {
-          source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, z2, hs2, ts, tss, rts, rds, ds);
+          source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, zz, z2, sx - x, sy - y, S);
         }
       }
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;
-      const auto c = column_setup(P, tab, g.z, g.w, x, y);
+      auto c = column_setup(P, tab, g.z, g.w, x, y);
       const __amdgpu_buffer_rsrc_t br =
           make_rsrc(bank + (size_t)(t - run.tr0 + run.brow0) * P.K * P.nt, bank_bytes);
       int cur = wz[0];
       float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
+      float2 dl = make_float2(hi.x - lo.x, hi.y - lo.y);
 #pragma unroll
       for (int i = 0; i < ZP; ++i) {
         if (i >= nzv) continue;
@@ -136,25 +165,23 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           ++cur;
           lo = hi;
           hi = tab.at(c, cur + 1);
+          dl = make_float2(hi.x - lo.x, hi.y - lo.y);
         }
         if (!valid || z0 + i == 0) continue;
         const float rdg = __builtin_amdgcn_rsqf(hg2 + z2[i]);
-        float wt;
-        if (!dip_weight(D, dsx * rds[i] + dgx * rdg, dsy * rds[i] + dgy * rdg, zz[i] * (rds[i] + rdg),
-                        &wt))
-          continue;
-        const float tg = fmaxf(lo.x + fz[i] * (hi.x - lo.x), P.tmin);
-        const float tsg = lo.y + fz[i] * (hi.y - lo.y);
-        int j = (int)((tss[i] + tsg) * P.inv_dtstar + 0.5f);
-        j = j < 0 ? 0 : (j > P.K - 1 ? P.K - 1 : j);
-        const float f = (ts[i] + tg) * P.inv_dt;
+        const float w = dip_taper(D, fmaf(dgx, rdg, S.sxr[i]), fmaf(dgy, rdg, S.syr[i]), fmaf(zz[i], rdg, S.zr[i]));
+        if (!(w > 0.0f)) continue;
+        const float tg = fmaxf(fmaf(fz[i], dl.x, lo.x), tmin);
+        const float tsg = fmaf(fz[i], dl.y, lo.y);
+        const float f = fmaf(tg, P.inv_dt, S.tsn[i]);
         const int i0 = (int)f;
-        if (!(i0 > 0 && i0 + 1 < P.nt)) continue;
-        const float frt = f - (float)i0;
-        const float2 b = bank_pair(br, j * P.nt + i0, bank);
-        const float rr = ds[i] * rdg;
-        const float tr = rr * rr * rr * tg * rts[i];
-        acc[i] += wt * (tr + __builtin_amdgcn_rcpf(tr)) * (b.x + frt * (b.y - b.x));
+        if ((unsigned)(i0 - 1) >= nt - 2u) continue;  // i0 > 0 && i0 + 1 < nt
+        // level: round(t*/dt*) clamped to [0, K-1] (the half is in tsl)
+        const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
+        const float frt = __builtin_amdgcn_fractf(f);  // f >= 1 here: f - i0
+        const float2 b = bank_pair(br, (int)__umul24((unsigned)j, nt) + i0, bank);
+        const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
+        acc[i] = fmaf(w * (tr + __builtin_amdgcn_rcpf(tr)), fmaf(frt, b.y - b.x, b.x), acc[i]);
       }
     }
   }

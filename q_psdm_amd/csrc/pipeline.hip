@@ -137,7 +137,8 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
   QPSDM_REQUIRE(p->image.nt == p->bank.nt && p->image.K == p->bank.K && p->image.dtstar == p->bank.dtstar,
                 "ctx_create: image and bank parameters disagree on nt/K/dtstar");
   QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16, "ctx_create: bank dtype");
-  QPSDM_REQUIRE(p->table_dtype == QPSDM_F32 || p->table_dtype == QPSDM_F16B || p->table_dtype == QPSDM_U16B,
+  QPSDM_REQUIRE(p->table_dtype == QPSDM_F32 || p->table_dtype == QPSDM_F16B || p->table_dtype == QPSDM_U16B ||
+                    p->table_dtype == QPSDM_U16X,
                 "ctx_create: table dtype");
   qpsdm_ctx *c = new qpsdm_ctx();
   c->p = *p;
@@ -211,7 +212,7 @@ extern "C" int qpsdm_ctx_set_tables(qpsdm_ctx *c, const float *tab, int on_devic
     c->tab = t32;
   } else {
     void *q = nullptr;
-    QPSDM_HIP(hipMalloc(&q, qpsdm_tables_f16b_bytes(&g)));
+    QPSDM_HIP(hipMalloc(&q, qpsdm_tables_bytes(&g, c->p.table_dtype)));
     int rc = qpsdm_tables_quantize(&g, t32, q, c->p.table_dtype, c->compute);
     QPSDM_HIP(hipStreamSynchronize(c->compute));
     QPSDM_HIP(hipFree(t32));

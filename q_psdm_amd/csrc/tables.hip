@@ -85,6 +85,59 @@ __global__ void blend_kernel(const float *a, const float *b, size_t n, float c1,
     o[i] = __fadd_rn(__fmul_rn(a[i], c1), __fmul_rn(b[i], c2));  // no contraction: as the host blend
 }
 
+// One work-group per edge-sharing brick: min/max of t and t* over its
+// 9 x 9 x 8 nodes (indices clamped to the table), offset = min, step =
+// range / 65535, code = round((v - off) / step).
+__global__ __launch_bounds__(256) void quant_x_kernel(qpsdm_tgeom g, int nbv, int nbu, int nbw, const float2 *tab,
+                                                      unsigned char *out) {
+  constexpr int N = qpsdm_dev::kXCodes;
+  const size_t nb_node = (size_t)nbv * nbu * nbw;
+  const size_t b = blockIdx.x;
+  const size_t node = b / nb_node;
+  size_t r = b - node * nb_node;
+  const int bw = (int)(r % nbw);
+  r /= nbw;
+  const int bu = (int)(r % nbu), bv = (int)(r / nbu);
+  const float2 *T = tab + node * (size_t)g.nv * g.nu * g.nw;
+  auto at = [&](int k) {
+    const int lw = k & 7, lu = (k >> 3) % 9, lv = (k >> 3) / 9;
+    const int v = min(8 * bv + lv, g.nv - 1), u = min(8 * bu + lu, g.nu - 1), w = min(8 * bw + lw, g.nw - 1);
+    return T[((size_t)v * g.nu + u) * g.nw + w];
+  };
+  float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const float2 v = at(k);
+    mn0 = fminf(mn0, v.x), mx0 = fmaxf(mx0, v.x), mn1 = fminf(mn1, v.y), mx1 = fmaxf(mx1, v.y);
+  }
+  __shared__ float red[4][256];
+  red[0][threadIdx.x] = mn0, red[1][threadIdx.x] = mx0, red[2][threadIdx.x] = mn1, red[3][threadIdx.x] = mx1;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[0][threadIdx.x] = fminf(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+      red[1][threadIdx.x] = fmaxf(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+      red[2][threadIdx.x] = fminf(red[2][threadIdx.x], red[2][threadIdx.x + st]);
+      red[3][threadIdx.x] = fmaxf(red[3][threadIdx.x], red[3][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  const float t0 = red[0][0], s0 = red[2][0];
+  const float tst = fmaxf(red[1][0] - t0, 1e-30f) / 65535.0f, sst = fmaxf(red[3][0] - s0, 1e-30f) / 65535.0f;
+  unsigned char *o = out + b * qpsdm_dev::kXBrickBytes;
+  if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const float2 v = at(k);
+    const unsigned ct = min(__float2uint_rn(fmaxf((v.x - t0) / tst, 0.0f)), 65535u);
+    const unsigned cs = min(__float2uint_rn(fmaxf((v.y - s0) / sst, 0.0f)), 65535u);
+    *(unsigned *)(o + 16 + k * 4) = ct | (cs << 16);
+  }
+}
+
+__global__ void copy_kernel(const float4 *a, float4 *b, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 int check_geom(const qpsdm_tgeom *g) {
   QPSDM_REQUIRE(g && g->nnx >= 1 && g->nny >= 1 && g->nu >= 2 && g->nv >= 2 && g->nw >= 2,
                 "tables: bad geometry");
@@ -111,6 +164,17 @@ extern "C" size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g) {
   return qpsdm_dev::b16_node_bytes(*g) * g->nnx * g->nny;
 }
 
+extern "C" size_t qpsdm_tables_bytes(const qpsdm_tgeom *g, int dtype) {
+  if (check_geom(g)) return 0;
+  switch (dtype) {
+    case QPSDM_F32: return (size_t)g->nv * g->nu * g->nw * 8 * g->nnx * g->nny;
+    case QPSDM_F16B:
+    case QPSDM_U16B: return qpsdm_dev::b16_node_bytes(*g) * g->nnx * g->nny;
+    case QPSDM_U16X: return qpsdm_dev::u16x_node_bytes(*g) * g->nnx * g->nny;
+    default: return 0;
+  }
+}
+
 extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, void *out, int code,
                                      void *stream) {
   int rc = check_geom(g);
@@ -119,8 +183,16 @@ extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, voi
   const size_t nb = (size_t)((g->nv + 7) / 8) * ((g->nu + 7) / 8) * ((g->nw + 7) / 8) * g->nnx *
                     g->nny;
   QPSDM_REQUIRE(nb < (1ull << 31), "quantize: too many bricks for one launch");
-  QPSDM_REQUIRE(code == QPSDM_F16B || code == QPSDM_U16B, "quantize: code must be F16B or U16B");
-  if (code == QPSDM_U16B)
+  QPSDM_REQUIRE(code == QPSDM_F16B || code == QPSDM_U16B || code == QPSDM_U16X,
+                "quantize: code must be F16B, U16B or U16X");
+  if (code == QPSDM_U16X) {
+    int nbv, nbu, nbw;
+    qpsdm_dev::u16x_bricks(*g, &nbv, &nbu, &nbw);
+    const size_t nx = (size_t)nbv * nbu * nbw * g->nnx * g->nny;
+    QPSDM_REQUIRE(nx < (1ull << 31), "quantize: too many bricks for one launch");
+    hipLaunchKernelGGL(quant_x_kernel, dim3((unsigned)nx), dim3(256), 0, (hipStream_t)stream, *g, nbv, nbu, nbw,
+                       (const float2 *)tab, (unsigned char *)out);
+  } else if (code == QPSDM_U16B)
     hipLaunchKernelGGL(quant_kernel<true>, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
                        (const float2 *)tab, (unsigned char *)out);
   else
@@ -137,6 +209,17 @@ extern "C" int qpsdm_tables_blend(const float *a, const float *b, size_t n, floa
   // same float weights as get_ttt_all: c2 = c, c1 = 1 - c2
   const float c2 = c, c1 = 1.0f - c2;
   hipLaunchKernelGGL(blend_kernel, dim3(2048), dim3(256), 0, (hipStream_t)stream, a, b, n, c1, c2, o);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_copy_f32(const float *src, float *dst, size_t n, void *stream) {
+  QPSDM_REQUIRE(src && dst, "copy: null args");
+  QPSDM_REQUIRE(n % 4 == 0 && ((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 15) == 0,
+                "copy: n must be a multiple of 4 and both buffers 16-byte aligned");
+  if (n == 0) return QPSDM_OK;
+  hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, (hipStream_t)stream, (const float4 *)src,
+                     (float4 *)dst, n / 4);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }

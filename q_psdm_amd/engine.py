@@ -107,13 +107,24 @@ class Survey:
         return tab
 
     def quantize_tables(self, tab, code="f16", stream=None):
-        """16-bit brick tables; code "f16" (FP16 codes) or "u16" (fixed point).
-        Returned as a uint8 tensor tagged with .qpsdm_code."""
-        nbytes = L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
+        """16-bit brick tables; code "f16" (FP16 codes in 8^3 bricks), "u16"
+        (fixed point in 8^3 bricks) or "u16x" (fixed point in edge-sharing
+        9x9x8 bricks).  Returned as a uint8 tensor tagged with .qpsdm_code."""
+        c = {"f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B, "u16x": L.QPSDM_U16X}[code]
+        nbytes = L.lib().qpsdm_tables_bytes(C.byref(self.tg), c)
         out = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-        c = L.QPSDM_F16B if code == "f16" else L.QPSDM_U16B
         check(L.lib().qpsdm_tables_quantize(C.byref(self.tg), _ptr(tab), _ptr(out), c, _stream(stream)))
         out.qpsdm_code = c
+        return out
+
+    def lookup(self, tab, px, py, xyz, stream=None):
+        """(t, t*) [n][2] at points xyz [n][3] from the surface position
+        (px, py), read from tab in its storage type as the imaging kernel does."""
+        _, tcode = self._codes(torch.empty((1, self.cfg.K, self.cfg.nt), device="meta"), tab)
+        assert xyz.dtype == torch.float32 and xyz.dim() == 2 and xyz.shape[1] == 3
+        out = torch.empty((xyz.shape[0], 2), dtype=torch.float32, device="cuda")
+        check(L.lib().qpsdm_tables_lookup(C.byref(self.ip), _ptr(tab), tcode, px, py, _ptr(xyz), xyz.shape[0],
+                                          _ptr(out), _stream(stream)))
         return out
 
     # -- kernel (a) --------------------------------------------------------
@@ -147,12 +158,7 @@ class Survey:
         assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
         assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
         assert bank.dim() == 3 and bank.shape[0] >= ntr and bank.shape[1:] == (c.K, c.nt)
-        if tab.dtype == torch.float32:
-            assert tuple(tab.shape) == self.table_shape()
-        else:
-            assert tab.numel() == L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
-        bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
-        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else getattr(tab, "qpsdm_code", L.QPSDM_F16B)
+        bcode, tcode = self._codes(bank, tab)
         lp = None
         if launch is not None:
             lp = L.Launch(*launch)
@@ -176,10 +182,11 @@ class Survey:
     def _codes(self, bank, tab):
         if tab.dtype == torch.float32:
             assert tuple(tab.shape) == self.table_shape()
+            tcode = L.QPSDM_F32
         else:
-            assert tab.numel() == L.lib().qpsdm_tables_f16b_bytes(C.byref(self.tg))
+            tcode = getattr(tab, "qpsdm_code", L.QPSDM_F16B)
+            assert tab.numel() == L.lib().qpsdm_tables_bytes(C.byref(self.tg), tcode)
         bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
-        tcode = L.QPSDM_F32 if tab.dtype == torch.float32 else getattr(tab, "qpsdm_code", L.QPSDM_F16B)
         return bcode, tcode
 
     def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, src=None, stream=None):
@@ -239,7 +246,7 @@ class BatchGraph:
     read their current contents."""
 
     def __init__(self, survey, traces, bank, img, geom, tab, shots, launch=None, nwg=0, surrogate=None,
-                 tab_f32=None, mask=None, src=None):
+                 tab_f32=None, mask=None, src=None, stages=7):
         c = survey.cfg
         assert traces.dtype == torch.float32 and traces.shape[1] == c.nt and traces.is_contiguous()
         assert bank.shape == (traces.shape[0], c.K, c.nt) and img.shape == (c.ny, c.nx, c.nz)
@@ -258,7 +265,7 @@ class BatchGraph:
             C.byref(survey.bp), _ptr(traces), traces.shape[0], _ptr(bank), bcode, C.byref(survey.ip),
             C.byref(lp) if lp else None, _ptr(img), _ptr(geom), _ptr(tab), tcode, _ptr(shots.dev), shots.n,
             surrogate.h if surrogate is not None else None, opt(tab_f32), opt(mask), opt(src),
-            _ptr(self.work), nwg, C.byref(h)))
+            _ptr(self.work), nwg, stages, C.byref(h)))
         self.h = h
 
     def launch(self, stream=None):
@@ -274,6 +281,12 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+def copy_f32(src, dst, stream=None):
+    """Device copy by the library's plain copy kernel (counter calibration)."""
+    assert src.dtype == dst.dtype == torch.float32 and src.numel() == dst.numel()
+    check(L.lib().qpsdm_copy_f32(_ptr(src), _ptr(dst), src.numel(), _stream(stream)))
 
 
 def build_tables_into(model, tgeom, out, stream=None):
@@ -335,7 +348,8 @@ class Pipeline:
                                        survey.cfg.dx, survey.cfg.dy)
         p.geom = geom_params
         p.bank_dtype = L.QPSDM_F32 if bank_dtype == "f32" else L.QPSDM_BF16
-        p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B}[table_dtype]
+        p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B,
+                         "u16x": L.QPSDM_U16X}[table_dtype]
         p.batch_traces = batch_traces
         if launch is not None:
             p.shape = L.Launch(*launch)

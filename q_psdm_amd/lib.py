@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqpsdm.so")
 
-QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B = 0, 1, 2, 3
+QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X = 0, 1, 2, 3, 4
 
 
 class Window(C.Structure):
@@ -67,7 +67,7 @@ class ImageParams(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
-                ("trace_chunk", C.c_int)]
+                ("trace_chunk", C.c_int), ("parts", C.c_int)]
 
 
 class CtxParams(C.Structure):
@@ -114,7 +114,8 @@ EXPORTS = [
     "qpsdm_surrogate_nparams", "qpsdm_surrogate_create", "qpsdm_surrogate_destroy",
     "qpsdm_surrogate_get_params", "qpsdm_surrogate_set_params", "qpsdm_surrogate_samples",
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
-    "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables",
+    "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables", "qpsdm_copy_f32",
+    "qpsdm_tables_bytes", "qpsdm_tables_lookup",
 ]
 
 _lib = None
@@ -193,7 +194,13 @@ def lib():
         L.qpsdm_count_batch.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp]
         L.qpsdm_batch_graph_create.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, C.c_int,
                                                C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp,
-                                               C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.POINTER(vp)]
+                                               C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int,
+                                               C.POINTER(vp)]
+        L.qpsdm_copy_f32.argtypes = [vp, vp, C.c_size_t, vp]
+        L.qpsdm_tables_bytes.restype = C.c_size_t
+        L.qpsdm_tables_bytes.argtypes = [C.POINTER(TGeom), C.c_int]
+        L.qpsdm_tables_lookup.argtypes = [C.POINTER(ImageParams), vp, C.c_int, C.c_float, C.c_float, vp, C.c_int,
+                                          vp, vp]
         L.qpsdm_image_batch_src.argtypes = [C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, vp, C.c_int,
                                             vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]
         f5 = C.c_float * 5
