@@ -9,8 +9,8 @@ Kirchhoff sum into the full image), inputs resident in HBM.  Three forms
 run the same shots:
 
   fp32   FP32 bank, FP32 nearest-node tables, one launch per shot
-  mixed  BF16 bank, 16-bit fixed-point brick tables, the persistent batched
-         kernel under a captured hipGraph
+  mixed  BF16 bank, 16-bit fixed-point brick tables (9x9x8 bricks sharing
+         their edges), the persistent batched kernel under captured hipGraphs
   fast   mixed + the learned source leg: per-shot source tables from the
          BF16-MFMA surrogate (trained on the device first, table fallback
          where it misses by more than 0.5 ms), inside the same graph
@@ -66,7 +66,7 @@ class Workload:
         self.sv = engine.Survey(cfg, engine.tstar_max_from_tables(self.tab))
         log(f"tables {tuple(self.tab.shape)} {self.tab.numel() * 4 / 1e9:.2f} GB in {time.time() - t0:.1f}s; "
             f"dtstar={self.sv.dtstar * 1e3:.3f} ms nsb={self.sv.win.nsb} nww={self.sv.win.nww}")
-        self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16")
+        self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16x")
         shots = synth.shots_config2(nshots=nshot)
         self.geom, self.ranges = synth.geometry(shots)
         self.g_d = torch.from_numpy(self.geom).cuda()
@@ -92,11 +92,30 @@ class Workload:
                                    shots=engine.ShotList(desc), updates=sum(self.counts[i] for i in idx)))
 
 
+def tune_form(form, W, args, torch, engine):
+    """Launch shape of one form from the autotuner (on a 400-trace slice of
+    the first shot), unless --ztile pins it."""
+    from q_psdm_amd import autotune
+    if args.ztile:
+        return (args.tile_x, args.tile_y, args.ztile, 0), args.nwg
+    st = W.steps[0]
+    n = min(400, st["traces"].shape[0])
+    if form == "fp32":
+        bank = W.sv.qbank(st["traces"][:n])
+        launch, nwg, res = autotune.tune(W.sv, st["geom"], bank, W.tab, ntr=n, kind="shot")
+    else:
+        bank = W.sv.qbank(st["traces"][:n], dtype=torch.bfloat16)
+        launch, nwg, res = autotune.tune(W.sv, st["geom"], bank, W.tab_u16, shots=st["shots"], ntr=n, kind="batch")
+    log(f"[{form}] autotuned launch {launch[:3]} nwg {nwg or 'resident'} "
+        f"({min(r[2] for r in res):.1f} ms vs {max(r[2] for r in res):.1f} ms worst on the slice)")
+    return launch, nwg
+
+
 def time_form(form, W, args, torch, engine, sur=None, mask=None):
     """Runs warmup + timed steps of one form; returns (image, stats)."""
     sv, cfg = W.sv, W.cfg
     img = sv.new_image()
-    launch = (args.tile_x, args.tile_y, args.ztile, 0)
+    launch, nwg = tune_form(form, W, args, torch, engine)
     ev = []
     if form == "fp32":
         bank = torch.empty((max(n for _, n in W.ranges), cfg.K, cfg.nt), dtype=torch.float32, device="cuda")
@@ -122,9 +141,9 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
             bank = torch.empty((st["traces"].shape[0], cfg.K, cfg.nt), dtype=torch.bfloat16, device="cuda")
             su = sur if form == "fast" else None
             g_pre = engine.BatchGraph(sv, st["traces"], bank, img, st["geom"], W.tab_u16, st["shots"], launch=launch,
-                                      surrogate=su, tab_f32=W.tab, mask=mask, stages=3)
+                                      nwg=nwg, surrogate=su, tab_f32=W.tab, mask=mask, stages=3)
             g_img = engine.BatchGraph(sv, st["traces"], bank, img, st["geom"], W.tab_u16, st["shots"], launch=launch,
-                                      surrogate=su, tab_f32=W.tab, mask=mask, src=g_pre.src, stages=4)
+                                      nwg=nwg, surrogate=su, tab_f32=W.tab, mask=mask, src=g_pre.src, stages=4)
             graphs.append((g_pre, g_img, bank))
 
         def run(k):
@@ -157,7 +176,8 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
         launches = len(img_ms)
     avg_img = float(np.mean(img_ms)) / 1e3
     achieved = upd / launches * BYTES_PER_UPDATE[form] / avg_img / 1e9
-    stats = {"value": upd / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+    stats = {"value": upd / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "launch": list(launch[:3]),
+             "nwg": nwg,
              "imaging_kernel": KERNEL_NAME[form], "imaging_avg_launch_ms": avg_img * 1e3,
              "updates_per_launch": upd / launches, "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
              "imaging_share_of_step": sum(img_ms) / (elapsed * 1e3)}
@@ -181,7 +201,7 @@ def train_surrogate(W, args, torch, engine):
     sur = engine.Surrogate.for_survey(cfg, out_scale=(0.02, 0.004, 0.05))
     t0 = time.time()
     X, T = sur.samples(args.train_samples, seed=0x5EED5)
-    hist = sur.train(X, T, steps=args.train_steps, batch=args.train_batch, lr0=2e-3, lr1=2e-5)
+    hist = sur.train(X, T, steps=args.train_steps, batch=args.train_batch, lr0=4e-3, lr1=4e-5)
     torch.cuda.synchronize()
     t_train = time.time() - t0
     del X, T
@@ -305,14 +325,14 @@ def probe(args):
     time_form(args.probe, W, args, torch, engine, sur=sur, mask=mask)
 
 
-def counter_run(counter, form, args, tmp):
+def counter_run(counter, form, args, tmp, launch, nwg):
     rp = shutil.which("rocprofv3")
     if not rp:
         return None, "rocprofv3 not found"
     d = os.path.join(tmp, counter)
     cmd = [rp, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "probe", "--", sys.executable,
            os.path.join(ROOT, "bench.py"), "--probe", form, "--shots-per-step", str(args.shots_per_step),
-           "--ztile", str(args.ztile), "--tile-x", str(args.tile_x), "--tile-y", str(args.tile_y)]
+           "--tile-x", str(launch[0]), "--tile-y", str(launch[1]), "--ztile", str(launch[2]), "--nwg", str(nwg)]
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         # the child's progress goes to this process's stderr (a long run stays visibly alive)
@@ -333,12 +353,12 @@ def counter_run(counter, form, args, tmp):
     return (float(np.mean(kern)), copy[0]), None
 
 
-def measure_traffic(form, args):
+def measure_traffic(form, args, launch, nwg):
     tmp = tempfile.mkdtemp(prefix="qpsdm_pmc_", dir="/tmp")
     try:
         out = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            res, err = counter_run(counter, form, args, tmp)
+            res, err = counter_run(counter, form, args, tmp, launch, nwg)
             if err:
                 log(f"traffic: {err}")
                 return None, err
@@ -360,11 +380,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots-per-step", type=int, default=2)
     ap.add_argument("--forms", default="fp32,mixed,fast")
-    ap.add_argument("--ztile", type=int, default=4)
+    ap.add_argument("--ztile", type=int, default=0, help="pin depth points per lane (0 = autotune)")
     ap.add_argument("--tile-x", type=int, default=8)
     ap.add_argument("--tile-y", type=int, default=8)
+    ap.add_argument("--nwg", type=int, default=0, help="persistent grid with a pinned shape (0 = resident)")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=15000)
     ap.add_argument("--train-batch", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-traces", type=int, default=256)
@@ -414,7 +435,7 @@ def main():
     if not args.no_traffic:
         del images
         torch.cuda.empty_cache()
-        t, err = measure_traffic(head, args)
+        t, err = measure_traffic(head, args, results[head]["launch"], results[head]["nwg"])
         if t:
             traffic = t["FETCH_SIZE"]["bytes"] + t["WRITE_SIZE"]["bytes"]
             results[head]["traffic_detail"] = t
@@ -440,7 +461,7 @@ def main():
         "config": {"workload": "401x401x301 image @10 m; 2000 receivers/shot over 4x4 km; nt=2001 @2 ms; K=32; "
                                "25 m tables on 5x5 nodes; layered+anomaly model with randomized Q",
                    "form": head, "shots_per_step": args.shots_per_step,
-                   "tile": [args.tile_x, args.tile_y, args.ztile],
+                   "tile": R["launch"],
                    "updates_per_step": R["updates_per_launch"] * (1 if head != "fp32" else args.shots_per_step)},
         "roofline": {"bound": "hbm", "kernel": R["imaging_kernel"], "achieved": R["achieved_GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": R["frac"], "traffic": traffic,

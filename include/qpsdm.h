@@ -202,8 +202,6 @@ typedef struct {
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
   int trace_chunk;            /* reserved: traces per tile work item (0 = all)         */
-  int parts;                  /* persistent kernel: tile ranges with their own work
-                                 counters, homed on blockIdx.x % parts (0 = 8, 1..8)   */
 } qpsdm_launch;
 
 /* Table lookup as the imaging kernel does it (nearest node of the surface
@@ -241,13 +239,17 @@ typedef struct {
 } qpsdm_shot;
 
 /* shots_dev: device array of nshot descriptors; geom/bank hold every trace
- * of the batch (bank row = geom row); work_dev: 8 device uint32 counters
+ * of the batch (bank row = geom row); work_dev: a device uint32 work counter
  * (reset by the call); nwg persistent work-groups (0 = resident capacity).
  * The image params' own dip bounds are not used; taperzone is. */
 int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                       const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
                       int table_dtype, const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,
                       void *stream);
+/* Persistent grid (work-groups) that qpsdm_image_batch uses with nwg = 0
+ * for this launch shape and these storage types: the resident capacity. */
+int qpsdm_image_batch_grid(const qpsdm_image_params *p, const qpsdm_launch *shape, int bank_dtype, int table_dtype,
+                           int *nwg);
 int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom_dev, const qpsdm_shot *shots_dev,
                       int nshot, unsigned long long *count_dev, void *stream);
 

@@ -1,0 +1,92 @@
+"""Launch-shape autotuner for the imaging kernels.
+
+The imaging kernels are bound by vector-instruction issue, and the best
+tile shape trades the per-trace column set-up (amortised over more depth
+points per lane) against registers (fewer resident waves) and the
+granularity of the per-trace tile skip.  The best shape depends on the
+table and bank storage types, the image and table geometry and the trace
+count, so it is measured: every candidate images a slice of a
+representative shot, timed with HIP events, and the fastest is kept.
+
+Candidates: column tiles 8x8, 16x4 and 4x16 (one wave of columns) times
+depth points per lane ZP in {2, 3, 4, 6, 8}; for the persistent batched
+kernel also the grid (resident capacity, or twice it).  Results are cached
+per (geometry, storage types, kernel) in memory for the process.
+"""
+import itertools
+
+import torch
+
+TILES = ((8, 8), (16, 4), (4, 16))
+DEPTHS = (2, 3, 4, 6, 8)
+_cache = {}
+
+
+def _key(sv, bank, tab, kind):
+    c = sv.cfg
+    return (c.nx, c.ny, c.nz, c.K, c.nt, tuple(sorted(c.tgeom.items())), str(bank.dtype),
+            getattr(tab, "qpsdm_code", 0), kind)
+
+
+def _time(fn, reps=2):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(sv, geom, bank, tab, shots=None, ntr=400, kind="batch", log=None, reps=1):
+    """Best launch for imaging `bank`/`tab` on this survey.
+
+    sv: engine.Survey; geom: device (sx, sy, gx, gy) rows; bank: its Q bank;
+    tab: tables in the storage type to tune for; shots: engine.ShotList for
+    kind="batch" (the first shot is sliced to ntr traces), or None for
+    kind="shot" (rows [0, ntr) as one run).  Returns (launch tuple, nwg,
+    table of (launch, nwg, ms))."""
+    from . import engine
+    key = _key(sv, bank, tab, kind)
+    if key in _cache:
+        return _cache[key]
+    img = sv.new_image()
+    n = min(ntr, geom.shape[0])
+    results = []
+    if kind == "batch":
+        tr0, n0, *tans = shots.host[0]
+        one = engine.ShotList([(tr0, min(n, n0)) + tuple(tans)])
+        grids = (0, -2)
+    else:
+        grids = (0,)
+    for (tx, ty), zp, g in itertools.product(TILES, DEPTHS, grids):
+        launch = (tx, ty, zp, 0)
+        if kind == "batch":
+            nwg = 0
+            if g == -2:
+                nwg = 2 * _resident(sv, geom, bank, tab, one, launch)
+            fn = lambda: sv.image_batch(img, geom, bank, tab, one, launch=launch, nwg=nwg)
+        else:
+            nwg = 0
+            fn = lambda: sv.image_shot(img, geom, 0, n, bank, tab, launch=launch)
+        ms = _time(fn, reps)
+        results.append((launch, nwg, ms))
+        if log:
+            log(f"autotune {kind}: tile {tx}x{ty}x{zp} nwg {nwg or 'auto'}: {ms:.2f} ms")
+    best = min(results, key=lambda r: r[2])
+    _cache[key] = (best[0], best[1], results)
+    return _cache[key]
+
+
+def _resident(sv, geom, bank, tab, shots, launch):
+    """Resident-capacity grid of the persistent kernel for this launch (the
+    library's own choice when nwg = 0), read back by a zero-shot query."""
+    from . import lib as L
+    import ctypes as C
+    out = C.c_int(0)
+    bcode, tcode = sv._codes(bank, tab)
+    lp = L.Launch(*launch)
+    L.check(L.lib().qpsdm_image_batch_grid(C.byref(sv.ip), C.byref(lp), bcode, tcode, C.byref(out)))
+    return out.value

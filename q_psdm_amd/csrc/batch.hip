@@ -1,16 +1,14 @@
 // Batched imaging: shot gathers of uneven trace counts and per-shot
-// apertures through one persistent kernel, and a captured hipGraph of
-// [Q bank of the batch -> counter reset -> persistent imaging].
+// apertures through one persistent kernel, and captured hipGraphs of
+// [Q bank of the batch -> source tables] and [counter reset -> persistent
+// imaging].
 //
-// The persistent grid is sized to the resident capacity of the device.
-// Image tiles are split into eight contiguous ranges, one per group of
-// work-groups with the same blockIdx.x % 8 (the XCD a work-group lands on
-// under round-robin dispatch), so tiles processed together on one XCD are
-// spatial neighbours sharing table and bank lines in its L2; each group takes
-// tiles from its own atomic counter and then steals from the other ranges.
-// A tile is owned by one work-group for every shot of the batch, so the image
-// needs no atomics and the summation order (shots, then traces, in order) is
-// fixed.
+// The persistent grid is sized to the resident capacity of the device; its
+// work-groups take image tiles from one atomic counter in depth-slab order
+// (neighbouring columns of a slab together, so tiles in flight share table
+// and bank lines in L2) until the tiles run out.  A tile is owned by one
+// work-group for every shot of the batch, so the image needs no atomics and
+// the summation order (shots, then traces, in order) is fixed.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -26,7 +24,6 @@ int qpsdm_qbank_prepare(const qpsdm_bank_params *p);
 
 namespace {
 
-constexpr int kParts = 8;
 
 // Per-shot runs; with src_hdr set, each shot's source leg comes from its own
 // source table (header {sx, sy, su0, sv0} per shot, then the tables).
@@ -77,24 +74,17 @@ template <int ZP, typename BankT, typename Tab>
 __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                          const float4 *__restrict__ geom,
                                                          const BankT *__restrict__ bank, ShotRuns runs,
-                                                         int nshot, unsigned *counters, int ncol, int nzg,
-                                                         int parts) {
-  const int total = ncol * nzg;
-  const int part = blockIdx.x % parts;
-  for (int step = 0; step < parts; ++step) {
-    const int q = (part + step) % parts;
-    const int lo = (int)((long long)total * q / parts), hi = (int)((long long)total * (q + 1) / parts);
-    for (;;) {
-      unsigned k = 0;
-      if (threadIdx.x == 0) k = atomicAdd(&counters[q], 1u);
-      // broadcast as a scalar: the tile and everything derived from it stay
-      // wave-uniform (scalar depth walk, scalar buffer offsets)
-      k = __builtin_amdgcn_readfirstlane(k);
-      const int tile = lo + (int)k;
-      if (tile >= hi) break;
-      // depth group slowest: a range covers whole depth slabs of the image
-      qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
-    }
+                                                         int nshot, unsigned *counter, int ncol, int total) {
+  for (;;) {
+    unsigned k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(counter, 1u);
+    // broadcast as a scalar: the tile and everything derived from it stay
+    // wave-uniform (scalar depth walk, scalar buffer offsets)
+    const int tile = (int)__builtin_amdgcn_readfirstlane(k);
+    if (tile >= total) break;
+    // depth group slowest: consecutive tiles are neighbouring columns of one
+    // depth slab, which share table and bank lines in L2
+    qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
   }
 }
 
@@ -120,7 +110,6 @@ struct Go {
   hipStream_t s;
   bool run;
   int *nwg_out;
-  int parts;
 };
 
 template <int ZP, typename BankT, typename Tab>
@@ -139,9 +128,9 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   const hipStream_t s = go.s;
   const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
   const int nzg = (P.nz + ZP - 1) / ZP;
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, kParts);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, 1);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, go.nshot, go.work,
-                     ncol, nzg, go.parts);
+                     ncol, ncol * nzg);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }
@@ -189,11 +178,18 @@ extern "C" int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_la
   QPSDM_REQUIRE(p, "image_batch: null params");
   QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
   if (nshot == 0) return QPSDM_OK;
-  const int parts = (shape && shape->parts > 0) ? shape->parts : kParts;
-  QPSDM_REQUIRE(parts >= 1 && parts <= kParts, "image_batch: parts must be in 1..%d", kParts);
   const Go go{img, (const float4 *)geom, shot_runs(p, shots, nshot, src), nshot, work, nwg, (hipStream_t)stream,
-              true, nullptr, parts};
+              true, nullptr};
   return batch_dispatch(p, shape, bank, bank_dtype, tab, table_dtype, go);
+}
+
+extern "C" int qpsdm_image_batch_grid(const qpsdm_image_params *p, const qpsdm_launch *shape, int bank_dtype,
+                                      int table_dtype, int *nwg) {
+  QPSDM_REQUIRE(p && nwg, "image_batch_grid: null args");
+  *nwg = 0;
+  const Go q{nullptr, nullptr, ShotRuns{nullptr, p->taperzone, nullptr, nullptr, 0, 0, 0}, 0, nullptr, 0, nullptr,
+             false, nwg};
+  return batch_dispatch(p, shape, nullptr, bank_dtype, nullptr, table_dtype, q);
 }
 
 extern "C" int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img,
@@ -240,8 +236,7 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   int rc = qpsdm_qbank_prepare(bp);
   if (rc) return rc;
   if (nwg <= 0) {  // resolve the resident grid size outside the capture
-    const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg,
-               kParts};
+    const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg};
     rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
     if (rc) return rc;
   }

@@ -25,7 +25,11 @@ struct Dip {
   float Tx, Ty;
 };
 
+// Computed on the host for single-run launches and on the device for
+// batches: no contraction, so both give the same IEEE-rounded constants and
+// the two paths stay bit-identical.
 __host__ __device__ inline Dip make_dip(float tanx1, float tanx2, float tany1, float tany2, float tz) {
+#pragma clang fp contract(off)
   Dip d;
   d.tx1 = tanx1;
   d.tx2 = tanx2;

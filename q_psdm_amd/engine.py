@@ -199,7 +199,7 @@ class Survey:
         assert bank.dim() == 3 and bank.shape[0] >= shots.rows and bank.shape[1:] == (c.K, c.nt)
         bcode, tcode = self._codes(bank, tab)
         if work is None:
-            work = torch.empty(8, dtype=torch.int32, device="cuda")
+            work = torch.empty(1, dtype=torch.int32, device="cuda")
         lp = L.Launch(*launch) if launch is not None else None
         if src is not None:
             assert src.numel() >= L.lib().qpsdm_surrogate_src_bytes(C.byref(self.ip), shots.n)
@@ -252,7 +252,7 @@ class BatchGraph:
         assert bank.shape == (traces.shape[0], c.K, c.nt) and img.shape == (c.ny, c.nx, c.nz)
         assert shots.rows <= traces.shape[0] <= geom.shape[0]
         bcode, tcode = survey._codes(bank, tab)
-        self.work = torch.empty(8, dtype=torch.int32, device="cuda")
+        self.work = torch.empty(1, dtype=torch.int32, device="cuda")
         if surrogate is not None and src is None:
             src = surrogate.new_src(survey, shots.n)
         self.src = src

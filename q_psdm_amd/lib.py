@@ -67,7 +67,7 @@ class ImageParams(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
-                ("trace_chunk", C.c_int), ("parts", C.c_int)]
+                ("trace_chunk", C.c_int)]
 
 
 class CtxParams(C.Structure):
@@ -115,7 +115,7 @@ EXPORTS = [
     "qpsdm_surrogate_get_params", "qpsdm_surrogate_set_params", "qpsdm_surrogate_samples",
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
     "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables", "qpsdm_copy_f32",
-    "qpsdm_tables_bytes", "qpsdm_tables_lookup",
+    "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
 ]
 
 _lib = None
@@ -197,6 +197,8 @@ def lib():
                                                C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                                                C.POINTER(vp)]
         L.qpsdm_copy_f32.argtypes = [vp, vp, C.c_size_t, vp]
+        L.qpsdm_image_batch_grid.argtypes = [C.POINTER(ImageParams), C.POINTER(Launch), C.c_int, C.c_int,
+                                             C.POINTER(C.c_int)]
         L.qpsdm_tables_bytes.restype = C.c_size_t
         L.qpsdm_tables_bytes.argtypes = [C.POINTER(TGeom), C.c_int]
         L.qpsdm_tables_lookup.argtypes = [C.POINTER(ImageParams), vp, C.c_int, C.c_float, C.c_float, vp, C.c_int,

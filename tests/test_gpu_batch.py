@@ -47,7 +47,8 @@ def _per_shot(sv, img, g_d, bank, tab, desc, launch=None):
 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
-@pytest.mark.parametrize("launch", [None, (8, 8, 2, 0), (16, 4, 6, 0), (4, 16, 8, 0)])
+@pytest.mark.parametrize("launch", [None] + [(tx, ty, zp, 0) for tx, ty in ((8, 8), (16, 4), (4, 16))
+                                            for zp in (2, 3, 4, 6, 8)])
 def test_batch_equals_per_shot(gpu, mk, launch):
     """Same tiles and per-tile summation order (shots, then traces): the
     persistent kernel reproduces the per-shot launches bit for bit."""
@@ -64,6 +65,11 @@ def test_batch_equals_per_shot(gpu, mk, launch):
         img = sv.new_image()
         sv.image_batch(img, g_d, bank, tab, shots, launch=launch, nwg=nwg)
         torch.cuda.synchronize()
+        if not torch.equal(img, ref):
+            d = (img - ref).abs()
+            bad = torch.nonzero(d > 0)
+            print(f"launch {launch} nwg {nwg}: {bad.shape[0]} points differ, max {float(d.max()):.3e}, "
+                  f"first {bad[:4].tolist()}, rel-L2 {rel_l2(img.cpu().numpy(), ref.cpu().numpy()):.2e}")
         assert torch.equal(img, ref), (launch, nwg)
 
 
@@ -144,3 +150,33 @@ def test_graph_replays_match_eager(gpu):
             assert float(got.abs().sum()) > 0
     finally:
         g.close()
+
+
+def test_autotuner_picks_a_valid_shape(gpu):
+    from q_psdm_amd import autotune
+    cfg = synth.tiny_layers()
+    sv, tab = _survey(cfg)
+    geom, ranges, desc = _uneven_batch(cfg, [9, 7], [40.0, 50.0], seed=3)
+    traces = np.random.default_rng(1).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
+    bank = sv.qbank(torch.from_numpy(traces).cuda())
+    g_d = torch.from_numpy(geom).cuda()
+    shots = engine.ShotList(desc)
+    launch, nwg, res = autotune.tune(sv, g_d, bank, tab, shots=shots, ntr=9, kind="batch")
+    assert len(res) == len(autotune.TILES) * len(autotune.DEPTHS) * 2
+    assert launch[0] * launch[1] == 64 and launch[2] in autotune.DEPTHS
+    ref = sv.new_image()
+    _per_shot(sv, ref, g_d, bank, tab, desc, launch)
+    img = sv.new_image()
+    sv.image_batch(img, g_d, bank, tab, shots, launch=launch, nwg=nwg)
+    torch.cuda.synchronize()
+    if not torch.equal(img, ref):
+        d = (img - ref).abs()
+        bad = torch.nonzero(d > 0)
+        print(f"autotuned {launch} nwg {nwg}: {bad.shape[0]} points differ, max {float(d.max()):.3e}, "
+              f"first {bad[:4].tolist()}")
+        for nw in (0, 3, nwg):
+            im2 = sv.new_image()
+            sv.image_batch(im2, g_d, bank, tab, shots, launch=launch, nwg=nw)
+            torch.cuda.synchronize()
+            print(f"  nwg {nw}: equal {torch.equal(im2, ref)}")
+    assert torch.equal(img, ref)
