@@ -421,16 +421,26 @@ __device__ inline bool dip_weight(const Dip &P, float X, float Y, float Z, float
   return true;
 }
 
-// Samples b[idx], b[idx + 1] of a trace's bank row (resource over K*nt).
-__device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const float *) {
-  const float b0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
-  const float b1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4 + 4, 0, 0));
-  return make_float2(b0, b1);
+// Samples b[idx], b[idx + 1] of a trace's bank row (resource over K*nt),
+// as raw bits (bank_raw: nothing waits on the loads) and decoded to FP32
+// (bank_decode, where the samples are consumed).
+__device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const float *) {
+  return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0),
+                    __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4 + 4, 0, 0));
 }
-__device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
-  const unsigned short h0 = __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2, 0, 0);
-  const unsigned short h1 = __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2 + 2, 0, 0);
-  return make_float2(__uint_as_float((unsigned)h0 << 16), __uint_as_float((unsigned)h1 << 16));
+__device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
+  return make_uint2(__builtin_amdgcn_raw_buffer_load_b16(r, idx * 2, 0, 0),
+                    __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2 + 2, 0, 0));
+}
+__device__ inline float2 bank_decode(uint2 v, const float *) {
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ inline float2 bank_decode(uint2 v, const __hip_bfloat16 *) {
+  return make_float2(__uint_as_float(v.x << 16), __uint_as_float(v.y << 16));
+}
+template <typename BankT>
+__device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const BankT *b) {
+  return bank_decode(bank_raw(r, idx, b), b);
 }
 
 }  // namespace qpsdm_dev

@@ -158,8 +158,18 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       int cur = wz[0];
       float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
       float2 dl = make_float2(hi.x - lo.x, hi.y - lo.y);
+      // Three phases: the weights, time fractions and sample indices of all
+      // depth points; then every sample gather in one branch-free block (so
+      // they are in flight together); then decode and accumulate.  A lane
+      // without an update gathers sample 0 with weight 0; the arithmetic of a
+      // point no lane updates is skipped.
+      float cw[ZP], cf[ZP];
+      int ci[ZP];
 #pragma unroll
       for (int i = 0; i < ZP; ++i) {
+        cw[i] = 0.0f;
+        cf[i] = 0.0f;
+        ci[i] = 0;
         if (i >= nzv) continue;
         while (cur < wz[i]) {
           ++cur;
@@ -167,21 +177,30 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           hi = tab.at(c, cur + 1);
           dl = make_float2(hi.x - lo.x, hi.y - lo.y);
         }
-        if (!valid || z0 + i == 0) continue;
+        if (z0 + i == 0) continue;
         const float rdg = __builtin_amdgcn_rsqf(hg2 + z2[i]);
         const float w = dip_taper(D, fmaf(dgx, rdg, S.sxr[i]), fmaf(dgy, rdg, S.syr[i]), fmaf(zz[i], rdg, S.zr[i]));
-        if (!(w > 0.0f)) continue;
         const float tg = fmaxf(fmaf(fz[i], dl.x, lo.x), tmin);
         const float tsg = fmaf(fz[i], dl.y, lo.y);
         const float f = fmaf(tg, P.inv_dt, S.tsn[i]);
         const int i0 = (int)f;
-        if ((unsigned)(i0 - 1) >= nt - 2u) continue;  // i0 > 0 && i0 + 1 < nt
+        // an update: inside the taper, and i0 > 0 && i0 + 1 < nt
+        const bool ok = valid && w > 0.0f && (unsigned)(i0 - 1) < nt - 2u;
+        if (__builtin_amdgcn_ballot_w64(ok) == 0) continue;
         // level: round(t*/dt*) clamped to [0, K-1] (the half is in tsl)
         const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
-        const float frt = __builtin_amdgcn_fractf(f);  // f >= 1 here: f - i0
-        const float2 b = bank_pair(br, (int)__umul24((unsigned)j, nt) + i0, bank);
+        ci[i] = ((int)__umul24((unsigned)j, nt) + i0) & -(int)ok;
+        cf[i] = __builtin_amdgcn_fractf(f);  // f >= 1 where ok: f - i0
         const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
-        acc[i] = fmaf(w * (tr + __builtin_amdgcn_rcpf(tr)), fmaf(frt, b.y - b.x, b.x), acc[i]);
+        cw[i] = ok ? w * (tr + __builtin_amdgcn_rcpf(tr)) : 0.0f;
+      }
+      uint2 cb[ZP];
+#pragma unroll
+      for (int i = 0; i < ZP; ++i) cb[i] = bank_raw(br, ci[i], bank);
+#pragma unroll
+      for (int i = 0; i < ZP; ++i) {
+        const float2 b = bank_decode(cb[i], bank);
+        acc[i] = fmaf(cw[i], fmaf(cf[i], b.y - b.x, b.x), acc[i]);
       }
     }
   }

@@ -168,9 +168,13 @@ __device__ inline void base_leg(const Base &b, float sx, float sy, float x, floa
   *la = -logf(fmaxf(av * L, 1e-30f));
 }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (the epilogue, not
+// the matrix product, bounds these kernels: an IEEE division is a
+// multi-instruction sequence per value).
+__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigm(z); }
 __device__ inline float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigm(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
