@@ -379,19 +379,37 @@ inline void src_subgrid(const qpsdm_image_params *p, int *nus, int *nvs) {
   *nvs = std::min(g.nv, (int)ceil((p->ny - 1) * (double)p->dy / g.dv) + 2);
 }
 
+// Order-preserving map of a float's bits to a signed integer: a < b as
+// floats (neither NaN) implies key(a) <= key(b).  Trace geometry is loaded
+// into scalar registers, and the scalar unit has no float compares, so the
+// per-trace tests below compare keys there instead of spending vector
+// instructions on wave-uniform values.
+__device__ inline int ord_key(float v) {
+  const int b = __float_as_int(v);
+  return b ^ ((b >> 31) & 0x7fffffff);
+}
+
+__device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// A tile's cull bounds: the tile's horizontal extent widened by Tx*zmax
+// (Ty*zmax), a 1e-4 relative and 0.01 m absolute margin, as keys.
 struct TileBox {
-  float xa, xb, ya, yb, limx, limy;
+  int xlo, xhi, ylo, yhi;
 };
 
 __device__ inline TileBox tile_box(const ImgDev &P, const Dip &D, int tx0, int ty0, int z0, int tz) {
-  TileBox b;
-  b.xa = (float)tx0 * P.dx;
-  b.xb = (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx;
-  b.ya = (float)ty0 * P.dy;
-  b.yb = (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy;
+  const float xa = (float)tx0 * P.dx;
+  const float xb = (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx;
+  const float ya = (float)ty0 * P.dy;
+  const float yb = (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy;
   const float zmax = (float)(min(z0 + tz, P.nz) - 1) * P.dz;
-  b.limx = D.Tx * zmax * 1.0001f + 0.01f;
-  b.limy = D.Ty * zmax * 1.0001f + 0.01f;
+  const float limx = D.Tx * zmax * 1.0001f + 0.01f;
+  const float limy = D.Ty * zmax * 1.0001f + 0.01f;
+  TileBox b;
+  b.xlo = uniform_int(ord_key(xa - limx));
+  b.xhi = uniform_int(ord_key(xb + limx));
+  b.ylo = uniform_int(ord_key(ya - limy));
+  b.yhi = uniform_int(ord_key(yb + limy));
   return b;
 }
 
@@ -399,9 +417,12 @@ __device__ inline TileBox tile_box(const ImgDev &P, const Dip &D, int tx0, int t
 // the receiver by more than Tx*z (or Ty*z): then the bisector's |tan| exceeds
 // Tx (Ty) at every point and the dip test rejects all of them.
 __device__ inline bool tile_culled(const TileBox &b, float sx, float sy, float gx, float gy) {
-  return (b.xa - fmaxf(sx, gx) > b.limx) || (fminf(sx, gx) - b.xb > b.limx) ||
-         (b.ya - fmaxf(sy, gy) > b.limy) || (fminf(sy, gy) - b.yb > b.limy);
+  const int ksx = ord_key(sx), ksy = ord_key(sy), kgx = ord_key(gx), kgy = ord_key(gy);
+  return max(ksx, kgx) < b.xlo || min(ksx, kgx) > b.xhi || max(ksy, kgy) < b.ylo || min(ksy, kgy) > b.yhi;
 }
+
+// A trace rejected upstream carries NaN coordinates.
+__device__ inline bool rejected(float sx) { return (__float_as_int(sx) & 0x7fffffff) > 0x7f800000; }
 
 // Dip-bound taper of the reference (src/q-psdm.cu:2322-2340) on the bisector
 // (X, Y, Z), Z > 0: tan = X/Z is compared as X against bound * Z, and 1/Z

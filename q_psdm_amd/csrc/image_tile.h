@@ -78,11 +78,24 @@ __device__ inline float sat(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1
 // Dip-bound taper weight of the reference (src/q-psdm.cu:2322-2340) as
 // clamped ramps on tan = X/Z: 1 inside [tan1, tan2], linear to 0 at the
 // widened bounds, 0 outside (the reference rejects those points).
-__device__ inline float dip_taper(const Dip &D, float X, float Y, float Z) {
+// The ramp offsets come in vector registers (Ramps): a vector ALU
+// instruction reads at most one scalar operand, so with slope and offset
+// both scalar every ramp would first copy one of them.
+struct Ramps {
+  float cx1, cx2, cy1, cy2;
+};
+
+__device__ inline Ramps vector_ramps(const Dip &D) {
+  Ramps R{D.cx1, D.cx2, D.cy1, D.cy2};
+  asm volatile("" : "+v"(R.cx1), "+v"(R.cx2), "+v"(R.cy1), "+v"(R.cy2));
+  return R;
+}
+
+__device__ inline float dip_taper(const Dip &D, const Ramps &R, float X, float Y, float Z) {
   const float rz = __builtin_amdgcn_rcpf(Z);
   const float tx = X * rz, ty = Y * rz;
-  const float wx = fminf(sat(fmaf(tx, D.rx1, D.cx1)), sat(fmaf(tx, -D.rx2, D.cx2)));
-  const float wy = fminf(sat(fmaf(ty, D.ry1, D.cy1)), sat(fmaf(ty, -D.ry2, D.cy2)));
+  const float wx = fminf(sat(fmaf(tx, D.rx1, R.cx1)), sat(fmaf(tx, -D.rx2, R.cx2)));
+  const float wy = fminf(sat(fmaf(ty, D.ry1, R.cy1)), sat(fmaf(ty, -D.ry2, R.cy2)));
   return wx * wy;
 }
 
@@ -127,11 +140,14 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
     const Run run = runs(r);
     const Dip &D = run.dip;
     const TileBox box = tile_box(P, D, tx0, ty0, z0, ZP);
+    const Ramps R = vector_ramps(D);
     for (int t = run.tr0; t < run.tr0 + run.ntr; ++t) {
       const float4 g = geom[t];
-      if (!(g.x == g.x)) continue;  // trace rejected upstream (NaN geometry)
+      if (rejected(g.x)) continue;
       if (tile_culled(box, g.x, g.y, g.z, g.w)) continue;
-      if (g.x != sx || g.y != sy || r != cache_run) {
+      // same source as the cached leg: compared bitwise, on the scalar unit
+      if (__float_as_int(g.x) != __float_as_int(sx) || __float_as_int(g.y) != __float_as_int(sy) ||
+          r != cache_run) {
         cache_run = r;
         sx = g.x;
         sy = g.y;
@@ -161,8 +177,9 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       // Three phases: the weights, time fractions and sample indices of all
       // depth points; then every sample gather in one branch-free block (so
       // they are in flight together); then decode and accumulate.  A lane
-      // without an update gathers sample 0 with weight 0; the arithmetic of a
-      // point no lane updates is skipped.
+      // without an update keeps weight 0 and gathers at an index that is
+      // either inside this trace's bank or past the buffer's end (where the
+      // buffer load returns 0), with a time fraction in [0, 1): its term is 0.
       float cw[ZP], cf[ZP];
       int ci[ZP];
 #pragma unroll
@@ -179,20 +196,22 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
         }
         if (z0 + i == 0) continue;
         const float rdg = __builtin_amdgcn_rsqf(hg2 + z2[i]);
-        const float w = dip_taper(D, fmaf(dgx, rdg, S.sxr[i]), fmaf(dgy, rdg, S.syr[i]), fmaf(zz[i], rdg, S.zr[i]));
+        const float w =
+            dip_taper(D, R, fmaf(dgx, rdg, S.sxr[i]), fmaf(dgy, rdg, S.syr[i]), fmaf(zz[i], rdg, S.zr[i]));
         const float tg = fmaxf(fmaf(fz[i], dl.x, lo.x), tmin);
-        const float tsg = fmaf(fz[i], dl.y, lo.y);
         const float f = fmaf(tg, P.inv_dt, S.tsn[i]);
         const int i0 = (int)f;
+        cf[i] = __builtin_amdgcn_fractf(f);  // f >= 1 where the point updates: f - i0
+        ci[i] = i0;
         // an update: inside the taper, and i0 > 0 && i0 + 1 < nt
-        const bool ok = valid && w > 0.0f && (unsigned)(i0 - 1) < nt - 2u;
-        if (__builtin_amdgcn_ballot_w64(ok) == 0) continue;
-        // level: round(t*/dt*) clamped to [0, K-1] (the half is in tsl)
-        const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
-        ci[i] = ((int)__umul24((unsigned)j, nt) + i0) & -(int)ok;
-        cf[i] = __builtin_amdgcn_fractf(f);  // f >= 1 where ok: f - i0
-        const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
-        cw[i] = ok ? w * (tr + __builtin_amdgcn_rcpf(tr)) : 0.0f;
+        if (valid && w > 0.0f && (unsigned)(i0 - 1) < nt - 2u) {
+          // level: round(t*/dt*) clamped to [0, K-1] (the half is in tsl)
+          const float tsg = fmaf(fz[i], dl.y, lo.y);
+          const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
+          ci[i] += (int)__umul24((unsigned)j, nt);
+          const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
+          cw[i] = w * (tr + __builtin_amdgcn_rcpf(tr));
+        }
       }
       uint2 cb[ZP];
 #pragma unroll
@@ -231,7 +250,7 @@ __device__ inline unsigned count_tile(const ImgDev &P, const float4 *__restrict_
     const TileBox box = tile_box(P, run.dip, tx0, ty0, z0, ZP);
     for (int t = run.tr0; t < run.tr0 + run.ntr; ++t) {
       const float4 g = geom[t];
-      if (!(g.x == g.x)) continue;
+      if (rejected(g.x)) continue;
       if (tile_culled(box, g.x, g.y, g.z, g.w)) continue;
       const float dsx = g.x - x, dsy = g.y - y, hs2 = dsx * dsx + dsy * dsy;
       const float dgx = g.z - x, dgy = g.w - y, hg2 = dgx * dgx + dgy * dgy;
