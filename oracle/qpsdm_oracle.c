@@ -616,9 +616,11 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
           const float rds = 1.0f / sqrtf(d2s), ds = d2s * rds;
           const float tsn = ts * inv_dt, tsl = fmaf(tss, inv_dts, 0.5f);
           const float A = ds * ds * ds * (1.0f / ts);
-          /* receiver leg and bisector; dip taper as clamped ramps */
+          /* receiver leg and bisector (scaled by ds: (ds/dg) dg + ds); dip
+           * taper as clamped ramps */
           const float rdg = 1.0f / sqrtf(hg2 + z * z);
-          const float X = fmaf(dgx, rdg, dsx * rds), Y = fmaf(dgy, rdg, dsy * rds), Z = fmaf(z, rdg, z * rds);
+          const float q = rdg * ds;
+          const float X = fmaf(dgx, q, dsx), Y = fmaf(dgy, q, dsy), Z = fmaf(z, q, z);
           const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
           const float wx = fminf(sat01(fmaf(tanx, rx1, cx1)), sat01(fmaf(tanx, -rx2, cx2)));
           const float wy = fminf(sat01(fmaf(tany, ry1, cy1)), sat01(fmaf(tany, -ry2, cy2)));

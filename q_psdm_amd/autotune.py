@@ -9,9 +9,10 @@ count, so it is measured: every candidate images a slice of a
 representative shot, timed with HIP events, and the fastest is kept.
 
 Candidates: column tiles 8x8, 16x4 and 4x16 (one wave of columns) times
-depth points per lane ZP in {2, 3, 4, 6, 8}; for the persistent batched
-kernel also the grid (resident capacity, or twice it).  Results are cached
-per (geometry, storage types, kernel) in memory for the process.
+depth points per lane ZP in {2, 3, 4, 6, 8}; for the batched kernel also
+the grid: persistent at the resident capacity or twice it, or one
+work-group per tile (nwg = -1).  Results are cached per (geometry, storage
+types, kernel) in memory for the process.
 """
 import itertools
 
@@ -19,6 +20,7 @@ import torch
 
 TILES = ((8, 8), (16, 4), (4, 16))
 DEPTHS = (2, 3, 4, 6, 8)
+GRIDS = (0, -2, -1)  # batched kernel: resident, twice resident (-2), one group per tile (-1)
 _cache = {}
 
 
@@ -58,15 +60,13 @@ def tune(sv, geom, bank, tab, shots=None, ntr=400, kind="batch", log=None, reps=
     if kind == "batch":
         tr0, n0, *tans = shots.host[0]
         one = engine.ShotList([(tr0, min(n, n0)) + tuple(tans)])
-        grids = (0, -2)
+        grids = GRIDS
     else:
         grids = (0,)
     for (tx, ty), zp, g in itertools.product(TILES, DEPTHS, grids):
         launch = (tx, ty, zp, 0)
         if kind == "batch":
-            nwg = 0
-            if g == -2:
-                nwg = 2 * _resident(sv, geom, bank, tab, one, launch)
+            nwg = g if g != -2 else 2 * _resident(sv, geom, bank, tab, one, launch)
             fn = lambda: sv.image_batch(img, geom, bank, tab, one, launch=launch, nwg=nwg)
         else:
             nwg = 0

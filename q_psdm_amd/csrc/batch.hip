@@ -88,6 +88,17 @@ __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, 
   }
 }
 
+// One work-group per tile (nwg = -1): the persistent kernel's tile body and
+// order without its loop, scheduled by the hardware dispatcher.
+template <int ZP, typename BankT, typename Tab>
+__global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+                                                               const float4 *__restrict__ geom,
+                                                               const BankT *__restrict__ bank, ShotRuns runs,
+                                                               int nshot, int ncol) {
+  const int tile = (int)blockIdx.x;
+  qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+}
+
 template <int ZP>
 __global__ __launch_bounds__(64) void count_batch_kernel(qk::ImgDev P, const float4 *__restrict__ geom,
                                                          ShotRuns runs, int nshot,
@@ -115,6 +126,16 @@ struct Go {
 template <int ZP, typename BankT, typename Tab>
 int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   auto kern = image_batch_kernel<ZP, BankT, Tab>;
+  const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
+  const int nzg = (P.nz + ZP - 1) / ZP;
+  if (go.nwg == -1) {
+    if (go.nwg_out) *go.nwg_out = -1;
+    if (!go.run) return QPSDM_OK;
+    hipLaunchKernelGGL((image_batch_tiles_kernel<ZP, BankT, Tab>), dim3(ncol * nzg), dim3(64), 0, go.s, P, tab,
+                       go.img, go.geom, bank, go.runs, go.nshot, ncol);
+    QPSDM_HIP(hipGetLastError());
+    return QPSDM_OK;
+  }
   int nwg = go.nwg;
   if (nwg <= 0) {
     int per_cu = 0, dev = 0, ncu = 0;
@@ -126,8 +147,6 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   if (go.nwg_out) *go.nwg_out = nwg;
   if (!go.run) return QPSDM_OK;
   const hipStream_t s = go.s;
-  const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
-  const int nzg = (P.nz + ZP - 1) / ZP;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, 1);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, go.nshot, go.work,
                      ncol, ncol * nzg);
@@ -177,6 +196,7 @@ extern "C" int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_la
                                      unsigned *work, int nwg, void *stream) {
   QPSDM_REQUIRE(p, "image_batch: null params");
   QPSDM_REQUIRE(img && geom && bank && tab && shots && work && nshot >= 0, "image_batch: null buffer");
+  QPSDM_REQUIRE(nwg >= -1, "image_batch: nwg must be > 0, 0 (resident) or -1 (one group per tile)");
   if (nshot == 0) return QPSDM_OK;
   const Go go{img, (const float4 *)geom, shot_runs(p, shots, nshot, src), nshot, work, nwg, (hipStream_t)stream,
               true, nullptr};
@@ -235,7 +255,8 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
   // everything that allocates or copies synchronously happens before capture
   int rc = qpsdm_qbank_prepare(bp);
   if (rc) return rc;
-  if (nwg <= 0) {  // resolve the resident grid size outside the capture
+  QPSDM_REQUIRE(nwg >= -1, "batch_graph_create: nwg must be > 0, 0 (resident) or -1 (one group per tile)");
+  if (nwg == 0) {  // resolve the resident grid size outside the capture
     const Go q{img, (const float4 *)geom, shot_runs(ip, shots, nshot, nullptr), nshot, work, 0, nullptr, false, &nwg};
     rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
     if (rc) return rc;

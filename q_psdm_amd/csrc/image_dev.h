@@ -131,7 +131,7 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->inv_du = 1.0f / g.du;
   o->inv_dv = 1.0f / g.dv;
   o->inv_dw = 1.0f / g.dw;
-  int tx = 8, ty = 8, tzp = 4;
+  int tx = 8, ty = 8, tzp = 6;
   if (sh) {
     if (sh->tile_x > 0) tx = sh->tile_x;
     if (sh->tile_y > 0) ty = sh->tile_y;
@@ -162,32 +162,38 @@ struct TabF32 {
   const float2 *p;
   size_t nstride;  // float2 per node
   int nu, nw;
-  // The four corner columns of one image column under one surface position.
+  // The four corner columns of one image column under one surface position:
+  // the (iv, iu) corner's byte offset per lane; the other three sit at fixed
+  // distances (nw and nu nw nodes), added as scalar offsets.
   struct Col {
     __amdgpu_buffer_rsrc_t r;
-    int o00, o01, o10, o11;  // byte offsets of the corner columns
+    int o;
     float a, b;
   };
   __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
     Col c;
     c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)(nstride * sizeof(float2)));
-    const unsigned nw8 = (unsigned)nw * 8u;
-    c.o00 = (int)__umul24(__umul24((unsigned)iv, (unsigned)nu) + (unsigned)iu, nw8);
-    c.o01 = c.o00 + (int)nw8;
-    c.o10 = c.o00 + (int)__umul24((unsigned)nu, nw8);
-    c.o11 = c.o10 + (int)nw8;
+    c.o = (int)__umul24(__umul24((unsigned)iv, (unsigned)nu) + (unsigned)iu, (unsigned)nw * 8u);
     c.a = a;
     c.b = b;
     return c;
   }
-  __device__ inline float2 at(Col &c, int w) const {
-    const int so = w * 8;
-    const float2 v00 = ld_f2(c.r, c.o00, so), v01 = ld_f2(c.r, c.o01, so);
-    const float2 v10 = ld_f2(c.r, c.o10, so), v11 = ld_f2(c.r, c.o11, so);
-    const float lx = fmaf(c.a, v01.x - v00.x, v00.x), ly = fmaf(c.a, v01.y - v00.y, v00.y);
-    const float hx = fmaf(c.a, v11.x - v10.x, v10.x), hy = fmaf(c.a, v11.y - v10.y, v10.y);
+  // The four corners of depth node w as loaded (fetch) and their bilinear
+  // combination (reduce); at = reduce(fetch).  Split so that a caller can
+  // issue the loads of one trace's column well before it needs the values.
+  struct Raw {
+    float2 v00, v01, v10, v11;
+  };
+  __device__ inline Raw fetch(Col &c, int w) const {
+    const int so = w * 8, du = nw * 8, dv = nu * nw * 8;
+    return Raw{ld_f2(c.r, c.o, so), ld_f2(c.r, c.o, so + du), ld_f2(c.r, c.o, so + dv), ld_f2(c.r, c.o, so + dv + du)};
+  }
+  __device__ inline float2 reduce(const Col &c, const Raw &q) const {
+    const float lx = fmaf(c.a, q.v01.x - q.v00.x, q.v00.x), ly = fmaf(c.a, q.v01.y - q.v00.y, q.v00.y);
+    const float hx = fmaf(c.a, q.v11.x - q.v10.x, q.v10.x), hy = fmaf(c.a, q.v11.y - q.v10.y, q.v10.y);
     return make_float2(fmaf(c.b, hx - lx, lx), fmaf(c.b, hy - ly, ly));
   }
+  __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
 };
 
 inline TabF32 make_tab_f32(const qpsdm_tgeom &g, const void *p) {
@@ -249,6 +255,13 @@ struct TabB16 {
     const float hx = fmaf(c.a, v11.x - v10.x, v10.x), hy = fmaf(c.a, v11.y - v10.y, v10.y);
     return make_float2(fmaf(c.b, hx - lx, lx), fmaf(c.b, hy - ly, ly));
   }
+  // No split: the fetch already combines (each corner's header is needed to
+  // decode it).
+  struct Raw {
+    float2 v;
+  };
+  __device__ inline Raw fetch(Col &c, int w) const { return Raw{at(c, w)}; }
+  __device__ inline float2 reduce(const Col &, const Raw &q) const { return q.v; }
 };
 
 // 16-bit fixed-point bricks sharing their v and u edges (QPSDM_U16X): brick
@@ -265,12 +278,14 @@ struct TabU16X {
   const unsigned char *p;
   size_t nstride;  // bytes per node
   int nbu, nbw;
+  // The other three corners' codes sit 32, 9 x 32 and 10 x 32 bytes after
+  // the (iv, iu) corner's (immediate offsets).
   struct Col {
     __amdgpu_buffer_rsrc_t r;
-    int base;   // header of the column's brick at depth brick 0
-    int o[4];   // code offsets of the four corners (w = 0 of the brick)
+    int base;  // header of the column's brick at depth brick 0
+    int o;     // code offset of the (iv, iu) corner (w = 0 of the brick)
     float a, b;
-    int hb;     // depth brick of the cached header
+    int hb;  // depth brick of the cached header
     float4 h;
   };
   __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
@@ -278,15 +293,20 @@ struct TabU16X {
     c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)nstride);
     c.base = (int)__umul24(__umul24((unsigned)(iv >> 3), (unsigned)nbu) + (unsigned)(iu >> 3),
                            (unsigned)(nbw * kXBrickBytes));
-    const int lv = iv & 7, lu = iu & 7;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) c.o[k] = c.base + 16 + ((lv + (k >> 1)) * 9 + lu + (k & 1)) * 32;
+    c.o = c.base + 16 + ((iv & 7) * 9 + (iu & 7)) * 32;
     c.a = a;
     c.b = b;
     c.hb = -1;
     return c;
   }
-  __device__ inline float2 at(Col &c, int w) const {
+  // Codes of the four corners at depth node w with their brick's header
+  // (fetch; loads the header when the depth brick changes), and the bilinear
+  // combination of the codes decoded once (reduce); at = reduce(fetch).
+  struct Raw {
+    unsigned q00, q01, q10, q11;
+    float4 h;
+  };
+  __device__ inline Raw fetch(Col &c, int w) const {
     const int bw = w >> 3;
     const int sb = bw * kXBrickBytes;
     if (bw != c.hb) {
@@ -294,16 +314,19 @@ struct TabU16X {
       c.hb = bw;
     }
     const int so = sb + (w & 7) * 4;
-    const unsigned q00 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[0], so, 0);
-    const unsigned q01 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[1], so, 0);
-    const unsigned q10 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[2], so, 0);
-    const unsigned q11 = __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o[3], so, 0);
-    auto lo = [](unsigned q) { return (float)(q & 0xffffu); };
-    auto hi = [](unsigned q) { return (float)(q >> 16); };
-    const float t0 = fmaf(c.a, lo(q01) - lo(q00), lo(q00)), t1 = fmaf(c.a, lo(q11) - lo(q10), lo(q10));
-    const float s0 = fmaf(c.a, hi(q01) - hi(q00), hi(q00)), s1 = fmaf(c.a, hi(q11) - hi(q10), hi(q10));
-    return make_float2(fmaf(c.h.y, fmaf(c.b, t1 - t0, t0), c.h.x), fmaf(c.h.w, fmaf(c.b, s1 - s0, s0), c.h.z));
+    return Raw{__builtin_amdgcn_raw_buffer_load_b32(c.r, c.o, so, 0),
+               __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o + 32, so, 0),
+               __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o + 9 * 32, so, 0),
+               __builtin_amdgcn_raw_buffer_load_b32(c.r, c.o + 10 * 32, so, 0), c.h};
   }
+  __device__ inline float2 reduce(const Col &c, const Raw &q) const {
+    auto lo = [](unsigned v) { return (float)(v & 0xffffu); };
+    auto hi = [](unsigned v) { return (float)(v >> 16); };
+    const float t0 = fmaf(c.a, lo(q.q01) - lo(q.q00), lo(q.q00)), t1 = fmaf(c.a, lo(q.q11) - lo(q.q10), lo(q.q10));
+    const float s0 = fmaf(c.a, hi(q.q01) - hi(q.q00), hi(q.q00)), s1 = fmaf(c.a, hi(q.q11) - hi(q.q10), hi(q.q10));
+    return make_float2(fmaf(q.h.y, fmaf(c.b, t1 - t0, t0), q.h.x), fmaf(q.h.w, fmaf(c.b, s1 - s0, s0), q.h.z));
+  }
+  __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
 };
 
 inline void u16x_bricks(const qpsdm_tgeom &g, int *nbv, int *nbu, int *nbw) {
@@ -418,11 +441,28 @@ __device__ inline TileBox tile_box(const ImgDev &P, const Dip &D, int tx0, int t
 // Tx (Ty) at every point and the dip test rejects all of them.
 __device__ inline bool tile_culled(const TileBox &b, float sx, float sy, float gx, float gy) {
   const int ksx = ord_key(sx), ksy = ord_key(sy), kgx = ord_key(gx), kgy = ord_key(gy);
-  return max(ksx, kgx) < b.xlo || min(ksx, kgx) > b.xhi || max(ksy, kgy) < b.ylo || min(ksy, kgy) > b.yhi;
+  // (bitwise or: all four coordinates are needed, so one 16-byte scalar load
+  // serves the test instead of dependent loads of single fields)
+  return (max(ksx, kgx) < b.xlo) | (min(ksx, kgx) > b.xhi) | (max(ksy, kgy) < b.ylo) | (min(ksy, kgy) > b.yhi);
 }
 
 // A trace rejected upstream carries NaN coordinates.
 __device__ inline bool rejected(float sx) { return (__float_as_int(sx) & 0x7fffffff) > 0x7f800000; }
+
+// First trace in [t, tend) that is neither rejected nor culled for the tile
+// box (tend if none), with its geometry in g; loads and tests on the scalar
+// unit.  The geometry is read through the constant address space: it is not
+// written during a launch, and that lets the compiler use a scalar load even
+// where the kernel also writes memory (the persistent kernel's atomics).
+typedef const __attribute__((address_space(4))) float *const_float_ptr;
+__device__ inline int next_live(const float4 *__restrict__ geom, const TileBox &box, int t, int tend, float4 &g) {
+  for (; t < tend; ++t) {
+    const const_float_ptr cg = (const_float_ptr)(geom + t);
+    g = make_float4(cg[0], cg[1], cg[2], cg[3]);
+    if (!(rejected(g.x) | tile_culled(box, g.x, g.y, g.z, g.w))) break;
+  }
+  return t;
+}
 
 // Dip-bound taper of the reference (src/q-psdm.cu:2322-2340) on the bisector
 // (X, Y, Z), Z > 0: tan = X/Z is compared as X against bound * Z, and 1/Z

@@ -34,13 +34,18 @@ struct Run {
   int snu, snv;
 };
 
-// Source-leg cache of one tile point, per source position: t in samples,
-// t* in bank levels plus the rounding half, the source's unit-vector terms
-// of the bisector (dsx/ds, dsy/ds, z/ds) and ds^3 / t (the source part of
-// the (ds/dg)^3 tg/ts weight).
+// Source-leg cache of the tile points, per source position: t in samples,
+// t* in bank levels plus the rounding half, the distance ds and ds^3 / t
+// (the source part of the (ds/dg)^3 tg/ts weight) per point, and the
+// column's horizontal offsets to the source.  The bisector
+// (dsx/ds + dgx/dg, dsy/ds + dgy/dg, z/ds + z/dg) is used scaled by ds,
+// (dsx + dgx q, dsy + dgy q, z + z q) with q = ds/dg, which leaves its
+// direction (the only thing the taper reads) unchanged and needs one value
+// per point instead of three.
 template <int ZP>
 struct SrcCache {
-  float tsn[ZP], tsl[ZP], sxr[ZP], syr[ZP], zr[ZP], A[ZP];
+  float tsn[ZP], tsl[ZP], ds[ZP], A[ZP];
+  float dsx, dsy;
 };
 
 // Source leg of the tile points: walk the column c of table tab down the
@@ -50,6 +55,8 @@ __device__ inline void source_leg(const ImgDev &P, const Tab &tab, typename Tab:
                                   const float (&fz)[ZP], const float (&zz)[ZP], const float (&z2)[ZP], float dsx,
                                   float dsy, SrcCache<ZP> &S) {
   const float hs2 = dsx * dsx + dsy * dsy;
+  S.dsx = dsx;
+  S.dsy = dsy;
   int cur = wz[0];
   float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
 #pragma unroll
@@ -66,9 +73,7 @@ __device__ inline void source_leg(const ImgDev &P, const Tab &tab, typename Tab:
     const float ds = d2 * rds;
     S.tsn[i] = ts * P.inv_dt;
     S.tsl[i] = fmaf(tss, P.inv_dtstar, 0.5f);
-    S.sxr[i] = dsx * rds;
-    S.syr[i] = dsy * rds;
-    S.zr[i] = zz[i] * rds;
+    S.ds[i] = ds;
     S.A[i] = ds * ds * ds * __builtin_amdgcn_rcpf(ts);
   }
 }
@@ -141,31 +146,34 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
     const Dip &D = run.dip;
     const TileBox box = tile_box(P, D, tx0, ty0, z0, ZP);
     const Ramps R = vector_ramps(D);
-    for (int t = run.tr0; t < run.tr0 + run.ntr; ++t) {
-      const float4 g = geom[t];
-      if (rejected(g.x)) continue;
-      if (tile_culled(box, g.x, g.y, g.z, g.w)) continue;
+    const int tend = run.tr0 + run.ntr;
+    // source leg of the tile points for trace geometry gs, unless cached
+    auto set_source = [&](const float4 &gs) {
       // same source as the cached leg: compared bitwise, on the scalar unit
-      if (__float_as_int(g.x) != __float_as_int(sx) || __float_as_int(g.y) != __float_as_int(sy) ||
-          r != cache_run) {
-        cache_run = r;
-        sx = g.x;
-        sy = g.y;
-        if (run.src) {
-          TabF32 st;
-          st.p = run.src;
-          st.nstride = (size_t)run.snv * run.snu * P.nw;
-          st.nu = run.snu;
-          st.nw = P.nw;
-          int iu, iv;
-          float a, b;
-          frac_index((x - sx - run.su0) * P.inv_du, run.snu, &iu, &a);
-          frac_index((y - sy - run.sv0) * P.inv_dv, run.snv, &iv, &b);
-          source_leg<ZP>(P, st, st.column(0, iv, iu, a, b), wz, fz, zz, z2, sx - x, sy - y, S);
-        } else {
-          source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, zz, z2, sx - x, sy - y, S);
-        }
+      if (__float_as_int(gs.x) == __float_as_int(sx) && __float_as_int(gs.y) == __float_as_int(sy) &&
+          r == cache_run)
+        return;
+      cache_run = r;
+      sx = gs.x;
+      sy = gs.y;
+      if (run.src) {
+        TabF32 st;
+        st.p = run.src;
+        st.nstride = (size_t)run.snv * run.snu * P.nw;
+        st.nu = run.snu;
+        st.nw = P.nw;
+        int iu, iv;
+        float a, b;
+        frac_index((x - sx - run.su0) * P.inv_du, run.snu, &iu, &a);
+        frac_index((y - sy - run.sv0) * P.inv_dv, run.snv, &iv, &b);
+        source_leg<ZP>(P, st, st.column(0, iv, iu, a, b), wz, fz, zz, z2, sx - x, sy - y, S);
+      } else {
+        source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, zz, z2, sx - x, sy - y, S);
       }
+    };
+    float4 g;
+    for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
+      set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;
       auto c = column_setup(P, tab, g.z, g.w, x, y);
@@ -196,8 +204,8 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
         }
         if (z0 + i == 0) continue;
         const float rdg = __builtin_amdgcn_rsqf(hg2 + z2[i]);
-        const float w =
-            dip_taper(D, R, fmaf(dgx, rdg, S.sxr[i]), fmaf(dgy, rdg, S.syr[i]), fmaf(zz[i], rdg, S.zr[i]));
+        const float q = rdg * S.ds[i];
+        const float w = dip_taper(D, R, fmaf(dgx, q, S.dsx), fmaf(dgy, q, S.dsy), fmaf(zz[i], q, zz[i]));
         const float tg = fmaxf(fmaf(fz[i], dl.x, lo.x), tmin);
         const float f = fmaf(tg, P.inv_dt, S.tsn[i]);
         const int i0 = (int)f;

@@ -61,7 +61,7 @@ def test_batch_equals_per_shot(gpu, mk, launch):
     ref = sv.new_image()
     _per_shot(sv, ref, g_d, bank, tab, desc, launch)
     shots = engine.ShotList(desc)
-    for nwg in (0, 3):  # resident grid, and a grid much smaller than the tile count
+    for nwg in (0, 3, -1):  # resident grid, a grid much smaller than the tile count, one group per tile
         img = sv.new_image()
         sv.image_batch(img, g_d, bank, tab, shots, launch=launch, nwg=nwg)
         torch.cuda.synchronize()
@@ -162,7 +162,7 @@ def test_autotuner_picks_a_valid_shape(gpu):
     g_d = torch.from_numpy(geom).cuda()
     shots = engine.ShotList(desc)
     launch, nwg, res = autotune.tune(sv, g_d, bank, tab, shots=shots, ntr=9, kind="batch")
-    assert len(res) == len(autotune.TILES) * len(autotune.DEPTHS) * 2
+    assert len(res) == len(autotune.TILES) * len(autotune.DEPTHS) * len(autotune.GRIDS)
     assert launch[0] * launch[1] == 64 and launch[2] in autotune.DEPTHS
     ref = sv.new_image()
     _per_shot(sv, ref, g_d, bank, tab, desc, launch)
@@ -174,7 +174,7 @@ def test_autotuner_picks_a_valid_shape(gpu):
         bad = torch.nonzero(d > 0)
         print(f"autotuned {launch} nwg {nwg}: {bad.shape[0]} points differ, max {float(d.max()):.3e}, "
               f"first {bad[:4].tolist()}")
-        for nw in (0, 3, nwg):
+        for nw in (0, 3, -1, nwg):
             im2 = sv.new_image()
             sv.image_batch(im2, g_d, bank, tab, shots, launch=launch, nwg=nw)
             torch.cuda.synchronize()
