@@ -301,15 +301,35 @@ def cpu_baseline(W, ntr):
     g = np.ascontiguousarray(W.geom[tr0:tr0 + ntr])
     x = W.traces[0][:ntr].cpu().numpy()
     t_host = W.tab.cpu().numpy()
+    cores = O.num_threads()
     t0 = time.perf_counter()
     b = O.qbank(x, o_bparams(sv), dtype=np.float32)
     t1 = time.perf_counter()
     _, n = O.image(b, g, t_host, o_iparams(sv), fp32=True)
     t2 = time.perf_counter()
-    log(f"cpu baseline: bank {t1 - t0:.1f}s image {t2 - t1:.1f}s updates {n}")
-    return {"value": n / (t2 - t0), "unit": "updates/s", "cores": O.num_threads(), "kind": "port",
+    log(f"cpu baseline: bank {t1 - t0:.1f}s image {t2 - t1:.1f}s updates {n} ({cores} threads)")
+    # the same path on one thread, on a sixteenth of the sample
+    n1tr = max(1, ntr // 16)
+    O.set_num_threads(1)
+    try:
+        s0 = time.perf_counter()
+        b1 = O.qbank(x[:n1tr], o_bparams(sv), dtype=np.float32)
+        _, n1 = O.image(b1, np.ascontiguousarray(g[:n1tr]), t_host, o_iparams(sv), fp32=True)
+        s1 = time.perf_counter()
+    finally:
+        O.set_num_threads(cores)
+    log(f"cpu baseline, 1 thread: {s1 - s0:.1f}s updates {n1}")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": n / (t2 - t0), "unit": "updates/s", "cores": cores, "kind": "port",
             "sample": f"shot 0, first {ntr} of its traces, full 401x401x301 image (FP32 Q bank + FP32 "
-                      f"Kirchhoff sum, OpenMP over image lines)", "seconds": t2 - t0}
+                      f"Kirchhoff sum, OpenMP over image lines)", "seconds": t2 - t0,
+            "value_1thread": n1 / (s1 - s0), "sample_1thread": f"shot 0, first {n1tr} traces, one thread",
+            "seconds_1thread": s1 - s0, "cpu_model": model, "nproc": os.cpu_count()}
 
 
 # ---- HBM traffic from counter runs --------------------------------------------

@@ -104,6 +104,7 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr,
                      const float *tab, const or_iparams *p, float *img);
 int64_t or_count(const float *geom, int ntr, const or_iparams *p);
 int or_num_threads(void);
+void or_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

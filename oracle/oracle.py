@@ -86,6 +86,8 @@ def lib():
         L.or_count.restype = C.c_int64
         L.or_count.argtypes = [fp, C.c_int, C.POINTER(IParams)]
         L.or_num_threads.restype = C.c_int
+        L.or_set_num_threads.restype = None
+        L.or_set_num_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -254,3 +256,7 @@ def count(geom, ip):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def set_num_threads(n):
+    lib().or_set_num_threads(int(n))

@@ -28,6 +28,15 @@ int or_num_threads(void) {
 #endif
 }
 
+/* Threads of the OpenMP loops that follow (n >= 1). */
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n < 1 ? 1 : n);
+#else
+  (void)n;
+#endif
+}
+
 /* ---------------------------------------------------------------------- */
 /* FFT lengths.  The reference's valid prime-factor lengths are the 240     */
 /* divisors of 720720 = 16*9*5*7*11*13 (src/fft.cc:274-349, NTAB 240);      */

@@ -34,22 +34,28 @@ struct ShotRuns {
   const float2 *src_tab;
   long long src_stride;
   int nus, nvs;
+  // The descriptor and header words are read through the constant address
+  // space (they do not change during a launch), so they are scalar loads
+  // and the run's fields stay in scalar registers.
   __device__ qk::Run operator()(int r) const {
-    const qpsdm_shot s = shots[r];
+    typedef const __attribute__((address_space(4))) int *cip;
+    typedef const __attribute__((address_space(4))) float *cfp;
+    const cip si = (cip)(shots + r);
+    const cfp sf = (cfp)(shots + r);
     qk::Run run;
-    run.tr0 = s.tr0;
-    run.ntr = s.ntr;
-    run.brow0 = s.tr0;
-    run.dip = qk::uniform_dip(qk::make_dip(s.tanx1, s.tanx2, s.tany1, s.tany2, tz));
+    run.tr0 = si[0];
+    run.ntr = si[1];
+    run.brow0 = run.tr0;
+    run.dip = qk::uniform_dip(qk::make_dip(sf[2], sf[3], sf[4], sf[5], tz));
     run.src = nullptr;
     run.su0 = run.sv0 = 0.f;
     run.snu = nus;
     run.snv = nvs;
     if (src_hdr) {
-      const float4 h = src_hdr[r];
+      const cfp h = (cfp)(src_hdr + r);
       run.src = src_tab + (size_t)r * src_stride;
-      run.su0 = h.z;
-      run.sv0 = h.w;
+      run.su0 = h[2];
+      run.sv0 = h[3];
     }
     return run;
   }

@@ -484,20 +484,33 @@ __device__ inline bool dip_weight(const Dip &P, float X, float Y, float Z, float
 
 // Samples b[idx], b[idx + 1] of a trace's bank row (resource over K*nt),
 // as raw bits (bank_raw: nothing waits on the loads) and decoded to FP32
-// (bank_decode, where the samples are consumed).
+// (bank_decode, where the samples are consumed).  A BF16 pair is one 32-bit
+// load at a 2-byte aligned offset (the device runs with unaligned access
+// enabled, as the compiler itself assumes for such pairs), so it takes one
+// load and one register: low half b[idx], high half b[idx + 1].
+template <typename BankT>
+struct BankRaw;
+template <>
+struct BankRaw<float> {
+  typedef uint2 type;
+};
+template <>
+struct BankRaw<__hip_bfloat16> {
+  typedef unsigned type;
+};
+
 __device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const float *) {
   return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0),
                     __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4 + 4, 0, 0));
 }
-__device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
-  return make_uint2(__builtin_amdgcn_raw_buffer_load_b16(r, idx * 2, 0, 0),
-                    __builtin_amdgcn_raw_buffer_load_b16(r, idx * 2 + 2, 0, 0));
+__device__ inline unsigned bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, idx * 2, 0, 0);
 }
 __device__ inline float2 bank_decode(uint2 v, const float *) {
   return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
-__device__ inline float2 bank_decode(uint2 v, const __hip_bfloat16 *) {
-  return make_float2(__uint_as_float(v.x << 16), __uint_as_float(v.y << 16));
+__device__ inline float2 bank_decode(unsigned v, const __hip_bfloat16 *) {
+  return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
 }
 template <typename BankT>
 __device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const BankT *b) {

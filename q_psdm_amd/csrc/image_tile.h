@@ -221,7 +221,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           cw[i] = w * (tr + __builtin_amdgcn_rcpf(tr));
         }
       }
-      uint2 cb[ZP];
+      typename BankRaw<BankT>::type cb[ZP];
 #pragma unroll
       for (int i = 0; i < ZP; ++i) cb[i] = bank_raw(br, ci[i], bank);
 #pragma unroll
