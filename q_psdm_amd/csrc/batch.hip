@@ -61,6 +61,27 @@ struct ShotRuns {
   }
 };
 
+// One shot as a run list of length one.
+struct OneShot {
+  qk::Run run;
+  __device__ qk::Run operator()(int) const { return run; }
+};
+
+// All shots of a batch through one image tile: the tile body per shot (one
+// run, so the compiler sees no loop over runs), its accumulators read from
+// and written back to the image around each shot.  Per image point the sum
+// runs over the shots in order and each shot's traces in order, exactly as
+// consecutive per-shot launches.
+template <int ZP, typename BankT, typename Tab>
+__device__ inline void tile_all_shots(const qk::ImgDev &P, const Tab &tab, float *__restrict__ img,
+                                      const float4 *__restrict__ geom, const BankT *__restrict__ bank,
+                                      int tile_col, int zgroup, const ShotRuns &runs, int nshot) {
+  for (int r = 0; r < nshot; ++r) {
+    const OneShot one{runs(r)};
+    qk::image_tile<ZP>(P, tab, img, geom, bank, tile_col, zgroup, one, 1);
+  }
+}
+
 ShotRuns shot_runs(const qpsdm_image_params *p, const qpsdm_shot *shots, int nshot, const void *src) {
   ShotRuns R{shots, p->taperzone, nullptr, nullptr, 0, 0, 0};
   if (src) {
@@ -90,19 +111,21 @@ __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, 
     if (tile >= total) break;
     // depth group slowest: consecutive tiles are neighbouring columns of one
     // depth slab, which share table and bank lines in L2
-    qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+    tile_all_shots<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
   }
 }
 
 // One work-group per tile (nwg = -1): the persistent kernel's tile body and
-// order without its loop, scheduled by the hardware dispatcher.
+// order without its loop, scheduled by the hardware dispatcher.  Register
+// use is held to four waves per SIMD (the few values that do not fit are
+// kept in scratch across the loop over shots, outside the trace loop).
 template <int ZP, typename BankT, typename Tab>
-__global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                                const float4 *__restrict__ geom,
                                                                const BankT *__restrict__ bank, ShotRuns runs,
                                                                int nshot, int ncol) {
   const int tile = (int)blockIdx.x;
-  qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+  tile_all_shots<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
 }
 
 template <int ZP>
