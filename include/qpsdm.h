@@ -37,6 +37,9 @@ extern "C" {
 #define QPSDM_U16X 4 /* tables: unsigned 16-bit codes, offset+step per 9x9x8 brick whose
                         v and u edges repeat the neighbour's first nodes (one brick holds
                         every bilinear corner of its cells) */
+#define QPSDM_F16 5  /* bank only: FP16 samples under a power-of-two scale per trace; a
+                        trace's row is a 16-byte header {float 1/scale, float scale, 0, 0}
+                        then [K][nt] f16, rows qpsdm_bank_row_bytes() apart */
 
 /* Two-window spectral set-up of one offset class; replaces the window block
  * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
@@ -176,8 +179,14 @@ typedef struct {
 int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces_dev, int ntr,
                         float *re_dev, float *im_dev, void *stream);
 
-/* Q filter bank: traces [ntr][nt] FP32 -> bank [ntr][K][nt] of dtype
- * QPSDM_F32 or QPSDM_BF16. */
+/* Q filter bank: traces [ntr][nt] FP32 -> bank of dtype QPSDM_F32 or
+ * QPSDM_BF16 ([ntr][K][nt]) or QPSDM_F16 (per trace: header + [K][nt], the
+ * scale 2^(14 - e) for the trace's largest |sample| in [2^e, 2^(e+1)), so
+ * every sample is below 2^15 in FP16). */
+/* Bytes between consecutive traces of a bank (K * nt * sample size for
+ * QPSDM_F32 and QPSDM_BF16; 16 + K * nt * 2 rounded up to 16 for QPSDM_F16);
+ * 0 for another dtype. */
+size_t qpsdm_bank_row_bytes(int K, int nt, int dtype);
 int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces_dev, int ntr, void *bank_dev,
                 int bank_dtype, void *stream);
 
@@ -218,7 +227,7 @@ int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab_dev, int ta
  * traces with the same source share one source-leg evaluation (a shot
  * gather pays it once).  A trace whose sx is NaN is skipped.  Image float
  * [ny][nx][nz] accumulated in place.  table_dtype QPSDM_F32 or QPSDM_F16B;
- * bank_dtype QPSDM_F32 or QPSDM_BF16. */
+ * bank_dtype QPSDM_F32, QPSDM_BF16 or QPSDM_F16. */
 int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                      const float *geom_dev, int tr0, int ntr, const void *bank_dev,
                      int bank_dtype, const void *tab_dev, int table_dtype, void *stream);

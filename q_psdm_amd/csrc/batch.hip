@@ -10,6 +10,7 @@
 // work-group for every shot of the batch, so the image needs no atomics and
 // the summation order (shots, then traces, in order) is fixed.
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -200,6 +201,7 @@ template <typename Tab>
 int launch_batch_bank(const qk::ImgDev &P, const Tab &tab, const void *bank, int bank_dtype, const Go &go) {
   if (bank_dtype == QPSDM_F32) return launch_batch_tz(P, tab, (const float *)bank, go);
   if (bank_dtype == QPSDM_BF16) return launch_batch_tz(P, tab, (const __hip_bfloat16 *)bank, go);
+  if (bank_dtype == QPSDM_F16) return launch_batch_tz(P, tab, (const __half *)bank, go);
   return qpsdm::set_error(QPSDM_EINVAL, "image_batch: bank dtype %d", bank_dtype);
 }
 

@@ -82,6 +82,17 @@ int launch_tz(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geo
   }
 }
 
+template <typename Tab>
+int launch_bank(const qk::ImgDev &P, const Tab &T, float *img, const float4 *g4, const void *bank, int bank_dtype,
+                hipStream_t s) {
+  switch (bank_dtype) {
+    case QPSDM_F32: return launch_tz(P, T, img, g4, (const float *)bank, s);
+    case QPSDM_BF16: return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
+    case QPSDM_F16: return launch_tz(P, T, img, g4, (const __half *)bank, s);
+    default: return qpsdm::set_error(QPSDM_EINVAL, "image: bank dtype %d", bank_dtype);
+  }
+}
+
 }  // namespace
 
 extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img,
@@ -94,30 +105,12 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
   if (ntr == 0) return QPSDM_OK;
   hipStream_t s = (hipStream_t)stream;
   const float4 *g4 = (const float4 *)geom;
-  if (table_dtype == QPSDM_F32) {
-    qk::TabF32 T = qk::make_tab_f32(p->tg, tab);
-    if (bank_dtype == QPSDM_F32)
-      return launch_tz(P, T, img, g4, (const float *)bank, s);
-    if (bank_dtype == QPSDM_BF16)
-      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
-  } else if (table_dtype == QPSDM_F16B) {
-    auto T = qk::make_tab_b16<false>(p->tg, tab);
-    if (bank_dtype == QPSDM_F32)
-      return launch_tz(P, T, img, g4, (const float *)bank, s);
-    if (bank_dtype == QPSDM_BF16)
-      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
-  } else if (table_dtype == QPSDM_U16B) {
-    auto T = qk::make_tab_b16<true>(p->tg, tab);
-    if (bank_dtype == QPSDM_F32)
-      return launch_tz(P, T, img, g4, (const float *)bank, s);
-    if (bank_dtype == QPSDM_BF16)
-      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
-  } else if (table_dtype == QPSDM_U16X) {
-    auto T = qk::make_tab_u16x(p->tg, tab);
-    if (bank_dtype == QPSDM_F32)
-      return launch_tz(P, T, img, g4, (const float *)bank, s);
-    if (bank_dtype == QPSDM_BF16)
-      return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
+  switch (table_dtype) {
+    case QPSDM_F32: return launch_bank(P, qk::make_tab_f32(p->tg, tab), img, g4, bank, bank_dtype, s);
+    case QPSDM_F16B: return launch_bank(P, qk::make_tab_b16<false>(p->tg, tab), img, g4, bank, bank_dtype, s);
+    case QPSDM_U16B: return launch_bank(P, qk::make_tab_b16<true>(p->tg, tab), img, g4, bank, bank_dtype, s);
+    case QPSDM_U16X: return launch_bank(P, qk::make_tab_u16x(p->tg, tab), img, g4, bank, bank_dtype, s);
+    default: break;
   }
   return qpsdm::set_error(QPSDM_EINVAL, "image: bad dtypes bank=%d table=%d", bank_dtype,
                           table_dtype);

@@ -498,6 +498,10 @@ template <>
 struct BankRaw<__hip_bfloat16> {
   typedef unsigned type;
 };
+template <>
+struct BankRaw<__half> {
+  typedef unsigned type;
+};
 
 __device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const float *) {
   return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0),
@@ -506,12 +510,41 @@ __device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const float 
 __device__ inline unsigned bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const __hip_bfloat16 *) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, idx * 2, 0, 0);
 }
+__device__ inline unsigned bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const __half *) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, idx * 2, 0, 0);
+}
 __device__ inline float2 bank_decode(uint2 v, const float *) {
   return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
 __device__ inline float2 bank_decode(unsigned v, const __hip_bfloat16 *) {
   return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
 }
+__device__ inline float2 bank_decode(unsigned v, const __half *) {
+  return make_float2(__half2float(__ushort_as_half((unsigned short)(v & 0xffffu))),
+                     __half2float(__ushort_as_half((unsigned short)(v >> 16))));
+}
+
+// Row `row` of a bank of K x nt samples per trace: a buffer resource over
+// its samples, and in *unit the factor from stored samples to filtered-trace
+// units (1 for FP32 and BF16; for FP16 the 1/scale of the row's header,
+// read on the scalar unit).
+__device__ inline __amdgpu_buffer_rsrc_t bank_row(const float *bank, size_t row, int K, int nt, float *unit) {
+  *unit = 1.0f;
+  return make_rsrc(bank + row * K * nt, (unsigned)(K * nt * 4));
+}
+__device__ inline __amdgpu_buffer_rsrc_t bank_row(const __hip_bfloat16 *bank, size_t row, int K, int nt,
+                                                  float *unit) {
+  *unit = 1.0f;
+  return make_rsrc(bank + row * K * nt, (unsigned)(K * nt * 2));
+}
+__device__ inline __amdgpu_buffer_rsrc_t bank_row(const __half *bank, size_t row, int K, int nt, float *unit) {
+  typedef const __attribute__((address_space(4))) float *cfp;
+  const size_t halves = (16 + ((size_t)K * nt * 2 + 15) / 16 * 16) / 2;  // qpsdm_bank_row_bytes / 2
+  const __half *r = bank + row * halves;
+  *unit = *(cfp)r;
+  return make_rsrc(r + 8, (unsigned)(K * nt * 2));
+}
+
 template <typename BankT>
 __device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const BankT *b) {
   return bank_decode(bank_raw(r, idx, b), b);

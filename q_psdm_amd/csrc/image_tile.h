@@ -17,6 +17,8 @@
 #ifndef QPSDM_IMAGE_TILE_H_
 #define QPSDM_IMAGE_TILE_H_
 
+#include <type_traits>
+
 #include "image_dev.h"
 
 namespace qpsdm_dev {
@@ -136,7 +138,6 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
   SrcCache<ZP> S;
   float sx = NAN, sy = NAN;
   int cache_run = -1;
-  const unsigned bank_bytes = (unsigned)(P.K * P.nt * sizeof(BankT));
   const unsigned nt = (unsigned)P.nt;
   const float tmin = __builtin_canonicalizef(P.tmin);  // lets fmaxf skip re-quieting it per point
   const float jmax = (float)(P.K - 1);
@@ -177,8 +178,8 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;
       auto c = column_setup(P, tab, g.z, g.w, x, y);
-      const __amdgpu_buffer_rsrc_t br =
-          make_rsrc(bank + (size_t)(t - run.tr0 + run.brow0) * P.K * P.nt, bank_bytes);
+      float unit;  // stored sample -> filtered-trace units (FP16 banks carry a per-trace scale)
+      const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
       int cur = wz[0];
       float2 lo = tab.at(c, cur), hi = tab.at(c, cur + 1);
       float2 dl = make_float2(hi.x - lo.x, hi.y - lo.y);
@@ -218,7 +219,10 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
           ci[i] += (int)__umul24((unsigned)j, nt);
           const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
-          cw[i] = w * (tr + __builtin_amdgcn_rcpf(tr));
+          if constexpr (std::is_same<BankT, __half>::value)
+            cw[i] = (w * unit) * (tr + __builtin_amdgcn_rcpf(tr));
+          else
+            cw[i] = w * (tr + __builtin_amdgcn_rcpf(tr));
         }
       }
       typename BankRaw<BankT>::type cb[ZP];

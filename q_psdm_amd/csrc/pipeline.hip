@@ -74,7 +74,7 @@ struct qpsdm_ctx {
   bool have_tab = false;
   Slot slot[kSlots];
   int cur = 0;
-  size_t rec_bytes = 0, slot_bytes = 0, bank_elem = 0;
+  size_t rec_bytes = 0, slot_bytes = 0;
   long long pushed = 0, rejected_host = 0;
 };
 
@@ -136,7 +136,8 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
   QPSDM_REQUIRE(p->batch_traces >= 1 && p->batch_traces <= (1 << 20), "ctx_create: batch_traces in [1, 2^20]");
   QPSDM_REQUIRE(p->image.nt == p->bank.nt && p->image.K == p->bank.K && p->image.dtstar == p->bank.dtstar,
                 "ctx_create: image and bank parameters disagree on nt/K/dtstar");
-  QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16, "ctx_create: bank dtype");
+  QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16 || p->bank_dtype == QPSDM_F16,
+                "ctx_create: bank dtype");
   QPSDM_REQUIRE(p->table_dtype == QPSDM_F32 || p->table_dtype == QPSDM_F16B || p->table_dtype == QPSDM_U16B ||
                     p->table_dtype == QPSDM_U16X,
                 "ctx_create: table dtype");
@@ -150,8 +151,7 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
   const int nt = p->image.nt, B = p->batch_traces;
   c->rec_bytes = 240 + 4 * (size_t)nt;
   c->slot_bytes = (size_t)B * (c->rec_bytes > (size_t)(nt + 4) * 4 ? c->rec_bytes : (size_t)(nt + 4) * 4);
-  c->bank_elem = (size_t)B * p->bank.K * nt;
-  const size_t esz = p->bank_dtype == QPSDM_F32 ? 4 : 2;
+  const size_t bank_bytes = (size_t)B * qpsdm_bank_row_bytes(p->bank.K, nt, p->bank_dtype);
   const size_t img_bytes = (size_t)p->image.nx * p->image.ny * p->image.nz * 4;
   if (hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->compute, hipStreamNonBlocking) != hipSuccess ||
@@ -162,7 +162,7 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
         hipMalloc(&s.raw, c->slot_bytes) != hipSuccess ||
         hipMalloc(&s.samples, (size_t)B * nt * 4) != hipSuccess ||
         hipMalloc(&s.geom, (size_t)B * 16) != hipSuccess ||
-        hipMalloc(&s.bank, c->bank_elem * esz) != hipSuccess ||
+        hipMalloc(&s.bank, bank_bytes) != hipSuccess ||
         hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
       return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: slot allocation failed"));

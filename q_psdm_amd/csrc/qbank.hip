@@ -14,12 +14,14 @@
 //      segments as one Hermitian pair, inverse FFT, and write the level's
 //      time trace, segment 1 up to tdmid and segment 2 after it.
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -40,6 +42,8 @@ struct BankDev {
   int nsb, itb1, itb2, nf1, nww;
   float tdmid;
   int K;
+  long long row;        // output elements between consecutive traces
+  int lead;             // output elements of the row header before level 0
   Plan plan;
   const float2 *tw;     // [nsb] exp(-2 pi i a / nsb)
   const float2 *H;      // [K][nww]
@@ -137,9 +141,26 @@ __device__ inline float gain_t2(float x, int it, float dt) {
 
 __device__ inline void store_out(float *p, float v) { *p = v; }
 __device__ inline void store_out(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+__device__ inline void store_out(__half *p, float v) { *p = __float2half(v); }
+
+// Largest value over the work-group (every thread gets it).
+__device__ inline float block_max(float m) {
+  __shared__ float red[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = red[0];
+  for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, red[w]);
+  return m;
+}
 
 // MODE 0: spectra only (re/im of the reference pre-processing).
-// MODE 1: full bank into OutT.
+// MODE 1: full bank into OutT.  FP16 output is scaled per trace by
+// 2^(15 - ex), ex the binary exponent (frexp) of the trace's largest
+// |sample| over all levels, found by a first pass over the levels: every
+// scaled sample is below 2^15, and the power of two is exact both ways.  The
+// row's 16-byte header holds {1/scale, scale, 0, 0}.
 template <int MODE, typename OutT>
 __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float *__restrict__ traces,
                                                          OutT *__restrict__ bank,
@@ -188,7 +209,9 @@ __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float 
   }
   if (MODE == 0) return;
   __syncthreads();
-  for (int j = 0; j < P.K; ++j) {
+  // level j: filter both windows' spectra, inverse FFT as one Hermitian
+  // pair; sample it of the level is then level_sample(Y, it)
+  auto level = [&](int j) {
     const float2 *Hj = P.H + (size_t)j * P.nww;
     for (int n = threadIdx.x; n < N; n += blockDim.x) {
       float2 v = make_float2(0.f, 0.f);
@@ -205,15 +228,45 @@ __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float 
       A[n] = v;
     }
     __syncthreads();
-    const float2 *Y = fft_run<+1>(A, B, tw, N, P.plan);
-    OutT *o = bank + ((size_t)tr * P.K + j) * P.nt;
-    for (int it = threadIdx.x; it < P.nt; it += blockDim.x) {
-      const bool s2 = it * P.dt > P.tdmid;
-      const int n = it - (s2 ? P.itb2 : P.itb1);
-      float v = 0.f;
-      if (n >= 0 && n < N) v = s2 ? Y[n].y : Y[n].x;
-      store_out(o + it, v);
+    return fft_run<+1>(A, B, tw, N, P.plan);
+  };
+  auto level_sample = [&](const float2 *Y, int it) {
+    const bool s2 = it * P.dt > P.tdmid;
+    const int n = it - (s2 ? P.itb2 : P.itb1);
+    return (n >= 0 && n < N) ? (s2 ? Y[n].y : Y[n].x) : 0.0f;
+  };
+  float scale = 1.0f;
+  if constexpr (std::is_same<OutT, __half>::value) {
+    float m = 0.0f;
+    for (int j = 0; j < P.K; ++j) {
+      const float2 *Y = level(j);
+      for (int it = threadIdx.x; it < P.nt; it += blockDim.x) m = fmaxf(m, fabsf(level_sample(Y, it)));
+      __syncthreads();
     }
+    m = block_max(m);
+    int ex = 0;
+    if (m > 0.0f && m <= 3.4e38f) {
+      (void)frexpf(m, &ex);  // m in [2^(ex-1), 2^ex)
+      ex = max(-100, min(ex, 100));
+    }
+    scale = ldexpf(1.0f, 15 - ex);
+    if (threadIdx.x == 0) {
+      float *h = reinterpret_cast<float *>(bank + (size_t)tr * P.row);
+      h[0] = ldexpf(1.0f, ex - 15);
+      h[1] = scale;
+      h[2] = 0.0f;
+      h[3] = 0.0f;
+    }
+    // the row's tail padding (to a 16-byte multiple) is written too, so the
+    // output is fully determined by the input
+    const long long used = P.lead + (long long)P.K * P.nt;
+    for (long long i = used + threadIdx.x; i < P.row; i += blockDim.x)
+      bank[(size_t)tr * P.row + i] = __float2half(0.0f);
+  }
+  for (int j = 0; j < P.K; ++j) {
+    const float2 *Y = level(j);
+    OutT *o = bank + (size_t)tr * P.row + P.lead + (size_t)j * P.nt;
+    for (int it = threadIdx.x; it < P.nt; it += blockDim.x) store_out(o + it, level_sample(Y, it) * scale);
     __syncthreads();
   }
 }
@@ -336,6 +389,12 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
   P.nww = p->win.nww;
   P.tdmid = p->win.tdmid;
   P.K = p->K;
+  P.row = (long long)p->K * p->nt;
+  P.lead = 0;
+  if (dtype == QPSDM_F16) {
+    P.row = (long long)(qpsdm_bank_row_bytes(p->K, p->nt, QPSDM_F16) / 2);
+    P.lead = 8;  // the 16-byte header
+  }
   P.plan = c->plan;
   P.tw = c->tw;
   P.H = c->H;
@@ -352,8 +411,11 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
   } else if (dtype == QPSDM_BF16) {
     hipLaunchKernelGGL((qbank_kernel<1, __hip_bfloat16>), dim3(ntr), dim3(kThreads), lds, s, P,
                        traces, (__hip_bfloat16 *)bank, (float *)nullptr, (float *)nullptr);
+  } else if (dtype == QPSDM_F16) {
+    hipLaunchKernelGGL((qbank_kernel<1, __half>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
+                       (__half *)bank, (float *)nullptr, (float *)nullptr);
   } else {
-    return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16", dtype);
+    return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16", dtype);
   }
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
@@ -374,6 +436,17 @@ extern "C" int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *trac
                                    float *re, float *im, void *stream) {
   QPSDM_REQUIRE(re && im, "spectra: null outputs");
   return launch(p, traces, ntr, nullptr, QPSDM_F32, re, im, stream);
+}
+
+extern "C" size_t qpsdm_bank_row_bytes(int K, int nt, int dtype) {
+  if (K <= 0 || nt <= 0) return 0;
+  const size_t n = (size_t)K * nt;
+  switch (dtype) {
+    case QPSDM_F32: return n * 4;
+    case QPSDM_BF16: return n * 2;
+    case QPSDM_F16: return 16 + (n * 2 + 15) / 16 * 16;
+    default: return 0;
+  }
 }
 
 extern "C" int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,

@@ -136,13 +136,36 @@ class Survey:
                                           _stream(stream)))
         return re, im
 
+    _BANK_CODES = {torch.float32: L.QPSDM_F32, torch.bfloat16: L.QPSDM_BF16, torch.float16: L.QPSDM_F16}
+
+    def bank_shape(self, n, dtype=torch.float32):
+        """Tensor shape of an n-trace bank: (n, K, nt) for FP32 and BF16; for
+        FP16 (n, row) halves per trace, a 16-byte header {1/scale, scale}
+        then the K x nt scaled samples."""
+        c = self.cfg
+        if dtype == torch.float16:
+            return (n, L.lib().qpsdm_bank_row_bytes(c.K, c.nt, L.QPSDM_F16) // 2)
+        return (n, c.K, c.nt)
+
+    def _check_bank(self, bank, rows):
+        assert bank.dtype in self._BANK_CODES and bank.is_contiguous() and bank.shape[0] >= rows
+        assert tuple(bank.shape[1:]) == self.bank_shape(1, bank.dtype)[1:]
+
+    def decode_bank(self, bank):
+        """Bank in filtered-trace units as FP32 (n, K, nt) (host-side check)."""
+        if bank.dtype != torch.float16:
+            return bank.float()
+        c = self.cfg
+        hdr = bank[:, :8].contiguous().view(torch.float32)  # [n][4]: 1/scale, scale, 0, 0
+        return bank[:, 8:8 + c.K * c.nt].float().reshape(-1, c.K, c.nt) * hdr[:, 0].reshape(-1, 1, 1)
+
     def qbank(self, traces, dtype=torch.float32, out=None, stream=None):
         n = traces.shape[0]
         assert traces.dtype == torch.float32 and traces.dim() == 2 and traces.shape[1] == self.cfg.nt
         if out is None:
-            out = torch.empty((n, self.cfg.K, self.cfg.nt), dtype=dtype, device="cuda")
-        assert out.shape == (n, self.cfg.K, self.cfg.nt) and out.is_contiguous()
-        code = L.QPSDM_F32 if out.dtype == torch.float32 else L.QPSDM_BF16
+            out = torch.empty(self.bank_shape(n, dtype), dtype=dtype, device="cuda")
+        self._check_bank(out, n)
+        code = self._BANK_CODES[out.dtype]
         check(L.lib().qpsdm_qbank(C.byref(self.bp), _ptr(traces), n, _ptr(out), code, _stream(stream)))
         return out
 
@@ -157,7 +180,7 @@ class Survey:
         c = self.cfg
         assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
         assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
-        assert bank.dim() == 3 and bank.shape[0] >= ntr and bank.shape[1:] == (c.K, c.nt)
+        self._check_bank(bank, ntr)
         bcode, tcode = self._codes(bank, tab)
         lp = None
         if launch is not None:
@@ -186,8 +209,7 @@ class Survey:
         else:
             tcode = getattr(tab, "qpsdm_code", L.QPSDM_F16B)
             assert tab.numel() == L.lib().qpsdm_tables_bytes(C.byref(self.tg), tcode)
-        bcode = L.QPSDM_F32 if bank.dtype == torch.float32 else L.QPSDM_BF16
-        return bcode, tcode
+        return self._BANK_CODES[bank.dtype], tcode
 
     def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, src=None, stream=None):
         """Accumulate every shot of a batch into img with the persistent
@@ -196,7 +218,7 @@ class Survey:
         c = self.cfg
         assert img.shape == (c.ny, c.nx, c.nz) and img.dtype == torch.float32
         assert geom.dim() == 2 and geom.shape[1] == 4 and shots.rows <= geom.shape[0]
-        assert bank.dim() == 3 and bank.shape[0] >= shots.rows and bank.shape[1:] == (c.K, c.nt)
+        self._check_bank(bank, shots.rows)
         bcode, tcode = self._codes(bank, tab)
         if work is None:
             work = torch.empty(1, dtype=torch.int32, device="cuda")
@@ -249,7 +271,8 @@ class BatchGraph:
                  tab_f32=None, mask=None, src=None, stages=7):
         c = survey.cfg
         assert traces.dtype == torch.float32 and traces.shape[1] == c.nt and traces.is_contiguous()
-        assert bank.shape == (traces.shape[0], c.K, c.nt) and img.shape == (c.ny, c.nx, c.nz)
+        survey._check_bank(bank, traces.shape[0])
+        assert img.shape == (c.ny, c.nx, c.nz)
         assert shots.rows <= traces.shape[0] <= geom.shape[0]
         bcode, tcode = survey._codes(bank, tab)
         self.work = torch.empty(1, dtype=torch.int32, device="cuda")
@@ -347,7 +370,7 @@ class Pipeline:
             geom_params = L.GeomParams(1.0, -1e30, 1e30, -1e30, 1e30, 0.0, 0.0, 0.0, 0.0,
                                        survey.cfg.dx, survey.cfg.dy)
         p.geom = geom_params
-        p.bank_dtype = L.QPSDM_F32 if bank_dtype == "f32" else L.QPSDM_BF16
+        p.bank_dtype = {"f32": L.QPSDM_F32, "bf16": L.QPSDM_BF16, "f16": L.QPSDM_F16}[bank_dtype]
         p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B,
                          "u16x": L.QPSDM_U16X}[table_dtype]
         p.batch_traces = batch_traces

@@ -101,24 +101,29 @@ def test_batch_matches_fp64_oracle_and_count(gpu):
     assert abs(cnt - nref) <= 1e-5 * nref + 2
 
 
-def test_batch_mixed_precision_matches_per_shot(gpu):
+@pytest.mark.parametrize("bank_dtype,code", [(torch.bfloat16, "u16"), (torch.float16, "u16x"),
+                                             (torch.bfloat16, "u16x")])
+def test_batch_mixed_precision_matches_per_shot(gpu, bank_dtype, code):
     cfg = synth.tiny()
     sv, tab = _survey(cfg)
     geom, ranges, desc = _uneven_batch(cfg, [6, 6, 6], [30.0, 45.0, 60.0], seed=8)
     traces = np.random.default_rng(2).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
-    bank = sv.qbank(torch.from_numpy(traces).cuda(), dtype=torch.bfloat16)
-    q = sv.quantize_tables(tab, code="u16")
+    traces[4] *= 50.0  # rows with different scales in the FP16 bank
+    bank = sv.qbank(torch.from_numpy(traces).cuda(), dtype=bank_dtype)
+    q = sv.quantize_tables(tab, code=code)
     g_d = torch.from_numpy(geom).cuda()
     ref = sv.new_image()
     _per_shot(sv, ref, g_d, bank, q, desc)
-    img = sv.new_image()
-    sv.image_batch(img, g_d, bank, q, engine.ShotList(desc))
-    torch.cuda.synchronize()
-    assert torch.equal(img, ref)
+    for nwg in (0, -1):
+        img = sv.new_image()
+        sv.image_batch(img, g_d, bank, q, engine.ShotList(desc), nwg=nwg)
+        torch.cuda.synchronize()
+        assert torch.equal(img, ref), nwg
 
 
-def test_graph_replays_match_eager(gpu):
-    """The captured [Q bank -> persistent imaging] graph, replayed three
+@pytest.mark.parametrize("bank_dtype,nwg", [(torch.float32, 0), (torch.float16, -1)])
+def test_graph_replays_match_eager(gpu, bank_dtype, nwg):
+    """The captured [Q bank -> batched imaging] graph, replayed three
     times with the program's own eager work (new traces, image reset,
     synchronisations) between replays; each replay equals the eager result."""
     cfg = synth.tiny_layers()
@@ -127,10 +132,10 @@ def test_graph_replays_match_eager(gpu):
     ntr = geom.shape[0]
     g_d = torch.from_numpy(geom).cuda()
     traces = torch.zeros((ntr, cfg.nt), dtype=torch.float32, device="cuda")
-    bank = torch.empty((ntr, cfg.K, cfg.nt), dtype=torch.float32, device="cuda")
+    bank = torch.empty(sv.bank_shape(ntr, bank_dtype), dtype=bank_dtype, device="cuda")
     img = sv.new_image()
     shots = engine.ShotList(desc)
-    g = engine.BatchGraph(sv, traces, bank, img, g_d, tab, shots)
+    g = engine.BatchGraph(sv, traces, bank, img, g_d, tab, shots, nwg=nwg)
     try:
         for rep in range(3):
             x = torch.from_numpy(np.random.default_rng(100 + rep).standard_normal((ntr, cfg.nt))
@@ -142,8 +147,8 @@ def test_graph_replays_match_eager(gpu):
             torch.cuda.synchronize()
             got = img.clone()
             eager = sv.new_image()
-            b = sv.qbank(x)
-            sv.image_batch(eager, g_d, b, tab, shots)
+            b = sv.qbank(x, dtype=bank_dtype)
+            sv.image_batch(eager, g_d, b, tab, shots, nwg=nwg)
             torch.cuda.synchronize()
             assert torch.equal(b, bank), rep
             assert torch.equal(got, eager), rep

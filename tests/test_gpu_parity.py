@@ -64,6 +64,36 @@ def test_qbank_bf16_rounds_fp32(gpu):
     assert rel_l2(b16.cpu().numpy(), b32.cpu().numpy()) < 4e-3
 
 
+@pytest.mark.parametrize("mk", [synth.tiny, synth.config1])
+def test_qbank_f16_scaled_rows(gpu, mk):
+    """FP16 bank: each row's header holds a power-of-two scale and its
+    inverse, every scaled sample is below 2^15, and the decoded bank matches
+    the FP32 bank and the FP64 oracle to FP16 rounding."""
+    cfg = mk()
+    sv, tab = _survey(cfg)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal((6, cfg.nt)).astype(np.float32)
+    x[1] *= 1e-3   # traces of very different amplitudes get their own scales
+    x[2] *= 1e4
+    tr = torch.from_numpy(x).cuda()
+    b16 = sv.qbank(tr, dtype=torch.float16)
+    assert tuple(b16.shape) == sv.bank_shape(6, torch.float16)
+    hdr = b16[:, :8].contiguous().view(torch.float32).cpu().numpy()
+    samples = b16[:, 8:8 + cfg.K * cfg.nt].float().cpu().numpy()
+    for i in range(6):
+        inv, sc = float(hdr[i, 0]), float(hdr[i, 1])
+        e = np.log2(sc)
+        assert e == np.round(e) and inv * sc == 1.0 and hdr[i, 2] == 0 and hdr[i, 3] == 0
+        m = np.abs(samples[i]).max()
+        assert 2.0 ** 13 <= m < 2.0 ** 15, (i, m)   # max scaled into [2^14, 2^15) up to FP16 rounding
+    got = sv.decode_bank(b16).cpu().numpy()
+    ref32 = sv.qbank(tr).cpu().numpy()
+    ref64 = O.qbank(x, o_bparams(sv))
+    for i in range(6):
+        assert rel_l2(got[i], ref32[i]) < 5e-4, i
+        assert rel_l2(got[i], ref64[i]) < 5e-4, i
+
+
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
 def test_image_matches_oracle(gpu, mk):
     cfg = mk()
@@ -267,13 +297,15 @@ def test_mixed_precision_image_vs_fp32_and_fp64(gpu):
     n = geom.shape[0]
     b32 = sv.qbank(tr_d)
     b16 = sv.qbank(tr_d, dtype=torch.bfloat16)
+    bh = sv.qbank(tr_d, dtype=torch.float16)
     q = sv.quantize_tables(tab)
     qu = sv.quantize_tables(tab, code="u16")
     qx = sv.quantize_tables(tab, code="u16x")
     half = n // 2
     imgs = {}
-    for name, bank, tb in (("fp32", b32, tab), ("bf16bank", b16, tab), ("f16tab", b32, q), ("u16tab", b32, qu),
-                           ("u16xtab", b32, qx), ("mixed", b16, q), ("mixed_u16", b16, qu), ("mixed_u16x", b16, qx)):
+    for name, bank, tb in (("fp32", b32, tab), ("bf16bank", b16, tab), ("f16bank", bh, tab), ("f16tab", b32, q),
+                           ("u16tab", b32, qu), ("u16xtab", b32, qx), ("mixed", b16, q), ("mixed_u16", b16, qu),
+                           ("mixed_u16x", b16, qx), ("mixed_f16_u16x", bh, qx)):
         img = sv.new_image()
         for tr0, m in ((0, half), (half, n - half)):
             sv.image_shot(img, g_d, tr0, m, bank[tr0:tr0 + m], tb)
@@ -288,3 +320,34 @@ def test_mixed_precision_image_vs_fp32_and_fp64(gpu):
     assert err["mixed_u16"][0] < 1e-3      # the mixed form's stated bound vs FP64
     assert err["u16xtab"][0] < 2e-4
     assert err["mixed_u16x"][0] < 1e-3
+    assert err["f16bank"][0] < 3e-4
+    assert err["mixed_f16_u16x"][0] < 5e-4
+
+
+def test_mixed_forms_on_incoherent_traces(gpu):
+    """White-noise traces image to an incoherent sum, so the bank's rounding
+    passes into the image at its own relative size: BF16 samples (8-bit
+    significand) leave about 1e-3, FP16 samples under a per-trace scale
+    (11-bit) about 1e-4.  Stated against the FP64 oracle."""
+    cfg = synth.tiny_layers()
+    sv, tab = _survey(cfg)
+    geom, ranges = synth.geometry(synth.shots_tiny(3, 8, extent=((cfg.nx - 1) * cfg.dx, (cfg.ny - 1) * cfg.dy)))
+    x = np.random.default_rng(23).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
+    tr_d, g_d = torch.from_numpy(x).cuda(), torch.from_numpy(geom).cuda()
+    qx = sv.quantize_tables(tab, code="u16x")
+    imgs = {}
+    for name, dt, tb in (("fp32", torch.float32, tab), ("bf16_u16x", torch.bfloat16, qx),
+                         ("f16_u16x", torch.float16, qx), ("f16", torch.float16, tab)):
+        bank = sv.qbank(tr_d, dtype=dt)
+        img = sv.new_image()
+        for tr0, n in ranges:
+            sv.image_shot(img, g_d, tr0, n, bank[tr0:tr0 + n], tb)
+        imgs[name] = img.cpu().numpy()
+    ob = O.qbank(x, o_bparams(sv))
+    ref, _ = O.image(ob, geom, tab.cpu().numpy(), o_iparams(sv))
+    err = {k: rel_l2(v, ref) for k, v in imgs.items()}
+    print("incoherent data, rel-L2 vs FP64: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()))
+    assert err["fp32"] < 1e-4
+    assert err["f16"] < 4e-4
+    assert err["f16_u16x"] < 1e-3       # the mixed form's bound holds with FP16 samples
+    assert err["bf16_u16x"] > err["f16_u16x"]
