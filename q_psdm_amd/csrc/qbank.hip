@@ -157,9 +157,14 @@ __device__ inline float block_max(float m) {
 
 // MODE 0: spectra only (re/im of the reference pre-processing).
 // MODE 1: full bank into OutT.  FP16 output is scaled per trace by
-// 2^(15 - ex), ex the binary exponent (frexp) of the trace's largest
-// |sample| over all levels, found by a first pass over the levels: every
-// scaled sample is below 2^15, and the power of two is exact both ways.  The
+// 2^(15 - ex), ex the binary exponent (frexp) of a bound on the trace's
+// largest |sample| over all levels: a level's samples come from an
+// unnormalised inverse FFT of a spectrum whose nonzero bins are
+// s1 + i s2 = (C1 + i C2) H at the kept bins and their mirror images, so
+// |sample| <= sum over bins of |bin| <= 2 sum_k |H_jk| (|C1_k| + |C2_k|).
+// Every scaled sample is then below 2^15, the power of two is exact both
+// ways, and FP16's relative precision does not depend on how far below the
+// bound the samples lie (they stay far above its subnormal range).  The
 // row's 16-byte header holds {1/scale, scale, 0, 0}.
 template <int MODE, typename OutT>
 __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float *__restrict__ traces,
@@ -237,13 +242,24 @@ __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float 
   };
   float scale = 1.0f;
   if constexpr (std::is_same<OutT, __half>::value) {
+    // eight lanes per level, each over every eighth bin, then summed across
+    // the eight (kThreads is a multiple of 64, so every pass is wave-uniform)
     float m = 0.0f;
-    for (int j = 0; j < P.K; ++j) {
-      const float2 *Y = level(j);
-      for (int it = threadIdx.x; it < P.nt; it += blockDim.x) m = fmaxf(m, fabsf(level_sample(Y, it)));
-      __syncthreads();
+    for (int base = 0; base < P.K * 8; base += kThreads) {
+      const int p = base + threadIdx.x;
+      float b = 0.0f;
+      if (p < P.K * 8) {
+        const float2 *Hj = P.H + (size_t)(p >> 3) * P.nww;
+        for (int k = p & 7; k < P.nww; k += 8)
+          b = fmaf(hypotf(Hj[k].x, Hj[k].y), hypotf(C1[k].x, C1[k].y) + hypotf(C2[k].x, C2[k].y), b);
+      }
+      b += __shfl_xor(b, 1, 64);
+      b += __shfl_xor(b, 2, 64);
+      b += __shfl_xor(b, 4, 64);
+      m = fmaxf(m, 2.0f * b);
     }
     m = block_max(m);
+    m *= 1.001f;  // margin for the rounding of the bound's own sum
     int ex = 0;
     if (m > 0.0f && m <= 3.4e38f) {
       (void)frexpf(m, &ex);  // m in [2^(ex-1), 2^ex)
