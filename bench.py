@@ -335,6 +335,33 @@ def cpu_baseline(W, ntr):
 
 # ---- HBM traffic from counter runs --------------------------------------------
 
+def time_segy_decode(torch, ntr=20000, ns=2001, reps=5):
+    """Device decode of SEG-Y IBM records (random words, valid ns keys):
+    ntr records per launch, HIP events on the launch stream.  Algorithmic
+    bytes per trace: the record read (240 + 4 ns) and the samples and header
+    subset written (4 ns + 40)."""
+    from q_psdm_amd import segy
+    rec = 240 + 4 * ns
+    raw = torch.randint(0, 256, (ntr, rec), dtype=torch.uint8, device="cuda")
+    raw[:, 114] = ns >> 8
+    raw[:, 115] = ns & 0xff
+    raw = raw.reshape(-1)
+    segy.decode_dev(raw, ntr, ns, segy.IBM)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        segy.decode_dev(raw, ntr, ns, segy.IBM)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    per = rec + 4 * ns + 40
+    gbs = ntr * per / (ms * 1e-3) / 1e9
+    log(f"[segy] IBM decode of {ntr} traces x {ns} samples: {ms:.3f} ms, {gbs:.0f} GB/s")
+    return {"kernel": "segy_decode_kernel<1> (IBM)", "traces_per_launch": ntr, "ns": ns, "avg_launch_ms": ms,
+            "algorithmic_bytes_per_trace": per, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
 def probe(args):
     """Child of a rocprofv3 counter run: one step of the fast form with an
     untrained surrogate (the imaging kernel's traffic does not depend on the
@@ -482,6 +509,7 @@ def main():
         else:
             results[head]["traffic_note"] = err
     cpu = None if args.no_cpu else cpu_baseline(W, args.cpu_traces)
+    seg = time_segy_decode(torch)
     R = results[head]
     out = {
         "metric": "image-point x trace updates/s @1 GPU (FP32 vs mixed+surrogate+batched)",
@@ -521,6 +549,7 @@ def main():
         "speedup_vs_fp32": (R["value"] / results["fp32"]["value"]) if "fp32" in results else None,
         "image_rel_l2": errors,
         "surrogate": sinfo,
+        "segy_decode": seg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

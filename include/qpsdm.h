@@ -379,6 +379,12 @@ int qpsdm_ctx_set_tables(qpsdm_ctx *c, const float *tab, int on_device);
 int qpsdm_ctx_build_tables(qpsdm_ctx *c, const qpsdm_model *m);
 /* Big-endian SU records with ns = nt (whole records only). */
 int qpsdm_ctx_push_su(qpsdm_ctx *c, const void *bytes, size_t nbytes);
+/* SEG-Y rev0 trace records (no 3600-byte reel header) with ns = nt, format
+ * 1 (IBM float) or 5 (IEEE float), header keys at keyidx (struct keyindex
+ * [75] of segy.h; NULL: standard positions), decoded on the device as
+ * fsgettr decodes them (src/segy.cc:350-1547). */
+struct keyindex;
+int qpsdm_ctx_push_segy(qpsdm_ctx *c, const void *bytes, size_t nbytes, int format, const struct keyindex *keyidx);
 /* Decoded host traces [ntr][nt] with image-relative geometry [ntr][4]. */
 int qpsdm_ctx_push_traces(qpsdm_ctx *c, const float *samples, const float *geom, int ntr);
 /* One decoded trace with its header, as psdm_kernel(segy tr, ...) takes it

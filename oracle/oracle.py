@@ -88,6 +88,11 @@ def lib():
         L.or_num_threads.restype = C.c_int
         L.or_set_num_threads.restype = None
         L.or_set_num_threads.argtypes = [C.c_int]
+        L.or_ibm_to_ieee.restype = C.c_uint32
+        L.or_ibm_to_ieee.argtypes = [C.c_uint32]
+        L.or_segy_key.restype = C.c_longlong
+        L.or_segy_key.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int]
+        L.or_segy_samples.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
@@ -260,3 +265,22 @@ def num_threads():
 
 def set_num_threads(n):
     lib().or_set_num_threads(int(n))
+
+
+def ibm_to_ieee(words):
+    """FP32 bits of big-endian-decoded IBM words (uint32 array)."""
+    w = np.ascontiguousarray(words, np.uint32).ravel()
+    f = lib().or_ibm_to_ieee
+    return np.array([f(int(x)) for x in w], np.uint32)
+
+
+def segy_samples(record, ns, fmt):
+    """FP32 samples of one SEG-Y record (bytes: 240-byte header + samples)."""
+    out = np.zeros(ns, np.uint32)
+    rc = lib().or_segy_samples(bytes(record), ns, fmt, out.ctypes.data_as(C.c_void_p))
+    assert rc == 0, fmt
+    return out.view(np.float32)
+
+
+def segy_key(header, start, nbytes, kind):
+    return lib().or_segy_key(bytes(header[:240]), start, nbytes, kind)

@@ -686,3 +686,58 @@ int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
     }
   return count;
 }
+
+/* ---------------------------------------------------------------------- */
+/* SEG-Y rev0 trace decode (src/segy.cc:350-1547, the format branches at
+ * 1496-1545).  ibm2ieee as the reference states it: zero fraction -> 0;
+ * else shift the 24-bit fraction left one bit at a time until bit 23 is set,
+ * decrementing the binary exponent 4*(E-64); drop the implicit bit, bias by
+ * 126, replace an exponent outside [0, 255] by 0. */
+uint32_t or_ibm_to_ieee(uint32_t w) {
+  if ((w & 0xffffffu) == 0) return 0u;
+  uint32_t mant = w & 0xffffffu;
+  int ex = ((int)((w & 0x7f000000u) >> 24) - 64) * 4;
+  const uint32_t sign = w & 0x80000000u;
+  while ((mant & 0x800000u) == 0) {
+    mant <<= 1;
+    ex--;
+  }
+  mant &= 0x7fffffu;
+  ex += 126;
+  if (ex < 0 || ex > 255) ex = 0;
+  return sign | ((uint32_t)ex << 23) | mant;
+}
+
+/* One header key as the reader stores it (src/segy.cc:369-1490): kind 0 an
+ * int field (2-byte key sign-extended), 1 a short field, 2 an unsigned short
+ * field (4-byte key truncated to its low 16 bits); -> the field's value. */
+long long or_segy_key(const unsigned char *hdr240, int start, int bytes, int kind) {
+  const unsigned char *p = hdr240 + start - 1;
+  if (bytes == 4) {
+    const int32_t v = (int32_t)(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]);
+    if (kind == 0) return v;
+    if (kind == 1) return (int16_t)(v & 0xffff);
+    return (uint16_t)(v & 0xffff);
+  }
+  const uint16_t u = (uint16_t)((p[0] << 8) | p[1]);
+  if (kind == 2) return u;
+  return (int16_t)u;
+}
+
+/* ns big-endian samples of one record (after its 240-byte header): format
+ * 1 IBM, 5 IEEE; the result as FP32 bits.  Returns 0, or -1 for another
+ * format. */
+int or_segy_samples(const unsigned char *rec, int ns, int format, uint32_t *out) {
+  const unsigned char *s = rec + 240;
+  for (int i = 0; i < ns; ++i) {
+    const unsigned char *p = s + 4 * (size_t)i;
+    const uint32_t w = ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+    if (format == 1)
+      out[i] = or_ibm_to_ieee(w);
+    else if (format == 5)
+      out[i] = w;
+    else
+      return -1;
+  }
+  return 0;
+}

@@ -107,3 +107,71 @@ void ref_get_vel_coor(const char *path, int line, coor_t *c, short *sc, int fci,
   get_vel_coor(path, line, c, sc, fci, nxi, fcv, nxv, flv, nyv);
 }
 }
+
+// ---- SEG-Y rev0 (src/segy.cc:83-2752, 2753-4477) ----
+// keyidx is int[2*75] (start, bytes) pairs, the layout of keyindex[75].
+extern "C" {
+
+void ref_initkeyindex(int *keyidx) { initkeyindex((keyindex *)keyidx); }
+
+void ref_setkeyindex(int *keyidx, const char *key, int start, int bytes) {
+  setkeyindex((keyindex *)keyidx, key, start, bytes);
+}
+
+// fsgethd of a file's first 3600 bytes into out (3600 B, host order).
+int ref_fsgethd(const char *path, unsigned char *out) {
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return -1;
+  hd3600 *hd = (hd3600 *)calloc(1, sizeof(hd3600));
+  fsgethd(fp, hd);
+  memcpy(out, hd, 3600);
+  free(hd);
+  fclose(fp);
+  return 0;
+}
+
+// fsgettr over a file after skip bytes, reusing one trace record as the
+// reference's callers do: returns the number of traces read (<= maxtr),
+// header bytes into hdrs[240*i], samples into data[i*maxns].
+int ref_fsgettr_all(const char *path, long skip, int format, const int *keyidx, int maxtr, int maxns,
+                    unsigned char *hdrs, float *data) {
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return -1;
+  fseek(fp, skip, SEEK_SET);
+  segy *tr = (segy *)calloc(1, sizeof(segy));
+  int n = 0;
+  while (n < maxtr) {
+    int ns = fsgettr(fp, tr, (short)format, (const keyindex *)keyidx);
+    if (ns == 0) break;
+    memcpy(hdrs + 240 * (size_t)n, tr, 240);
+    memcpy(data + (size_t)n * maxns, tr->data, sizeof(float) * (ns < maxns ? ns : maxns));
+    ++n;
+  }
+  free(tr);
+  fclose(fp);
+  return n;
+}
+
+// Writes a SEG-Y file with the reference's writer: inithd3600 + fsputhd,
+// then one fsputtr per trace from host-order headers hdrs[240*i] and
+// samples data[i*ns].
+int ref_fsput_file(const char *path, int ns, int dt_um, int format, const int *keyidx, int ntr,
+                   const unsigned char *hdrs, const float *data) {
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return -1;
+  hd3600 *hd = (hd3600 *)calloc(1, sizeof(hd3600));
+  inithd3600(hd, ns, dt_um, (short)format);
+  fsputhd(fp, hd);
+  free(hd);
+  segy *tr = (segy *)calloc(1, sizeof(segy));
+  for (int i = 0; i < ntr; ++i) {
+    memcpy(tr, hdrs + 240 * (size_t)i, 240);
+    memcpy(tr->data, data + (size_t)i * ns, sizeof(float) * ns);
+    fsputtr(fp, tr, (short)format, (const keyindex *)keyidx);
+  }
+  free(tr);
+  fclose(fp);
+  return 0;
+}
+
+}  // extern "C"

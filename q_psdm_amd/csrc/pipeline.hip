@@ -16,11 +16,23 @@
 #include <vector>
 
 #include "common.h"
+#include "ibm_float.h"
 #include "segy.h"
 
 namespace {
 
 constexpr int kSlots = 2;
+
+// A trace-header key in a record: 0-based byte offset and width (2 or 4;
+// any other width reads as 0, a key the reader does not decode).
+struct KeyPos {
+  int off, bytes;
+};
+struct GeomKeys {
+  KeyPos sx, sy, gx, gy;
+};
+// sx, sy, gx, gy at their SU / standard SEG-Y positions
+constexpr GeomKeys kStdGeom{{72, 4}, {76, 4}, {80, 4}, {84, 4}};
 
 struct Slot {
   unsigned char *host = nullptr;  // pinned staging: SU records or decoded samples+geom
@@ -29,7 +41,9 @@ struct Slot {
   float *geom = nullptr;          // [batch][4]
   void *bank = nullptr;           // [batch][K][nt]
   int ntr = 0;                    // traces staged
-  bool su = false;                // staging holds SU records (else samples + geom)
+  bool su = false;                // staging holds SU / SEG-Y records (else samples + geom)
+  int fmt = 5;                    // records' sample format: 1 IBM float, 5 IEEE float
+  GeomKeys keys = kStdGeom;       // records' sx, sy, gx, gy positions
   hipEvent_t copied = nullptr, done = nullptr;
   bool used = false;
 };
@@ -38,23 +52,73 @@ struct GeomDev {
   double scaled, fxv1, exv1, fyv1, eyv1, ox, oy;
 };
 
-// One work-group per SU record: swap the samples, read sx, sy, gx, gy (and
-// scalco-independent `scaled`), reject traces outside the model extent by
-// writing NaN geometry (src/q-psdm.cu:2115-2123), else image-relative FP32
-// coordinates with the origin removed in FP64.
-__global__ void su_decode_kernel(const unsigned char *__restrict__ raw, int ns, GeomDev g,
-                                 float *__restrict__ samples, float4 *__restrict__ geom) {
+// Big-endian key as a 4-byte integer field: a 2-byte key sign-extends.
+__device__ inline int key_i32(const unsigned char *r, KeyPos k) {
+  const unsigned char *p = r + k.off;
+  if (k.bytes == 4) return (int)(((unsigned)p[0] << 24) | ((unsigned)p[1] << 16) | ((unsigned)p[2] << 8) | p[3]);
+  if (k.bytes == 2) return (int)(short)((p[0] << 8) | p[1]);
+  return 0;
+}
+// Big-endian key as a 2-byte field: a 4-byte key keeps its low 16 bits.
+__device__ inline unsigned short key_u16(const unsigned char *r, KeyPos k) {
+  const unsigned char *p = r + k.off;
+  if (k.bytes == 2) return (unsigned short)((p[0] << 8) | p[1]);
+  if (k.bytes == 4) return (unsigned short)((p[2] << 8) | p[3]);
+  return 0;
+}
+
+// Sample word of a record (big-endian) -> FP32 bits.
+template <int FMT>
+__device__ inline float sample_in(unsigned w) {
+  w = __builtin_bswap32(w);
+  if constexpr (FMT == 1) {
+    bool range = false;
+    w = qpsdm_ibm::to_ieee_bits(w, &range);
+  }
+  return __uint_as_float(w);
+}
+
+// The samples of one record by one work-group: each thread loads all of its
+// words before converting and storing any, so every load of the record is
+// in flight at once (one pass for ns <= 8 * blockDim).
+template <int FMT>
+__device__ inline void decode_samples(const unsigned *__restrict__ src, float *__restrict__ dst, int ns) {
+  constexpr int U = 8;
+  const int step = blockDim.x;
+  for (int base = threadIdx.x; base < ns; base += U * step) {
+    unsigned w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * step;
+      w[u] = i < ns ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * step;
+      if (i < ns) dst[i] = sample_in<FMT>(w[u]);
+    }
+  }
+}
+
+// One work-group per SU / SEG-Y record: convert the samples (FMT 5:
+// big-endian IEEE, the SU case; FMT 1: big-endian IBM), read sx, sy, gx, gy
+// at their key positions (and scalco-independent `scaled`), reject traces
+// outside the model extent by writing NaN geometry
+// (src/q-psdm.cu:2115-2123), else image-relative FP32 coordinates with the
+// origin removed in FP64.
+template <int FMT>
+__global__ void record_decode_kernel(const unsigned char *__restrict__ raw, int ns, GeomKeys keys, GeomDev g,
+                                     float *__restrict__ samples, float4 *__restrict__ geom) {
   const size_t rec = 240 + 4 * (size_t)ns;
   const unsigned char *r = raw + (size_t)blockIdx.x * rec;
   const unsigned *src = (const unsigned *)(r + 240);
   float *dst = samples + (size_t)blockIdx.x * ns;
-  for (int i = threadIdx.x; i < ns; i += blockDim.x) dst[i] = __uint_as_float(__builtin_bswap32(src[i]));
+  decode_samples<FMT>(src, dst, ns);
   if (threadIdx.x == 0) {
-    const unsigned *h = (const unsigned *)r;
-    const double sx = (int)__builtin_bswap32(h[18]) * g.scaled;
-    const double sy = (int)__builtin_bswap32(h[19]) * g.scaled;
-    const double gx = (int)__builtin_bswap32(h[20]) * g.scaled;
-    const double gy = (int)__builtin_bswap32(h[21]) * g.scaled;
+    const double sx = key_i32(r, keys.sx) * g.scaled;
+    const double sy = key_i32(r, keys.sy) * g.scaled;
+    const double gx = key_i32(r, keys.gx) * g.scaled;
+    const double gy = key_i32(r, keys.gy) * g.scaled;
     const bool keep = !(sx < g.fxv1 || sx > g.exv1 || gx < g.fxv1 || gx > g.exv1 || sy < g.fyv1 ||
                         sy > g.eyv1 || gy < g.fyv1 || gy > g.eyv1);
     geom[blockIdx.x] = keep ? make_float4((float)(sx - g.ox), (float)(sy - g.oy), (float)(gx - g.ox),
@@ -63,7 +127,66 @@ __global__ void su_decode_kernel(const unsigned char *__restrict__ raw, int ns, 
   }
 }
 
+struct HdrKeys {
+  KeyPos sx, sy, gx, gy, fldr, cdp, offset, tracl, scalco, ns, dt;
+};
+
+// One work-group per SEG-Y record: samples as record_decode_kernel, and the
+// header subset of qpsdm_trhdr read at its key positions.
+template <int FMT>
+__global__ void segy_decode_kernel(const unsigned char *__restrict__ raw, int ns, HdrKeys k,
+                                   float *__restrict__ samples, qpsdm_trhdr *__restrict__ hdrs) {
+  const size_t rec = 240 + 4 * (size_t)ns;
+  const unsigned char *r = raw + (size_t)blockIdx.x * rec;
+  const unsigned *src = (const unsigned *)(r + 240);
+  float *dst = samples + (size_t)blockIdx.x * ns;
+  decode_samples<FMT>(src, dst, ns);
+  if (threadIdx.x == 0 && hdrs) {
+    qpsdm_trhdr h;
+    h.sx = key_i32(r, k.sx);
+    h.sy = key_i32(r, k.sy);
+    h.gx = key_i32(r, k.gx);
+    h.gy = key_i32(r, k.gy);
+    h.fldr = key_i32(r, k.fldr);
+    h.cdp = key_i32(r, k.cdp);
+    h.offset = key_i32(r, k.offset);
+    h.tracl = key_i32(r, k.tracl);
+    h.scalco = (short)key_u16(r, k.scalco);
+    h.pad = 0;
+    h.ns = key_u16(r, k.ns);
+    h.dt = key_u16(r, k.dt);
+    hdrs[blockIdx.x] = h;
+  }
+}
+
 }  // namespace
+
+extern "C" int qpsdm_segy_decode_dev(const void *raw_dev, int ntr, int ns, int format, const keyindex *keyidx,
+                                     float *samples_dev, qpsdm_trhdr *hdrs_dev, void *stream) {
+  QPSDM_REQUIRE(ntr >= 0 && ns > 0 && ns <= SU_NFLTS, "segy_decode_dev: bad sizes ntr=%d ns=%d", ntr, ns);
+  QPSDM_REQUIRE(format == 1 || format == 5, "segy_decode_dev: format %d (1 = IBM float, 5 = IEEE float)", format);
+  QPSDM_REQUIRE(ntr == 0 || (raw_dev && samples_dev), "segy_decode_dev: null buffer");
+  QPSDM_REQUIRE(((uintptr_t)raw_dev & 3) == 0, "segy_decode_dev: records must start on a 4-byte boundary");
+  if (ntr == 0) return QPSDM_OK;
+  // key numbers in struct segy order, standard positions when keyidx is NULL
+  auto pos = [&](int k, int std_off, int std_bytes) {
+    if (!keyidx) return KeyPos{std_off, std_bytes};
+    const int off = keyidx[k].start - 1, w = keyidx[k].bytes;
+    if ((w != 2 && w != 4) || off < 0 || off + w > 240) return KeyPos{0, 0};  // not read: 0
+    return KeyPos{off, w};
+  };
+  HdrKeys k{pos(21, 72, 4), pos(22, 76, 4), pos(23, 80, 4), pos(24, 84, 4), pos(2, 8, 4), pos(5, 20, 4),
+            pos(11, 36, 4), pos(0, 0, 4),   pos(20, 70, 2), pos(38, 114, 2), pos(39, 116, 2)};
+  const hipStream_t s = (hipStream_t)stream;
+  if (format == 1)
+    hipLaunchKernelGGL(segy_decode_kernel<1>, dim3(ntr), dim3(256), 0, s, (const unsigned char *)raw_dev, ns, k,
+                       samples_dev, hdrs_dev);
+  else
+    hipLaunchKernelGGL(segy_decode_kernel<5>, dim3(ntr), dim3(256), 0, s, (const unsigned char *)raw_dev, ns, k,
+                       samples_dev, hdrs_dev);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
 
 struct qpsdm_ctx {
   qpsdm_ctx_params p;
@@ -99,8 +222,12 @@ int submit(qpsdm_ctx *c) {
   if (s.su) {
     GeomDev g{c->p.geom.scaled, c->p.geom.fxv1, c->p.geom.exv1, c->p.geom.fyv1, c->p.geom.eyv1,
               c->p.geom.ox, c->p.geom.oy};
-    hipLaunchKernelGGL(su_decode_kernel, dim3(s.ntr), dim3(256), 0, c->compute, s.raw, nt, g,
-                       s.samples, (float4 *)s.geom);
+    if (s.fmt == 1)
+      hipLaunchKernelGGL(record_decode_kernel<1>, dim3(s.ntr), dim3(256), 0, c->compute, s.raw, nt, s.keys, g,
+                         s.samples, (float4 *)s.geom);
+    else
+      hipLaunchKernelGGL(record_decode_kernel<5>, dim3(s.ntr), dim3(256), 0, c->compute, s.raw, nt, s.keys, g,
+                         s.samples, (float4 *)s.geom);
     QPSDM_HIP(hipGetLastError());
   }
   int rc = qpsdm_qbank(&c->p.bank, s.samples, s.ntr, s.bank, c->p.bank_dtype, c->compute);
@@ -119,13 +246,58 @@ int submit(qpsdm_ctx *c) {
   return QPSDM_OK;
 }
 
-int ensure_mode(qpsdm_ctx *c, bool su) {
+inline bool same_keys(const GeomKeys &a, const GeomKeys &b) {
+  return a.sx.off == b.sx.off && a.sx.bytes == b.sx.bytes && a.sy.off == b.sy.off && a.sy.bytes == b.sy.bytes &&
+         a.gx.off == b.gx.off && a.gx.bytes == b.gx.bytes && a.gy.off == b.gy.off && a.gy.bytes == b.gy.bytes;
+}
+
+// The current slot takes what comes next: records of one format and key
+// layout, or decoded traces; a slot holding something else is submitted.
+int ensure_mode(qpsdm_ctx *c, bool su, int fmt = 5, const GeomKeys &keys = kStdGeom) {
   Slot &s = c->slot[c->cur];
-  if (s.ntr > 0 && s.su != su) {
+  if (s.ntr > 0 && (s.su != su || (su && (s.fmt != fmt || !same_keys(s.keys, keys))))) {
     int rc = submit(c);
     if (rc) return rc;
   }
-  c->slot[c->cur].su = su;
+  Slot &n = c->slot[c->cur];
+  n.su = su;
+  n.fmt = fmt;
+  n.keys = keys;
+  return QPSDM_OK;
+}
+
+// Records (240-byte header + nt big-endian samples each) into the staging
+// slots; every record's ns (read at ns_key) must equal nt.
+int push_records(qpsdm_ctx *c, const unsigned char *b, size_t nbytes, int fmt, const GeomKeys &keys,
+                 KeyPos ns_key, const char *who) {
+  QPSDM_REQUIRE(c->have_tab, "%s: tables not loaded", who);
+  QPSDM_REQUIRE(nbytes % c->rec_bytes == 0, "%s: %zu bytes is not a whole number of %zu-byte records", who, nbytes,
+                c->rec_bytes);
+  const size_t n = nbytes / c->rec_bytes;
+  for (size_t i = 0; i < n; ++i) {
+    const unsigned char *p = b + i * c->rec_bytes + ns_key.off;
+    const int ns = ns_key.bytes == 2 ? (p[0] << 8) | p[1] : (p[2] << 8) | p[3];
+    QPSDM_REQUIRE(ns == c->p.image.nt, "%s: record %zu has ns=%d, expected %d", who, i, ns, c->p.image.nt);
+  }
+  int rc = ensure_mode(c, true, fmt, keys);
+  if (rc) return rc;
+  size_t done = 0;
+  while (done < n) {
+    Slot &s = c->slot[c->cur];
+    const size_t take = std::min(n - done, (size_t)(c->p.batch_traces - s.ntr));
+    memcpy(s.host + (size_t)s.ntr * c->rec_bytes, b + done * c->rec_bytes, take * c->rec_bytes);
+    s.ntr += (int)take;
+    done += take;
+    c->pushed += (long long)take;
+    if (s.ntr == c->p.batch_traces) {
+      rc = submit(c);
+      if (rc) return rc;
+      Slot &t = c->slot[c->cur];
+      t.su = true;
+      t.fmt = fmt;
+      t.keys = keys;
+    }
+  }
   return QPSDM_OK;
 }
 
@@ -239,33 +411,37 @@ extern "C" int qpsdm_ctx_build_tables(qpsdm_ctx *c, const qpsdm_model *m) {
 // producer reads them, src/q-psdm.cu:1887-1916).
 extern "C" int qpsdm_ctx_push_su(qpsdm_ctx *c, const void *bytes, size_t nbytes) {
   QPSDM_REQUIRE(c && (bytes || nbytes == 0), "ctx_push_su: null args");
-  QPSDM_REQUIRE(c->have_tab, "ctx_push_su: tables not loaded");
-  QPSDM_REQUIRE(nbytes % c->rec_bytes == 0, "ctx_push_su: %zu bytes is not a whole number of %zu-byte records",
-                nbytes, c->rec_bytes);
-  const unsigned char *b = (const unsigned char *)bytes;
-  const size_t n = nbytes / c->rec_bytes;
-  for (size_t i = 0; i < n; ++i) {
-    const unsigned char *r = b + i * c->rec_bytes;
-    const int ns = (r[114] << 8) | r[115];
-    QPSDM_REQUIRE(ns == c->p.image.nt, "ctx_push_su: record %zu has ns=%d, expected %d", i, ns, c->p.image.nt);
+  return push_records(c, (const unsigned char *)bytes, nbytes, 5, kStdGeom, KeyPos{114, 2}, "ctx_push_su");
+}
+
+// SEG-Y rev0 trace records (no reel header) of ns = nt samples in format 1
+// (IBM float) or 5 (IEEE float), sx/sy/gx/gy and ns at their keyindex
+// positions (NULL: standard), decoded on the device as fsgettr decodes them
+// (src/segy.cc:350-1547).
+extern "C" int qpsdm_ctx_push_segy(qpsdm_ctx *c, const void *bytes, size_t nbytes, int format,
+                                   const keyindex *keyidx) {
+  QPSDM_REQUIRE(c && (bytes || nbytes == 0), "ctx_push_segy: null args");
+  QPSDM_REQUIRE(format == 1 || format == 5, "ctx_push_segy: format %d (1 = IBM float, 5 = IEEE float)", format);
+  GeomKeys keys = kStdGeom;
+  KeyPos ns_key{114, 2};
+  if (keyidx) {
+    auto pos = [&](int k, KeyPos *out) {
+      const int off = keyidx[k].start - 1, w = keyidx[k].bytes;
+      if ((w != 2 && w != 4) || off < 0 || off + w > 240)
+        return qpsdm::set_error(QPSDM_EINVAL, "ctx_push_segy: key %d at byte %d width %d does not fit the header", k,
+                                keyidx[k].start, w);
+      *out = KeyPos{off, w};
+      return QPSDM_OK;
+    };
+    // key numbers in struct segy order: sx 21, sy 22, gx 23, gy 24, ns 38
+    int rc = pos(21, &keys.sx);
+    if (!rc) rc = pos(22, &keys.sy);
+    if (!rc) rc = pos(23, &keys.gx);
+    if (!rc) rc = pos(24, &keys.gy);
+    if (!rc) rc = pos(38, &ns_key);
+    if (rc) return rc;
   }
-  int rc = ensure_mode(c, true);
-  if (rc) return rc;
-  size_t done = 0;
-  while (done < n) {
-    Slot &s = c->slot[c->cur];
-    const size_t take = std::min(n - done, (size_t)(c->p.batch_traces - s.ntr));
-    memcpy(s.host + (size_t)s.ntr * c->rec_bytes, b + done * c->rec_bytes, take * c->rec_bytes);
-    s.ntr += (int)take;
-    done += take;
-    c->pushed += (long long)take;
-    if (s.ntr == c->p.batch_traces) {
-      rc = submit(c);
-      if (rc) return rc;
-      c->slot[c->cur].su = true;
-    }
-  }
-  return QPSDM_OK;
+  return push_records(c, (const unsigned char *)bytes, nbytes, format, keys, ns_key, "ctx_push_segy");
 }
 
 // Decoded traces from host memory: samples [ntr][nt], geometry [ntr][4]

@@ -401,6 +401,11 @@ class Pipeline:
     def push_su(self, data):
         check(L.lib().qpsdm_ctx_push_su(self.ctx, data, len(data)))
 
+    def push_segy(self, data, fmt=1, keyidx=None):
+        """SEG-Y rev0 trace records (no reel header), format 1 (IBM) or 5
+        (IEEE), keys at keyidx (segy.keyindex(); None: standard)."""
+        check(L.lib().qpsdm_ctx_push_segy(self.ctx, data, len(data), fmt, keyidx))
+
     def push_traces(self, samples, geom):
         samples = np.ascontiguousarray(samples, np.float32)
         geom = np.ascontiguousarray(geom, np.float32)

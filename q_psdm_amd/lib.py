@@ -116,7 +116,19 @@ EXPORTS = [
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
     "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables", "qpsdm_copy_f32",
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
+    "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
+    "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
 ]
+
+SEGY_TOTAL_KEY_NUM = 75
+
+
+class KeyIndex(C.Structure):
+    """struct keyindex of segy.h: 1-based start byte and width of a key."""
+    _fields_ = [("start", C.c_int), ("bytes", C.c_int)]
+
+
+KeyIndexArray = KeyIndex * SEGY_TOTAL_KEY_NUM
 
 _lib = None
 
@@ -163,6 +175,14 @@ def lib():
         L.qpsdm_su_decode.restype = C.c_long
         L.qpsdm_su_decode.argtypes = [vp, C.c_size_t, C.c_int, vp, vp]
         L.qpsdm_su_sgettr.argtypes = [vp, vp]
+        L.qpsdm_segy_fgethd.argtypes = [vp, vp]
+        L.qpsdm_segy_initkeyindex.argtypes = [vp]
+        L.qpsdm_segy_setkeyindex.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+        L.qpsdm_segy_sgettr.argtypes = [vp, vp, C.c_int, vp]
+        L.qpsdm_segy_decode.restype = C.c_long
+        L.qpsdm_segy_decode.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp, vp]
+        L.qpsdm_segy_decode_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.qpsdm_ctx_push_segy.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
         L.qpsdm_su_sputtr.argtypes = [vp, vp]
         L.getscale.restype = C.c_float
         L.getscale.argtypes = [C.c_short]
