@@ -180,9 +180,10 @@ int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces_dev, int
                         float *re_dev, float *im_dev, void *stream);
 
 /* Q filter bank: traces [ntr][nt] FP32 -> bank of dtype QPSDM_F32 or
- * QPSDM_BF16 ([ntr][K][nt]) or QPSDM_F16 (per trace: header + [K][nt], the
- * scale 2^(14 - e) for the trace's largest |sample| in [2^e, 2^(e+1)), so
- * every sample is below 2^15 in FP16). */
+ * QPSDM_BF16 ([ntr][K][nt]) or QPSDM_F16 (per trace: a 16-byte header
+ * {1/scale, scale, 0, 0} then [K][nt] samples times scale = 2^(15 - ex), ex
+ * the binary exponent of a bound on the trace's largest |sample| over all
+ * levels taken from its spectrum, so every stored sample is below 2^15). */
 /* Bytes between consecutive traces of a bank (K * nt * sample size for
  * QPSDM_F32 and QPSDM_BF16; 16 + K * nt * 2 rounded up to 16 for QPSDM_F16);
  * 0 for another dtype. */
@@ -231,6 +232,29 @@ int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab_dev, int ta
 int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                      const float *geom_dev, int tr0, int ntr, const void *bank_dev,
                      int bank_dtype, const void *tab_dev, int table_dtype, void *stream);
+
+/* Dip-angle gathers at selected image columns.  The reference declares
+ * them -- options opcrp, cdps, aspect, dipx, dipy, ddipx, ddipy
+ * (src/q-psdm.cu:651-702), ndipx = (int)(dipx / ddipx + 0.5)
+ * (src/q-psdm.cu:828-829), buffers gthrxd/gthryd (src/q-psdm.cu:211-212) --
+ * but does not fill them.  Every update the imaging sum makes at column
+ * cols[c] = (ix, iy) (int pairs, device memory) is also added, in trace
+ * order, to bin k of gx [ncol][nbx][nz] and gy [ncol][nby][nz]: k - ndip is
+ * the nearest multiple of ddip to the bisector's dip angle atan(X / Z) in x
+ * (atan(Y / Z) in y), k in [0, 2 ndip]; updates beyond the outer bins are not
+ * binned.  aspect: 1 both axes (the reference's "All"), 2 x only, 3 y only,
+ * 4 none; an axis not asked for may be NULL.  Gathers accumulate (+=) as the
+ * image does.  Tables QPSDM_F32 or QPSDM_U16X; any bank dtype. */
+typedef struct {
+  float dipx, dipy;   /* largest dip binned (degrees)  */
+  float ddipx, ddipy; /* bin width (degrees)            */
+  int aspect;         /* 1 all, 2 x, 3 y, 4 none        */
+} qpsdm_gather_params;
+/* Bins per axis for g: 2 ndip + 1, or 0 for an axis aspect leaves out. */
+int qpsdm_gather_bins(const qpsdm_gather_params *g, int *nbx, int *nby);
+int qpsdm_gathers_shot(const qpsdm_image_params *p, const qpsdm_gather_params *g, const int *cols_dev, int ncol,
+                       float *gx_dev, float *gy_dev, const float *geom_dev, int tr0, int ntr, const void *bank_dev,
+                       int bank_dtype, const void *tab_dev, int table_dtype, void *stream);
 
 /* Update count of a run (pairs (trace, point) inside the dip aperture, z > 0),
  * added to *count_dev (uint64). */

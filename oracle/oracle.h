@@ -106,6 +106,10 @@ int64_t or_count(const float *geom, int ntr, const or_iparams *p);
 int or_num_threads(void);
 void or_set_num_threads(int n);
 uint32_t or_ibm_to_ieee(uint32_t w);
+void or_gather_edges(int ndip, double ddip, float *e);
+void or_gathers_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_iparams *p,
+                    const int *cols, int ncol, const float *ex, int nbx, const float *ey, int nby, float *gx,
+                    float *gy);
 long long or_segy_key(const unsigned char *hdr240, int start, int bytes, int kind);
 int or_segy_samples(const unsigned char *rec, int ns, int format, uint32_t *out);
 

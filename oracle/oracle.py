@@ -88,6 +88,9 @@ def lib():
         L.or_num_threads.restype = C.c_int
         L.or_set_num_threads.restype = None
         L.or_set_num_threads.argtypes = [C.c_int]
+        L.or_gather_edges.argtypes = [C.c_int, C.c_double, fp]
+        L.or_gathers_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(IParams), C.c_void_p, C.c_int, fp, C.c_int,
+                                     fp, C.c_int, fp, fp]
         L.or_ibm_to_ieee.restype = C.c_uint32
         L.or_ibm_to_ieee.argtypes = [C.c_uint32]
         L.or_segy_key.restype = C.c_longlong
@@ -284,3 +287,29 @@ def segy_samples(record, ns, fmt):
 
 def segy_key(header, start, nbytes, kind):
     return lib().or_segy_key(bytes(header[:240]), start, nbytes, kind)
+
+
+def gather_edges(dip, ddip):
+    """Boundaries of the 2 ndip + 1 bins, ndip = (int)(dip / ddip + 0.5)."""
+    ndip = int(float(np.float32(dip) / np.float32(ddip)) + 0.5)
+    e = np.zeros(2 * ndip + 2, np.float32)
+    lib().or_gather_edges(ndip, float(np.float32(ddip)), _f(e))
+    return e
+
+
+def gathers(bank, geom, tab, ip, cols, ex=None, ey=None):
+    """FP32 dip-angle gathers at cols [ncol][2] (ix, iy): (gx, gy)."""
+    bank = np.ascontiguousarray(bank, np.float32)
+    geom = np.ascontiguousarray(geom, np.float32)
+    tab = np.ascontiguousarray(tab, np.float32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    ncol = cols.shape[0]
+    nbx = 0 if ex is None else ex.size - 1
+    nby = 0 if ey is None else ey.size - 1
+    gx = np.zeros((ncol, max(nbx, 1), ip.nz), np.float32)
+    gy = np.zeros((ncol, max(nby, 1), ip.nz), np.float32)
+    dummy = np.zeros(1, np.float32)
+    lib().or_gathers_f32(_f(bank), _f(geom), geom.shape[0], _f(tab), C.byref(ip), cols.ctypes.data_as(C.c_void_p),
+                         ncol, _f(ex if ex is not None else dummy), nbx, _f(ey if ey is not None else dummy), nby,
+                         _f(gx), _f(gy))
+    return (gx if nbx else None), (gy if nby else None)

@@ -741,3 +741,94 @@ int or_segy_samples(const unsigned char *rec, int ns, int format, uint32_t *out)
   }
   return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Dip-angle gathers (options at src/q-psdm.cu:651-702, bin count          */
+/* ndip = (int)(dip / ddip + 0.5) at src/q-psdm.cu:828-829; the reference  */
+/* leaves gthrxd/gthryd, src/q-psdm.cu:211-212, unfilled): the FP32 path's */
+/* update of each (trace, point) at the selected columns, binned by the    */
+/* bisector's dip tangent against bin boundaries e[0..n] (bin b covers     */
+/* [e[b], e[b+1])) and accumulated in trace order.  gx [ncol][nbx][nz],    */
+/* gy [ncol][nby][nz]; nbx or nby 0 skips that axis.                        */
+/* ---------------------------------------------------------------------- */
+static int bin_linear(float v, const float *e, int n) {
+  if (!(v >= e[0] && v < e[n])) return -1;
+  int b = 0;
+  while (b + 1 < n && v >= e[b + 1]) ++b;
+  return b;
+}
+
+void or_gather_edges(int ndip, double ddip, float *e) {
+  const double rad = M_PI / 180.0;
+  for (int b = 0; b <= 2 * ndip + 1; ++b) e[b] = (float)tan(((double)(b - ndip) - 0.5) * ddip * rad);
+}
+
+void or_gathers_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_iparams *p,
+                    const int *cols, int ncol, const float *ex, int nbx, const float *ey, int nby, float *gx,
+                    float *gy) {
+  const or_tgeom *g = &p->tg;
+  const float tz = p->taperzone;
+  const float tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;
+  const float tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);
+  const float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
+  const float rx1 = 1.0f / (tx1 - tx1c), rx2 = 1.0f / (tx2c - tx2);
+  const float ry1 = 1.0f / (ty1 - ty1c), ry2 = 1.0f / (ty2c - ty2);
+  const float cx1 = -tx1c * rx1, cx2 = tx2c * rx2, cy1 = -ty1c * ry1, cy2 = ty2c * ry2;
+  const float inv_dt = 1.0f / p->dt, inv_dts = 1.0f / p->dtstar;
+#pragma omp parallel for schedule(dynamic)
+  for (int c = 0; c < ncol; ++c) {
+    const float x = (float)cols[2 * c] * p->dx, y = (float)cols[2 * c + 1] * p->dy;
+    for (int itr = 0; itr < ntr; ++itr) {
+      const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
+      const float gxx = geom[4 * itr + 2], gyy = geom[4 * itr + 3];
+      if (sx != sx) continue;
+      const float *bt = bank + (size_t)itr * p->K * p->nt;
+      const float dsx = sx - x, dsy = sy - y, dgx = gxx - x, dgy = gyy - y;
+      const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
+      for (int iz = 1; iz < p->nz; ++iz) {
+        const float z = (float)iz * p->dz;
+        float ts, tss;
+        lookup_f32(g, tab, sx, sy, x, y, z, &ts, &tss);
+        ts = fmaxf(ts, p->tmin);
+        const float d2s = hs2 + z * z;
+        const float rds = 1.0f / sqrtf(d2s), ds = d2s * rds;
+        const float tsn = ts * inv_dt, tsl = fmaf(tss, inv_dts, 0.5f);
+        const float A = ds * ds * ds * (1.0f / ts);
+        const float rdg = 1.0f / sqrtf(hg2 + z * z);
+        const float q = rdg * ds;
+        const float X = fmaf(dgx, q, dsx), Y = fmaf(dgy, q, dsy), Z = fmaf(z, q, z);
+        const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
+        const float wx = fminf(sat01(fmaf(tanx, rx1, cx1)), sat01(fmaf(tanx, -rx2, cx2)));
+        const float wy = fminf(sat01(fmaf(tany, ry1, cy1)), sat01(fmaf(tany, -ry2, cy2)));
+        const float wt = wx * wy;
+        if (!(wt > 0.0f)) continue;
+        float tg, tsg;
+        lookup_f32(g, tab, gxx, gyy, x, y, z, &tg, &tsg);
+        tg = fmaxf(tg, p->tmin);
+        const float f = fmaf(tg, inv_dt, tsn);
+        const int i0 = (int)f;
+        if (!(i0 > 0 && i0 + 1 < p->nt)) continue;
+        int j = (int)fmaf(tsg, inv_dts, tsl);
+        j = j < 0 ? 0 : (j > p->K - 1 ? p->K - 1 : j);
+        const float fr = f - (float)i0;
+        const float *b = bt + (size_t)j * p->nt + i0;
+        const float tr = A * (rdg * rdg * rdg) * tg;
+        const float W = wt * (tr + 1.0f / tr), sv = fmaf(fr, b[1] - b[0], b[0]);
+        if (nbx) {
+          const int k = bin_linear(tanx, ex, nbx);
+          if (k >= 0) {
+            float *o = &gx[((size_t)c * nbx + k) * p->nz + iz];
+            *o = fmaf(W, sv, *o);
+          }
+        }
+        if (nby) {
+          const int k = bin_linear(tany, ey, nby);
+          if (k >= 0) {
+            float *o = &gy[((size_t)c * nby + k) * p->nz + iz];
+            *o = fmaf(W, sv, *o);
+          }
+        }
+      }
+    }
+  }
+}

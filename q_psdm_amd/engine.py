@@ -189,6 +189,36 @@ class Survey:
                                        _ptr(geom), tr0, ntr, _ptr(bank), bcode, _ptr(tab), tcode,
                                        _stream(stream)))
 
+    _ASPECTS = {"all": 1, "x": 2, "y": 3, "none": 4}
+
+    def gather_params(self, dipx=50.0, dipy=50.0, ddipx=1.0, ddipy=1.0, aspect="all"):
+        """The reference's gather options (dipx, dipy, ddipx, ddipy in degrees,
+        aspect All / X / Y / None) and the bins per axis they give."""
+        g = L.GatherParams(dipx, dipy, ddipx, ddipy, self._ASPECTS[aspect.lower()])
+        nbx, nby = C.c_int(), C.c_int()
+        check(L.lib().qpsdm_gather_bins(C.byref(g), C.byref(nbx), C.byref(nby)))
+        return g, nbx.value, nby.value
+
+    def new_gathers(self, ncol, nb):
+        return torch.zeros((ncol, nb, self.cfg.nz), dtype=torch.float32, device="cuda") if nb else None
+
+    def gathers_shot(self, gx, gy, cols, geom, tr0, ntr, bank, tab, gp, stream=None):
+        """Accumulate the dip-angle gathers of the run at image columns cols
+        (int32 device tensor [ncol][2] of (ix, iy)) into gx [ncol][nbx][nz]
+        and gy [ncol][nby][nz] (either None when its axis is not asked for)."""
+        assert cols.dtype == torch.int32 and cols.dim() == 2 and cols.shape[1] == 2
+        assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
+        self._check_bank(bank, ntr)
+        g, nbx, nby = gp
+        ncol = cols.shape[0]
+        for t, nb in ((gx, nbx), (gy, nby)):
+            if nb:
+                assert t is not None and tuple(t.shape) == (ncol, nb, self.cfg.nz) and t.dtype == torch.float32
+        bcode, tcode = self._codes(bank, tab)
+        check(L.lib().qpsdm_gathers_shot(C.byref(self.ip), C.byref(g), _ptr(cols), ncol,
+                                         _ptr(gx) if nbx else None, _ptr(gy) if nby else None, _ptr(geom), tr0, ntr,
+                                         _ptr(bank), bcode, _ptr(tab), tcode, _stream(stream)))
+
     def count_shot(self, geom, tr0, ntr, counter=None, stream=None):
         assert geom.dim() == 2 and geom.shape[1] == 4 and 0 <= tr0 and tr0 + ntr <= geom.shape[0]
         own = counter is None

@@ -70,6 +70,11 @@ class Launch(C.Structure):
                 ("trace_chunk", C.c_int)]
 
 
+class GatherParams(C.Structure):
+    _fields_ = [("dipx", C.c_float), ("dipy", C.c_float), ("ddipx", C.c_float), ("ddipy", C.c_float),
+                ("aspect", C.c_int)]
+
+
 class CtxParams(C.Structure):
     _fields_ = [("image", ImageParams), ("bank", BankParams), ("geom", GeomParams),
                 ("bank_dtype", C.c_int), ("table_dtype", C.c_int), ("batch_traces", C.c_int),
@@ -118,6 +123,7 @@ EXPORTS = [
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
     "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
     "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
+    "qpsdm_gather_bins", "qpsdm_gathers_shot",
 ]
 
 SEGY_TOTAL_KEY_NUM = 75
@@ -183,6 +189,9 @@ def lib():
         L.qpsdm_segy_decode.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp, vp]
         L.qpsdm_segy_decode_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
         L.qpsdm_ctx_push_segy.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+        L.qpsdm_gather_bins.argtypes = [C.POINTER(GatherParams), ip, ip]
+        L.qpsdm_gathers_shot.argtypes = [C.POINTER(ImageParams), C.POINTER(GatherParams), vp, C.c_int, vp, vp, vp,
+                                         C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]
         L.qpsdm_su_sputtr.argtypes = [vp, vp]
         L.getscale.restype = C.c_float
         L.getscale.argtypes = [C.c_short]
