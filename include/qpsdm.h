@@ -389,7 +389,7 @@ typedef struct {
   qpsdm_image_params image;  /* grid, traces, dip taper, table geometry       */
   qpsdm_bank_params bank;    /* windows and Q bank (nt, K, dtstar as image)   */
   qpsdm_geom_params geom;    /* SU coordinate scale, model extent, origin     */
-  int bank_dtype;            /* QPSDM_F32 or QPSDM_BF16                       */
+  int bank_dtype;            /* QPSDM_F32, QPSDM_BF16 or QPSDM_F16            */
   int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B or QPSDM_U16X */
   int batch_traces;          /* traces per staging slot                       */
   qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */
@@ -419,7 +419,18 @@ int qpsdm_ctx_flush(qpsdm_ctx *c);
 /* Image [ny][nx][nz] to host memory; divided by nsb when normalize != 0
  * (src/q-psdm.cu:1760-1766). */
 int qpsdm_ctx_image(qpsdm_ctx *c, float *host_img, int normalize);
+/* Zeroes the image and the gathers. */
 int qpsdm_ctx_reset_image(qpsdm_ctx *c);
+/* Dip-angle gathers (qpsdm_gathers_shot) at the columns cols (host int
+ * pairs (ix, iy) inside the image), accumulated beside the image by every
+ * later batch: the reference's opcrp/cdps/aspect/dipx/dipy/ddipx/ddipy
+ * options (src/q-psdm.cu:651-702, 820-835) and gthrxd/gthryd buffers
+ * (src/q-psdm.cu:211-212).  Replaces an earlier set; ncol = 0 stops
+ * gathering.  Needs QPSDM_F32 or QPSDM_U16X tables. */
+int qpsdm_ctx_set_gathers(qpsdm_ctx *c, const qpsdm_gather_params *g, const int *cols, int ncol);
+/* The gathers to host memory, gx [ncol][nbx][nz] and gy [ncol][nby][nz] (an
+ * axis not gathered may be NULL), divided by nsb when normalize != 0. */
+int qpsdm_ctx_gathers(qpsdm_ctx *c, float *gx, float *gy, int normalize);
 long long qpsdm_ctx_traces_pushed(const qpsdm_ctx *c);
 
 /* Image writer of qpsdm() (src/q-psdm.cu:1769-1800): for each CDP in cdps

@@ -199,6 +199,11 @@ struct qpsdm_ctx {
   int cur = 0;
   size_t rec_bytes = 0, slot_bytes = 0;
   long long pushed = 0, rejected_host = 0;
+  // dip-angle gathers accumulated beside the image (ngcol = 0: none)
+  qpsdm_gather_params gp{};
+  int ngcol = 0, nbx = 0, nby = 0;
+  int *gcols = nullptr;
+  float *gx = nullptr, *gy = nullptr;
 };
 
 namespace {
@@ -235,6 +240,11 @@ int submit(qpsdm_ctx *c) {
   rc = qpsdm_image_shot(&c->p.image, &c->p.shape, c->img, s.geom, 0, s.ntr, s.bank, c->p.bank_dtype,
                         c->tab, c->p.table_dtype, c->compute);
   if (rc) return rc;
+  if (c->ngcol) {
+    rc = qpsdm_gathers_shot(&c->p.image, &c->gp, c->gcols, c->ngcol, c->gx, c->gy, s.geom, 0, s.ntr, s.bank,
+                            c->p.bank_dtype, c->tab, c->p.table_dtype, c->compute);
+    if (rc) return rc;
+  }
   QPSDM_HIP(hipEventRecord(s.done, c->compute));
   s.used = true;
   s.ntr = 0;
@@ -358,6 +368,9 @@ extern "C" int qpsdm_ctx_destroy(qpsdm_ctx *c) {
     if (s.done) (void)hipEventDestroy(s.done);
   }
   if (c->img) (void)hipFree(c->img);
+  if (c->gcols) (void)hipFree(c->gcols);
+  if (c->gx) (void)hipFree(c->gx);
+  if (c->gy) (void)hipFree(c->gy);
   if (c->tab) (void)hipFree(c->tab);
   if (c->copy) (void)hipStreamDestroy(c->copy);
   if (c->compute) (void)hipStreamDestroy(c->compute);
@@ -518,7 +531,81 @@ extern "C" int qpsdm_ctx_reset_image(qpsdm_ctx *c) {
   int rc = qpsdm_ctx_flush(c);
   if (rc) return rc;
   QPSDM_HIP(hipMemsetAsync(c->img, 0, (size_t)c->p.image.nx * c->p.image.ny * c->p.image.nz * 4, c->compute));
+  const size_t nz = (size_t)c->p.image.nz;
+  if (c->gx) QPSDM_HIP(hipMemsetAsync(c->gx, 0, (size_t)c->ngcol * c->nbx * nz * 4, c->compute));
+  if (c->gy) QPSDM_HIP(hipMemsetAsync(c->gy, 0, (size_t)c->ngcol * c->nby * nz * 4, c->compute));
   QPSDM_HIP(hipStreamSynchronize(c->compute));
+  return QPSDM_OK;
+}
+
+// Dip-angle gathers at the columns cols (host (ix, iy) pairs), accumulated
+// by every later batch beside the image (the reference's opcrp/cdps/aspect/
+// dip options, src/q-psdm.cu:651-702, 820-835, and its gthrxd/gthryd
+// buffers, src/q-psdm.cu:211-212).  Replaces an earlier set; ncol = 0 stops
+// gathering.
+extern "C" int qpsdm_ctx_set_gathers(qpsdm_ctx *c, const qpsdm_gather_params *g, const int *cols, int ncol) {
+  QPSDM_REQUIRE(c && ncol >= 0 && (cols || ncol == 0) && (g || ncol == 0), "ctx_set_gathers: bad args");
+  int nbx = 0, nby = 0;
+  if (ncol) {
+    int rc = qpsdm_gather_bins(g, &nbx, &nby);
+    if (rc) return rc;
+    QPSDM_REQUIRE(c->p.table_dtype == QPSDM_F32 || c->p.table_dtype == QPSDM_U16X,
+                  "ctx_set_gathers: gathers need QPSDM_F32 or QPSDM_U16X tables");
+    for (int i = 0; i < ncol; ++i)
+      QPSDM_REQUIRE(cols[2 * i] >= 0 && cols[2 * i] < c->p.image.nx && cols[2 * i + 1] >= 0 &&
+                        cols[2 * i + 1] < c->p.image.ny,
+                    "ctx_set_gathers: column %d (%d, %d) outside the %d x %d image", i, cols[2 * i],
+                    cols[2 * i + 1], c->p.image.nx, c->p.image.ny);
+  }
+  int rc = qpsdm_ctx_flush(c);
+  if (rc) return rc;
+  QPSDM_HIP(hipStreamSynchronize(c->compute));
+  if (c->gcols) QPSDM_HIP(hipFree(c->gcols));
+  if (c->gx) QPSDM_HIP(hipFree(c->gx));
+  if (c->gy) QPSDM_HIP(hipFree(c->gy));
+  c->gcols = nullptr;
+  c->gx = c->gy = nullptr;
+  c->ngcol = c->nbx = c->nby = 0;
+  if (!ncol) return QPSDM_OK;
+  const size_t nz = (size_t)c->p.image.nz;
+  QPSDM_HIP(hipMalloc(&c->gcols, (size_t)ncol * 8));
+  QPSDM_HIP(hipMemcpy(c->gcols, cols, (size_t)ncol * 8, hipMemcpyHostToDevice));
+  if (nbx) {
+    QPSDM_HIP(hipMalloc(&c->gx, (size_t)ncol * nbx * nz * 4));
+    QPSDM_HIP(hipMemset(c->gx, 0, (size_t)ncol * nbx * nz * 4));
+  }
+  if (nby) {
+    QPSDM_HIP(hipMalloc(&c->gy, (size_t)ncol * nby * nz * 4));
+    QPSDM_HIP(hipMemset(c->gy, 0, (size_t)ncol * nby * nz * 4));
+  }
+  c->gp = *g;
+  c->ngcol = ncol;
+  c->nbx = nbx;
+  c->nby = nby;
+  return QPSDM_OK;
+}
+
+// The gathers to host memory, divided by nsb when normalize != 0 (as the
+// image, src/q-psdm.cu:1760-1766).
+extern "C" int qpsdm_ctx_gathers(qpsdm_ctx *c, float *gx, float *gy, int normalize) {
+  QPSDM_REQUIRE(c, "ctx_gathers: null ctx");
+  QPSDM_REQUIRE(c->ngcol, "ctx_gathers: no gathers set (qpsdm_ctx_set_gathers)");
+  QPSDM_REQUIRE((!c->nbx || gx) && (!c->nby || gy), "ctx_gathers: null output for a gathered axis");
+  int rc = qpsdm_ctx_flush(c);
+  if (rc) return rc;
+  QPSDM_HIP(hipStreamSynchronize(c->compute));
+  const float sc = 1.0f / (float)c->p.bank.win.nsb;
+  const size_t nz = (size_t)c->p.image.nz;
+  float *dev[2] = {c->gx, c->gy};
+  float *host[2] = {gx, gy};
+  const int nb[2] = {c->nbx, c->nby};
+  for (int a = 0; a < 2; ++a) {
+    if (!nb[a]) continue;
+    const size_t n = (size_t)c->ngcol * nb[a] * nz;
+    QPSDM_HIP(hipMemcpy(host[a], dev[a], n * 4, hipMemcpyDeviceToHost));
+    if (normalize)
+      for (size_t i = 0; i < n; ++i) host[a][i] *= sc;
+  }
   return QPSDM_OK;
 }
 

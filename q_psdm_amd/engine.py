@@ -448,6 +448,23 @@ class Pipeline:
         L.lib().qpsdm_su_sgettr(segy_record, tr)
         check(L.lib().qpsdm_ctx_psdm_kernel(self.ctx, tr))
 
+    def set_gathers(self, cols, dipx=50.0, dipy=50.0, ddipx=1.0, ddipy=1.0, aspect="all"):
+        """Dip-angle gathers at image columns cols [ncol][2] (ix, iy),
+        accumulated beside the image from the next batch on."""
+        cols = np.ascontiguousarray(cols, np.int32).reshape(-1, 2)
+        g, nbx, nby = self.survey.gather_params(dipx, dipy, ddipx, ddipy, aspect)
+        check(L.lib().qpsdm_ctx_set_gathers(self.ctx, C.byref(g), cols.ctypes.data_as(C.c_void_p), cols.shape[0]))
+        self._gshape = (cols.shape[0], nbx, nby)
+
+    def gathers(self, normalize=False):
+        ncol, nbx, nby = self._gshape
+        nz = self.survey.cfg.nz
+        gx = np.zeros((ncol, nbx, nz), np.float32) if nbx else None
+        gy = np.zeros((ncol, nby, nz), np.float32) if nby else None
+        check(L.lib().qpsdm_ctx_gathers(self.ctx, gx.ctypes.data_as(C.c_void_p) if nbx else None,
+                                        gy.ctypes.data_as(C.c_void_p) if nby else None, 1 if normalize else 0))
+        return gx, gy
+
     def image(self, normalize=False):
         c = self.survey.cfg
         out = np.zeros((c.ny, c.nx, c.nz), np.float32)

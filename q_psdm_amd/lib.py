@@ -123,7 +123,7 @@ EXPORTS = [
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
     "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
     "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
-    "qpsdm_gather_bins", "qpsdm_gathers_shot",
+    "qpsdm_gather_bins", "qpsdm_gathers_shot", "qpsdm_ctx_set_gathers", "qpsdm_ctx_gathers",
 ]
 
 SEGY_TOTAL_KEY_NUM = 75
@@ -190,6 +190,8 @@ def lib():
         L.qpsdm_segy_decode_dev.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
         L.qpsdm_ctx_push_segy.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
         L.qpsdm_gather_bins.argtypes = [C.POINTER(GatherParams), ip, ip]
+        L.qpsdm_ctx_set_gathers.argtypes = [vp, C.POINTER(GatherParams), vp, C.c_int]
+        L.qpsdm_ctx_gathers.argtypes = [vp, vp, vp, C.c_int]
         L.qpsdm_gathers_shot.argtypes = [C.POINTER(ImageParams), C.POINTER(GatherParams), vp, C.c_int, vp, vp, vp,
                                          C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]
         L.qpsdm_su_sputtr.argtypes = [vp, vp]

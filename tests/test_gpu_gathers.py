@@ -98,3 +98,38 @@ def test_gathers_mixed_storage_and_accumulation(gpu):
     sv.gathers_shot(None, None, cols_d, g_d, 0, n, b32, tab, none)   # nothing to do
     with pytest.raises(Exception):
         sv.gathers_shot(gx, gy, cols_d, g_d, 0, n, b32, sv.quantize_tables(tab, code="u16"), gp)
+
+
+def test_pipeline_gathers_match_direct_calls(gpu):
+    """Gathers set on the pipeline context and fed by SU records equal the
+    operator called on the same batches (same kernel, same trace groups);
+    normalised by nsb like the image; reset with it."""
+    from tests.test_gpu_pipeline import _gp, _setup
+    cfg, sv, tab, wxy, rel, traces = _setup()
+    cols = COLS[(COLS[:, 0] < cfg.nx) & (COLS[:, 1] < cfg.ny)]
+    batch = 6
+    pl = engine.Pipeline(sv, _gp(cfg), batch_traces=batch)
+    pl.set_tables(tab)
+    pl.set_gathers(cols, dipx=40.0, dipy=40.0, ddipx=2.0, ddipy=2.0)
+    pl.push_su(engine.su_encode(traces, wxy))
+    gx, gy = pl.gathers()
+    nx, ny = pl.gathers(normalize=True)
+    gp = sv.gather_params(dipx=40.0, dipy=40.0, ddipx=2.0, ddipy=2.0)
+    _, nbx, nby = gp
+    cols_d = torch.from_numpy(np.ascontiguousarray(cols)).cuda()
+    g_d = torch.from_numpy(rel).cuda()
+    bank = sv.qbank(torch.from_numpy(traces).cuda())
+    dx, dy = sv.new_gathers(len(cols), nbx), sv.new_gathers(len(cols), nby)
+    for tr0 in range(0, traces.shape[0], batch):
+        n = min(batch, traces.shape[0] - tr0)
+        sv.gathers_shot(dx, dy, cols_d, g_d, tr0, n, bank[tr0:tr0 + n], tab, gp)
+    torch.cuda.synchronize()
+    assert np.array_equal(gx, dx.cpu().numpy()) and np.array_equal(gy, dy.cpu().numpy())
+    assert np.abs(gx).sum() > 0
+    np.testing.assert_allclose(nx, gx / sv.win.nsb, rtol=1e-6)
+    pl.reset_image()
+    zx, zy = pl.gathers()
+    assert not zx.any() and not zy.any()
+    with pytest.raises(Exception):
+        pl.set_gathers([[cfg.nx, 0]])
+    pl.close()
