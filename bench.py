@@ -453,8 +453,8 @@ def main():
     ap.add_argument("--probe-timeout", type=int, default=240)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; how the path would shard is described in "
-                          "DESIGN.md and not run", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "runs on one GPU only (--gpus 1): sharding the imaging across GPUs "
+                          "(by image slab or by shot) is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.probe:
         probe(args)
