@@ -29,11 +29,11 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxPasses = 16;
 
+// Passes of each radix, 4 bits per radix in the order 4, 2, 3, 5, 7, 11,
+// 13 (the order they run in).
 struct Plan {
-  int npass;
-  int radix[kMaxPasses];
+  unsigned long long code;
 };
 
 struct BankDev {
@@ -56,20 +56,55 @@ __device__ inline float2 cmul(float2 a, float2 b) {
 }
 __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
-// One Stockham pass of radix R (SIGN -1 forward, +1 inverse).
+// cos and sin of 2 pi m / R, m = 0 .. R-1 (FP64 values rounded to FP32), for
+// the odd-radix butterflies: compile-time constants, so the butterflies
+// take them as instruction literals instead of registers or LDS reads.
+template <int R>
+struct DftConst;
+template <>
+struct DftConst<3> {
+  static constexpr float c[3] = {1.0f, -0.5f, -0.5f};
+  static constexpr float s[3] = {0.0f, 0.866025388f, -0.866025388f};
+};
+template <>
+struct DftConst<5> {
+  static constexpr float c[5] = {1.0f, 0.309017003f, -0.809017003f, -0.809017003f, 0.309017003f};
+  static constexpr float s[5] = {0.0f, 0.95105654f, 0.587785244f, -0.587785244f, -0.95105654f};
+};
+template <>
+struct DftConst<7> {
+  static constexpr float c[7] = {1.0f, 0.623489797f, -0.222520933f, -0.90096885f, -0.90096885f, -0.222520933f, 0.623489797f};
+  static constexpr float s[7] = {0.0f, 0.781831503f, 0.974927902f, 0.433883727f, -0.433883727f, -0.974927902f, -0.781831503f};
+};
+template <>
+struct DftConst<11> {
+  static constexpr float c[11] = {1.0f, 0.841253519f, 0.415415019f, -0.142314836f, -0.654860735f, -0.959492981f, -0.959492981f, -0.654860735f, -0.142314836f, 0.415415019f, 0.841253519f};
+  static constexpr float s[11] = {0.0f, 0.540640831f, 0.909631968f, 0.989821434f, 0.755749583f, 0.281732559f, -0.281732559f, -0.755749583f, -0.989821434f, -0.909631968f, -0.540640831f};
+};
+template <>
+struct DftConst<13> {
+  static constexpr float c[13] = {1.0f, 0.885456026f, 0.568064749f, 0.120536678f, -0.3546049f, -0.748510778f, -0.970941842f, -0.970941842f, -0.748510778f, -0.3546049f, 0.120536678f, 0.568064749f, 0.885456026f};
+  static constexpr float s[13] = {0.0f, 0.46472317f, 0.822983861f, 0.992708862f, 0.935016215f, 0.663122654f, 0.239315659f, -0.239315659f, -0.663122654f, -0.935016215f, -0.992708862f, -0.822983861f, -0.46472317f};
+};
+
+// One Stockham pass of radix R (SIGN -1 forward, +1 inverse).  The
+// butterfly index b splits into (b / Ns, b % Ns) by a float reciprocal:
+// floor((b + 1/2) / Ns) is exact for b < 2^21 (the half keeps the product
+// away from integers by more than its rounding error).  Radices 2 and 4 are
+// written out; odd radices use the symmetric form of the DFT: with
+// a_q = v_q + v_{R-q}, d_q = v_q - v_{R-q} and W^{pq} = c + i s, outputs p
+// and R - p are v_0 + sum_q a_q c  +/-  i sum_q d_q s (q = 1 .. (R-1)/2),
+// about a third of the naive product count and (R-1)/2 complex pairs live
+// instead of R inputs and R twiddles.
 template <int R, int SIGN>
 __device__ void stockham_pass(const float2 *__restrict__ in, float2 *__restrict__ out,
                               const float2 *__restrict__ tw, int N, int Ns) {
   const int nb = N / R;
   const int step = N / (Ns * R);
-  float2 wr[R];
-#pragma unroll
-  for (int m = 0; m < R; ++m) {
-    float2 w = tw[m * (N / R)];
-    wr[m] = SIGN > 0 ? cconj(w) : w;
-  }
+  const float inv_ns = 1.0f / (float)Ns;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const int k = b % Ns;
+    const int hi = (int)(((float)b + 0.5f) * inv_ns);
+    const int k = b - hi * Ns;
     float2 v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) v[q] = in[b + q * nb];
@@ -78,42 +113,79 @@ __device__ void stockham_pass(const float2 *__restrict__ in, float2 *__restrict_
       float2 w = tw[q * k * step];
       v[q] = cmul(v[q], SIGN > 0 ? cconj(w) : w);
     }
-    const int base = (b / Ns) * Ns * R + k;
+    float2 *o = out + hi * Ns * R + k;
+    if constexpr (R == 2) {
+      o[0] = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
+      o[Ns] = make_float2(v[0].x - v[1].x, v[0].y - v[1].y);
+    } else if constexpr (R == 4) {
+      const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+      const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+      const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+      const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+      // W d13 with W = -i (forward) or +i (inverse)
+      const float2 wd = SIGN > 0 ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+      o[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+      o[Ns] = make_float2(d02.x + wd.x, d02.y + wd.y);
+      o[2 * Ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
+      o[3 * Ns] = make_float2(d02.x - wd.x, d02.y - wd.y);
+    } else {
+      constexpr int H = (R - 1) / 2;
+      float2 a[H], d[H];
 #pragma unroll
-    for (int p = 0; p < R; ++p) {
-      float2 acc = v[0];
-#pragma unroll
-      for (int q = 1; q < R; ++q) {
-        const float2 w = wr[(p * q) % R];
-        acc.x += v[q].x * w.x - v[q].y * w.y;
-        acc.y += v[q].x * w.y + v[q].y * w.x;
+      for (int q = 1; q <= H; ++q) {
+        a[q - 1] = make_float2(v[q].x + v[R - q].x, v[q].y + v[R - q].y);
+        d[q - 1] = make_float2(v[q].x - v[R - q].x, v[q].y - v[R - q].y);
       }
-      out[base + p * Ns] = acc;
+      float2 y0 = v[0];
+#pragma unroll
+      for (int q = 0; q < H; ++q) y0 = make_float2(y0.x + a[q].x, y0.y + a[q].y);
+      o[0] = y0;
+#pragma unroll
+      for (int p = 1; p <= H; ++p) {
+        float2 A = v[0], B = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+          constexpr float sg = SIGN > 0 ? 1.0f : -1.0f;
+          const float c = DftConst<R>::c[(p * q) % R], s = sg * DftConst<R>::s[(p * q) % R];
+          A.x = fmaf(a[q - 1].x, c, A.x);
+          A.y = fmaf(a[q - 1].y, c, A.y);
+          B.x = fmaf(d[q - 1].x, s, B.x);
+          B.y = fmaf(d[q - 1].y, s, B.y);
+        }
+        // outputs A + i B and A - i B, i B = (-B.y, B.x)
+        o[p * Ns] = make_float2(A.x - B.y, A.y + B.x);
+        o[(R - p) * Ns] = make_float2(A.x + B.y, A.y - B.x);
+      }
     }
   }
 }
 
-// Runs the plan; data starts in a, the result pointer is returned.
-template <int SIGN>
-__device__ float2 *fft_run(float2 *a, float2 *b, const float2 *tw, int N, const Plan &pl) {
-  int Ns = 1;
-  for (int p = 0; p < pl.npass; ++p) {
-    const int R = pl.radix[p];
-    switch (R) {
-      case 2: stockham_pass<2, SIGN>(a, b, tw, N, Ns); break;
-      case 3: stockham_pass<3, SIGN>(a, b, tw, N, Ns); break;
-      case 4: stockham_pass<4, SIGN>(a, b, tw, N, Ns); break;
-      case 5: stockham_pass<5, SIGN>(a, b, tw, N, Ns); break;
-      case 7: stockham_pass<7, SIGN>(a, b, tw, N, Ns); break;
-      case 11: stockham_pass<11, SIGN>(a, b, tw, N, Ns); break;
-      default: stockham_pass<13, SIGN>(a, b, tw, N, Ns); break;
-    }
+template <int R, int SIGN>
+__device__ inline void passes(float2 *&a, float2 *&b, const float2 *tw, int N, int &Ns, int count) {
+  for (int c = 0; c < count; ++c) {
+    stockham_pass<R, SIGN>(a, b, tw, N, Ns);
     Ns *= R;
     __syncthreads();
     float2 *t = a;
     a = b;
     b = t;
   }
+}
+
+// Runs the plan: the passes of each radix in a loop of their own, in the
+// order 4, 2, 3, 5, 7, 11, 13 (a switch over the radix inside one loop made
+// the compiler hold every butterfly's values at once: 248 registers and
+// stack spills).  Data starts in a; the result pointer is returned.
+template <int SIGN>
+__device__ float2 *fft_run(float2 *a, float2 *b, const float2 *tw, int N, unsigned long long code) {
+  int Ns = 1;
+  passes<4, SIGN>(a, b, tw, N, Ns, (int)(code & 15));
+  passes<2, SIGN>(a, b, tw, N, Ns, (int)((code >> 4) & 15));
+  passes<3, SIGN>(a, b, tw, N, Ns, (int)((code >> 8) & 15));
+  passes<5, SIGN>(a, b, tw, N, Ns, (int)((code >> 12) & 15));
+  passes<7, SIGN>(a, b, tw, N, Ns, (int)((code >> 16) & 15));
+  passes<11, SIGN>(a, b, tw, N, Ns, (int)((code >> 20) & 15));
+  passes<13, SIGN>(a, b, tw, N, Ns, (int)((code >> 24) & 15));
   return a;
 }
 
@@ -166,8 +238,12 @@ __device__ inline float block_max(float m) {
 // ways, and FP16's relative precision does not depend on how far below the
 // bound the samples lie (they stay far above its subnormal range).  The
 // row's 16-byte header holds {1/scale, scale, 0, 0}.
+// Held to three waves per SIMD: the compiler's own schedule of the odd
+// butterflies wants ~184 registers (two waves); at three it spills a few
+// values outside the butterflies and runs 15 % faster, at four it spills
+// inside them and runs slower.
 template <int MODE, typename OutT>
-__global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float *__restrict__ traces,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void qbank_kernel(BankDev P, const float *__restrict__ traces,
                                                          OutT *__restrict__ bank,
                                                          float *__restrict__ re_out,
                                                          float *__restrict__ im_out) {
@@ -196,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float 
     A[n] = make_float2(taper_apply(s1, n, N, tap), taper_apply(s2, n, N, tap));
   }
   __syncthreads();
-  const float2 *Z = fft_run<-1>(A, B, tw, N, P.plan);
+  const float2 *Z = fft_run<-1>(A, B, tw, N, P.plan.code);
   for (int k = threadIdx.x; k < P.nww; k += blockDim.x) {
     const int m = P.nf1 + k;
     const float2 zp = Z[m], zm = cconj(Z[N - m]);
@@ -233,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void qbank_kernel(BankDev P, const float 
       A[n] = v;
     }
     __syncthreads();
-    return fft_run<+1>(A, B, tw, N, P.plan);
+    return fft_run<+1>(A, B, tw, N, P.plan.code);
   };
   auto level_sample = [&](const float2 *Y, int it) {
     const bool s2 = it * P.dt > P.tdmid;
@@ -300,14 +376,17 @@ std::mutex g_mu;
 std::map<std::vector<double>, Coefs> g_cache;
 
 bool make_plan(int n, Plan *pl) {
-  pl->npass = 0;
+  pl->code = 0;
   const int order[] = {4, 2, 3, 5, 7, 11, 13};
-  for (int r : order) {
-    while (n % r == 0 && pl->npass < kMaxPasses) {
+  for (int i = 0; i < 7; ++i) {
+    const int r = order[i];
+    unsigned long long c = 0;
+    while (n % r == 0 && c < 15) {
       if (r == 2 && n % 4 == 0) break;
-      pl->radix[pl->npass++] = r;
+      ++c;
       n /= r;
     }
+    pl->code |= c << (4 * i);
   }
   return n == 1;
 }
