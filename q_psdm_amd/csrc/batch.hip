@@ -116,17 +116,19 @@ __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, 
   }
 }
 
-// One work-group per tile (nwg = -1): the persistent kernel's tile body and
-// order without its loop, scheduled by the hardware dispatcher.  Register
-// use is held to four waves per SIMD (the few values that do not fit are
-// kept in scratch across the loop over shots, outside the trace loop).
+// One work-group per tile (nwg = -1): one launch per shot of the batch, in
+// shot order, each the tile body for that shot alone (no loop over shots in
+// the kernel: its registers are the per-shot kernel's, with no spills), so
+// per image point the sum still runs over the shots in order and each
+// shot's traces in order.  Scheduled by the hardware dispatcher.
 template <int ZP, typename BankT, typename Tab>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+__global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                                const float4 *__restrict__ geom,
                                                                const BankT *__restrict__ bank, ShotRuns runs,
-                                                               int nshot, int ncol) {
+                                                               int shot, int ncol) {
   const int tile = (int)blockIdx.x;
-  tile_all_shots<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+  const OneShot one{runs(shot)};
+  qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
 }
 
 template <int ZP>
@@ -161,9 +163,11 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   if (go.nwg == -1) {
     if (go.nwg_out) *go.nwg_out = -1;
     if (!go.run) return QPSDM_OK;
-    hipLaunchKernelGGL((image_batch_tiles_kernel<ZP, BankT, Tab>), dim3(ncol * nzg), dim3(64), 0, go.s, P, tab,
-                       go.img, go.geom, bank, go.runs, go.nshot, ncol);
-    QPSDM_HIP(hipGetLastError());
+    for (int r = 0; r < go.nshot; ++r) {
+      hipLaunchKernelGGL((image_batch_tiles_kernel<ZP, BankT, Tab>), dim3(ncol * nzg), dim3(64), 0, go.s, P, tab,
+                         go.img, go.geom, bank, go.runs, r, ncol);
+      QPSDM_HIP(hipGetLastError());
+    }
     return QPSDM_OK;
   }
   int nwg = go.nwg;
