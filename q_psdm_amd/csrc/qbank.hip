@@ -465,6 +465,42 @@ int check_params(const qpsdm_bank_params *p) {
   return QPSDM_OK;
 }
 
+// One work-group holds a trace's windows, their spectra and the twiddles in
+// LDS: 8 (3 nsb + 2 nww) bytes plus the static taper table.  Past the
+// default per-work-group limit the kernel is opted in to the device's larger
+// limit where there is one (hipDeviceAttributeSharedMemPerBlockOptin); past
+// that, an error names the limit instead of a launch that fails.
+template <typename Kern>
+int fit_kernel_lds(Kern kern, size_t lds, int nsb) {
+  int dev = 0, base = 0, optin = 0;
+  QPSDM_HIP(hipGetDevice(&dev));
+  QPSDM_HIP(hipDeviceGetAttribute(&base, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    optin = 0;
+  }
+  const size_t total = lds + 1024;  // static LDS of the kernel (taper table, block reduction)
+  const size_t limit = (size_t)(optin > base ? optin : base);
+  QPSDM_REQUIRE(total <= limit,
+                "bank: window of nsb=%d needs %zu bytes of LDS per work-group; this device allows %zu", nsb, total,
+                limit);
+  if (total > (size_t)base)
+    QPSDM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+  return QPSDM_OK;
+}
+
+// spectra selects the spectra-only kernel; else dtype the bank's.
+int fit_lds(size_t lds, int nsb, bool spectra, int dtype) {
+  if (spectra) return fit_kernel_lds(qbank_kernel<0, float>, lds, nsb);
+  switch (dtype) {
+    case QPSDM_F32: return fit_kernel_lds(qbank_kernel<1, float>, lds, nsb);
+    case QPSDM_BF16: return fit_kernel_lds(qbank_kernel<1, __hip_bfloat16>, lds, nsb);
+    case QPSDM_F16: return fit_kernel_lds(qbank_kernel<1, __half>, lds, nsb);
+    default: return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16", dtype);
+  }
+}
+
 int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank, int dtype,
            float *re, float *im, void *stream) {
   int rc = check_params(p);
@@ -496,6 +532,8 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
   P.wtab = c->wtab;
   P.taper = c->taper;
   const size_t lds = sizeof(float2) * (3 * (size_t)P.nsb + 2 * P.nww);
+  rc = fit_lds(lds, P.nsb, re != nullptr, dtype);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (re) {
     hipLaunchKernelGGL((qbank_kernel<0, float>), dim3(ntr), dim3(kThreads), lds, s, P, traces,

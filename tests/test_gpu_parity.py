@@ -5,6 +5,7 @@ import torch
 
 from oracle import oracle as O
 from q_psdm_amd import engine, synth
+from q_psdm_amd import lib as L
 from tests.helpers import o_bparams, o_iparams, o_model, o_tgeom, o_window, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -354,3 +355,24 @@ def test_mixed_forms_on_incoherent_traces(gpu):
     assert err["f16"] < 4e-4
     assert err["f16_u16x"] < 1e-3       # the mixed form's bound holds with FP16 samples
     assert err["bf16_u16x"] > err["f16_u16x"]
+
+
+def test_qbank_long_window(gpu):
+    """An 8 s trace at 1 ms: its window (nsb = 4576) needs about 120 KB of
+    LDS per work-group, past the default 64 KB.  Where the device lets one
+    work-group have that much, the bank matches the FP64 oracle; where it
+    does not, the call reports the limit instead of failing the launch."""
+    cfg = synth.tiny()
+    cfg.nt, cfg.dt, cfg.K = 8001, 0.001, 3
+    sv = engine.Survey(cfg, 0.02)
+    x = np.random.default_rng(31).standard_normal((2, cfg.nt)).astype(np.float32)
+    print(f"long window: nsb={sv.win.nsb} nww={sv.win.nww}", flush=True)
+    try:
+        got = sv.qbank(torch.from_numpy(x).cuda()).cpu().numpy()
+    except L.QpsdmError as e:
+        assert "LDS" in str(e)
+        pytest.skip(f"device LDS limit: {e}")
+    ref = O.qbank(x, o_bparams(sv))
+    for i in range(2):
+        for j in range(cfg.K):
+            assert rel_l2(got[i, j], ref[i, j]) < 2e-5, (i, j)
