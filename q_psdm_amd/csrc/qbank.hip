@@ -505,7 +505,7 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
            float *re, float *im, void *stream) {
   int rc = check_params(p);
   if (rc) return rc;
-  QPSDM_REQUIRE(traces && ntr >= 0, "bank: bad traces");
+  QPSDM_REQUIRE((traces || ntr == 0) && ntr >= 0, "bank: bad traces");
   if (ntr == 0) return QPSDM_OK;
   Coefs *c = nullptr;
   rc = get_coefs(p, &c);
@@ -567,7 +567,7 @@ int qpsdm_qbank_prepare(const qpsdm_bank_params *p) {
 
 extern "C" int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces, int ntr,
                                    float *re, float *im, void *stream) {
-  QPSDM_REQUIRE(re && im, "spectra: null outputs");
+  QPSDM_REQUIRE((re && im) || ntr == 0, "spectra: null outputs");
   return launch(p, traces, ntr, nullptr, QPSDM_F32, re, im, stream);
 }
 
@@ -584,6 +584,6 @@ extern "C" size_t qpsdm_bank_row_bytes(int K, int nt, int dtype) {
 
 extern "C" int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
                            int dtype, void *stream) {
-  QPSDM_REQUIRE(bank, "bank: null output");
+  QPSDM_REQUIRE(bank || ntr == 0, "bank: null output");
   return launch(p, traces, ntr, bank, dtype, nullptr, nullptr, stream);
 }
