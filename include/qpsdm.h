@@ -274,12 +274,11 @@ typedef struct {
 
 /* shots_dev: device array of nshot descriptors; geom/bank hold every trace
  * of the batch (bank row = geom row); work_dev: a device uint32 work counter
- * (reset by the call); nwg persistent work-groups (0 = resident capacity),
- * or nwg = -1: one launch per shot (in shot order) with one work-group per
- * tile, scheduled by the hardware instead of the counter (same tiles and
- * summation order, so the same image bit for bit; the kernel then keeps no
- * state across tiles or shots, which lowers its register use).  The image
- * params' own dip bounds are not used; taperzone is. */
+ * (reset by the call before each shot); nwg persistent work-groups (0 =
+ * resident capacity), or nwg = -1: one work-group per tile, scheduled by the
+ * hardware instead of the counter.  Either grid is dispatched once per shot,
+ * in shot order (same tiles and summation order, so the same image bit for
+ * bit).  The image params' own dip bounds are not used; taperzone is. */
 int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                       const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
                       int table_dtype, const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,

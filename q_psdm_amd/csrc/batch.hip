@@ -6,9 +6,10 @@
 // The persistent grid is sized to the resident capacity of the device; its
 // work-groups take image tiles from one atomic counter in depth-slab order
 // (neighbouring columns of a slab together, so tiles in flight share table
-// and bank lines in L2) until the tiles run out.  A tile is owned by one
-// work-group for every shot of the batch, so the image needs no atomics and
-// the summation order (shots, then traces, in order) is fixed.
+// and bank lines in L2) until the tiles run out, one dispatch per shot.  A
+// tile is owned by one work-group per shot and the shots' dispatches run in
+// order on one stream, so the image needs no atomics and the summation
+// order (shots, then traces, in order) is fixed.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -68,21 +69,6 @@ struct OneShot {
   __device__ qk::Run operator()(int) const { return run; }
 };
 
-// All shots of a batch through one image tile: the tile body per shot (one
-// run, so the compiler sees no loop over runs), its accumulators read from
-// and written back to the image around each shot.  Per image point the sum
-// runs over the shots in order and each shot's traces in order, exactly as
-// consecutive per-shot launches.
-template <int ZP, typename BankT, typename Tab>
-__device__ inline void tile_all_shots(const qk::ImgDev &P, const Tab &tab, float *__restrict__ img,
-                                      const float4 *__restrict__ geom, const BankT *__restrict__ bank,
-                                      int tile_col, int zgroup, const ShotRuns &runs, int nshot) {
-  for (int r = 0; r < nshot; ++r) {
-    const OneShot one{runs(r)};
-    qk::image_tile<ZP>(P, tab, img, geom, bank, tile_col, zgroup, one, 1);
-  }
-}
-
 ShotRuns shot_runs(const qpsdm_image_params *p, const qpsdm_shot *shots, int nshot, const void *src) {
   ShotRuns R{shots, p->taperzone, nullptr, nullptr, 0, 0, 0};
   if (src) {
@@ -98,11 +84,18 @@ __global__ void reset_kernel(unsigned *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
+// Persistent grid: one dispatch per shot of the batch, in shot order (the
+// tile body for that shot alone, so its registers are the per-shot
+// kernel's); the work-groups take the shot's tiles from the counter, which
+// is cleared by a kernel before each dispatch.  Per image point the sum runs
+// over the shots in order and each shot's traces in order, exactly as
+// consecutive per-shot launches.
 template <int ZP, typename BankT, typename Tab>
 __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                          const float4 *__restrict__ geom,
                                                          const BankT *__restrict__ bank, ShotRuns runs,
-                                                         int nshot, unsigned *counter, int ncol, int total) {
+                                                         int shot, unsigned *counter, int ncol, int total) {
+  const OneShot one{runs(shot)};
   for (;;) {
     unsigned k = 0;
     if (threadIdx.x == 0) k = atomicAdd(counter, 1u);
@@ -112,7 +105,7 @@ __global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, 
     if (tile >= total) break;
     // depth group slowest: consecutive tiles are neighbouring columns of one
     // depth slab, which share table and bank lines in L2
-    tile_all_shots<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, runs, nshot);
+    qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
   }
 }
 
@@ -181,10 +174,12 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   if (go.nwg_out) *go.nwg_out = nwg;
   if (!go.run) return QPSDM_OK;
   const hipStream_t s = go.s;
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, 1);
-  hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, go.nshot, go.work,
-                     ncol, ncol * nzg);
-  QPSDM_HIP(hipGetLastError());
+  for (int r = 0; r < go.nshot; ++r) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, go.work, 1);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), 0, s, P, tab, go.img, go.geom, bank, go.runs, r, go.work, ncol,
+                       ncol * nzg);
+    QPSDM_HIP(hipGetLastError());
+  }
   return QPSDM_OK;
 }
 
