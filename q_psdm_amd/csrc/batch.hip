@@ -89,12 +89,13 @@ __global__ void reset_kernel(unsigned *c, int n) {
 // kernel's); the work-groups take the shot's tiles from the counter, which
 // is cleared by a kernel before each dispatch.  Per image point the sum runs
 // over the shots in order and each shot's traces in order, exactly as
-// consecutive per-shot launches.
+// consecutive per-shot launches.  Held to four waves per SIMD (128 VGPRs):
+// uncapped, the tile loop around the body takes 134 and three waves fit.
 template <int ZP, typename BankT, typename Tab>
-__global__ __launch_bounds__(64) void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
-                                                         const float4 *__restrict__ geom,
-                                                         const BankT *__restrict__ bank, ShotRuns runs,
-                                                         int shot, unsigned *counter, int ncol, int total) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const float4 *__restrict__ geom,
+                        const BankT *__restrict__ bank, ShotRuns runs, int shot, unsigned *counter, int ncol,
+                        int total) {
   const OneShot one{runs(shot)};
   for (;;) {
     unsigned k = 0;
