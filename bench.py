@@ -77,7 +77,7 @@ class Workload:
         self.sv = engine.Survey(cfg, engine.tstar_max_from_tables(self.tab))
         log(f"tables {tuple(self.tab.shape)} {self.tab.numel() * 4 / 1e9:.2f} GB in {time.time() - t0:.1f}s; "
             f"dtstar={self.sv.dtstar * 1e3:.3f} ms nsb={self.sv.win.nsb} nww={self.sv.win.nww}")
-        self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16x")
+        self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16q")
         shots = synth.shots_config2(nshots=nshot)
         self.geom, self.ranges = synth.geometry(shots)
         self.g_d = torch.from_numpy(self.geom).cuda()
@@ -185,10 +185,10 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
         pre_ms = [es[0][0].elapsed_time(es[0][1]) for es in ev]
         bank_ms = None
         launches = len(img_ms)
-    # kernel dispatches per timed imaging launch: the one-group-per-tile grid
-    # dispatches once per shot of the step; the roofline figures are per
-    # dispatch, like rocprofv3's kernel averages
-    per = W.S if form != "fp32" and nwg == -1 else 1
+    # kernel dispatches per timed imaging launch: both batched grids dispatch
+    # once per shot of the step; the roofline figures are per dispatch, like
+    # rocprofv3's kernel averages
+    per = W.S if form != "fp32" else 1
     avg_img = float(np.mean(img_ms)) / 1e3 / per
     launches *= per
     achieved = upd / launches * BYTES_PER_UPDATE[form] / avg_img / 1e9
@@ -344,7 +344,7 @@ def cpu_baseline(W, ntr):
 def time_end_to_end(W, torch, engine, nshot=2, launch=None):
     """End-to-end rate through the pipeline context: big-endian SU records of
     nshot shots in host memory -> pinned staging slots -> H2D copy (its own
-    stream) -> device decode -> FP16 Q bank -> imaging with U16X tables ->
+    stream) -> device decode -> FP16 Q bank -> imaging with U16Q tables ->
     image copied back to host memory.  Wall clock from the first push to
     the host image; integer header coordinates (metres)."""
     from q_psdm_amd import lib as L
@@ -355,7 +355,7 @@ def time_end_to_end(W, torch, engine, nshot=2, launch=None):
     wxy = np.round(rows.astype(np.float64)).astype(np.int32)
     su = engine.su_encode(traces, wxy)
     gp = L.GeomParams(1.0, -1e30, 1e30, -1e30, 1e30, 0.0, 0.0, 0.0, 0.0, cfg.dx, cfg.dy)
-    pl = engine.Pipeline(sv, gp, bank_dtype="f16", table_dtype="u16x", batch_traces=max(W.ranges[i][1] for i in idx),
+    pl = engine.Pipeline(sv, gp, bank_dtype="f16", table_dtype="u16q", batch_traces=max(W.ranges[i][1] for i in idx),
                          launch=launch)
     pl.set_tables(W.tab)
     pl.push_su(su[:240 + 4 * cfg.nt])       # warm the kernels on one trace
@@ -373,7 +373,7 @@ def time_end_to_end(W, torch, engine, nshot=2, launch=None):
     return {"shots": len(idx), "traces": int(rows.shape[0]), "su_bytes": len(su), "wall_ms": wall * 1e3,
             "updates_per_s": upd / wall, "image_bytes_to_host": cfg.nx * cfg.ny * cfg.nz * 4,
             "path": "SU bytes in host memory -> pinned slot -> H2D (copy stream) -> device decode -> f16 Q bank -> "
-                    "per-shot imaging (u16x tables) -> image to host"}
+                    "per-shot imaging (u16q tables) -> image to host"}
 
 
 def time_segy_decode(torch, ntr=20000, ns=2001, reps=5):
@@ -449,8 +449,8 @@ def counter_run(counter, form, args, tmp, launch, nwg):
             if x["Kernel_Name"].startswith("(anonymous namespace)::copy_kernel")]
     if not kern or not copy:
         return None, f"{counter}: kernel rows {len(kern)}, copy rows {len(copy)}"
-    # per dispatch (one shot per dispatch, except the persistent grid's one
-    # dispatch per batch), as the roofline's launch time is
+    # per dispatch (one shot per dispatch in every form), as the roofline's
+    # launch time is
     return (float(np.mean(kern)), copy[0]), None
 
 
@@ -566,9 +566,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fast": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables; "
+        "dtype": {"fast": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables as corner quads; "
                           "bf16-mfma surrogate",
-                  "mixed": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables",
+                  "mixed": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables as corner quads",
                   "fp32": "f32"}[head],
         "data": "synthetic: seeded layered + anomaly model (layer Q ~ U[30,200]), jittered geometry, "
                 "band-limited reflectivity traces with 5% noise",
@@ -587,8 +587,9 @@ def main():
                      if traffic else None,
                      "note": "achieved counts the algorithmic operand bytes of every update; neighbouring updates "
                              "share table corners and trace samples through L1/L2, so it exceeds both the HBM "
-                             "peak and the measured HBM traffic (measured_GBs). The kernel is limited by vector-"
-                             "instruction issue and memory latency, not by HBM bandwidth."},
+                             "peak and the measured HBM traffic (measured_GBs). The kernel is limited by its "
+                             "vector-memory pipeline (texture address and data units busy nearly all of its "
+                             "time) and by instruction issue, not by HBM bandwidth."},
         "forms": results,
         "speedup_vs_fp32": (R["value"] / results["fp32"]["value"]) if "fp32" in results else None,
         "image_rel_l2": errors,

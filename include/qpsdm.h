@@ -40,6 +40,10 @@ extern "C" {
 #define QPSDM_F16 5  /* bank only: FP16 samples under a power-of-two scale per trace; a
                         trace's row is a 16-byte header {float 1/scale, float scale, 0, 0}
                         then [K][nt] f16, rows qpsdm_bank_row_bytes() apart */
+#define QPSDM_U16Q 6 /* tables: the U16X bricks' codes stored per cell: at each depth node
+                        the four bilinear corners' codes of a cell as one 16-byte word
+                        (3.1x the U16X bytes; one vector load per table node instead of
+                        four) */
 
 /* Two-window spectral set-up of one offset class; replaces the window block
  * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
@@ -119,14 +123,14 @@ int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab_de
  * QPSDM_U16B have the same size). */
 size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g);
 
-/* Size in bytes of a table set in storage type dtype (F32, F16B, U16B, U16X). */
+/* Size in bytes of a table set in storage type dtype (F32, F16B, U16B, U16X, U16Q). */
 size_t qpsdm_tables_bytes(const qpsdm_tgeom *g, int dtype);
 
 /* Quantise an FP32 table set into 16-bit bricks: per 8^3 brick of (t, t*)
  * an offset and a scale per quantity, codes h = (v - off)/scale in [0, 1]
  * stored as FP16 (QPSDM_F16B) or as round(h * 65535) (QPSDM_U16B); or per
  * 9x9x8 edge-sharing brick an offset and a step, codes round((v - off)/step)
- * (QPSDM_U16X). */
+ * (QPSDM_U16X), or the same codes as corner quads (QPSDM_U16Q). */
 int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, int code,
                           void *stream);
 

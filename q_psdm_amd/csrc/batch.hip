@@ -215,6 +215,10 @@ int batch_dispatch(const qpsdm_image_params *p, const qpsdm_launch *shape, const
     case QPSDM_F16B: return launch_batch_bank(P, qk::make_tab_b16<false>(p->tg, tab), bank, bank_dtype, go);
     case QPSDM_U16B: return launch_batch_bank(P, qk::make_tab_b16<true>(p->tg, tab), bank, bank_dtype, go);
     case QPSDM_U16X: return launch_batch_bank(P, qk::make_tab_u16x(p->tg, tab), bank, bank_dtype, go);
+    case QPSDM_U16Q:
+      QPSDM_REQUIRE(qk::u16q_fits(p->tg), "image_batch: U16Q node of %zu bytes (below 2 GiB)",
+                    qk::u16q_node_bytes(p->tg));
+      return launch_batch_bank(P, qk::make_tab_u16q(p->tg, tab), bank, bank_dtype, go);
     default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: table dtype %d", table_dtype);
   }
 }

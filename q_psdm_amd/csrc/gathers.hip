@@ -62,6 +62,15 @@ __device__ inline float2 at_lane(const qk::TabU16X &t, const qk::TabU16X::Col &c
   return t.reduce(c, q);
 }
 
+__device__ inline float2 at_lane(const qk::TabU16Q &t, const qk::TabU16Q::Col &c, int w) {
+  const int sb = (w >> 3) * qk::kQBrickBytes;
+  qk::TabU16Q::Raw q;
+  q.h = qk::TabU16Q::header(c, sb);
+  const uint4 v = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.o, sb + (w & 7) * 16, 0));
+  q.q00 = v.x, q.q01 = v.y, q.q10 = v.z, q.q11 = v.w;
+  return t.reduce(c, q);
+}
+
 // Bin of tangent v among n bins with boundaries e[0..n] (LDS), -1 outside.
 __device__ inline int bin_of(float v, const float *e, int n) {
   if (!(v >= e[0] && v < e[n])) return -1;
@@ -238,7 +247,12 @@ extern "C" int qpsdm_gathers_shot(const qpsdm_image_params *p, const qpsdm_gathe
     case QPSDM_U16X:
       return launch_bank(P, G, E, qk::make_tab_u16x(p->tg, tab_dev), cols, ncol, gx_dev, gy_dev, g4, bank_dev,
                          bank_dtype, s);
+    case QPSDM_U16Q:
+      QPSDM_REQUIRE(qk::u16q_fits(p->tg), "gathers: U16Q node of %zu bytes (below 2 GiB)", qk::u16q_node_bytes(p->tg));
+      return launch_bank(P, G, E, qk::make_tab_u16q(p->tg, tab_dev), cols, ncol, gx_dev, gy_dev, g4, bank_dev,
+                         bank_dtype, s);
     default:
-      return qpsdm::set_error(QPSDM_EINVAL, "gathers: table dtype %d (QPSDM_F32 or QPSDM_U16X)", table_dtype);
+      return qpsdm::set_error(QPSDM_EINVAL, "gathers: table dtype %d (QPSDM_F32, QPSDM_U16X or QPSDM_U16Q)",
+                              table_dtype);
   }
 }

@@ -110,6 +110,9 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
     case QPSDM_F16B: return launch_bank(P, qk::make_tab_b16<false>(p->tg, tab), img, g4, bank, bank_dtype, s);
     case QPSDM_U16B: return launch_bank(P, qk::make_tab_b16<true>(p->tg, tab), img, g4, bank, bank_dtype, s);
     case QPSDM_U16X: return launch_bank(P, qk::make_tab_u16x(p->tg, tab), img, g4, bank, bank_dtype, s);
+    case QPSDM_U16Q:
+      QPSDM_REQUIRE(qk::u16q_fits(p->tg), "image: U16Q node of %zu bytes (below 2 GiB)", qk::u16q_node_bytes(p->tg));
+      return launch_bank(P, qk::make_tab_u16q(p->tg, tab), img, g4, bank, bank_dtype, s);
     default: break;
   }
   return qpsdm::set_error(QPSDM_EINVAL, "image: bad dtypes bank=%d table=%d", bank_dtype,
@@ -152,6 +155,10 @@ extern "C" int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab,
     case QPSDM_F16B: return launch_lookup(P, qk::make_tab_b16<false>(p->tg, tab), px, py, xyz, n, out, s);
     case QPSDM_U16B: return launch_lookup(P, qk::make_tab_b16<true>(p->tg, tab), px, py, xyz, n, out, s);
     case QPSDM_U16X: return launch_lookup(P, qk::make_tab_u16x(p->tg, tab), px, py, xyz, n, out, s);
+    case QPSDM_U16Q:
+      QPSDM_REQUIRE(qk::u16q_fits(p->tg), "tables_lookup: U16Q node of %zu bytes (below 2 GiB)",
+                    qk::u16q_node_bytes(p->tg));
+      return launch_lookup(P, qk::make_tab_u16q(p->tg, tab), px, py, xyz, n, out, s);
     default: return qpsdm::set_error(QPSDM_EINVAL, "tables_lookup: table dtype %d", table_dtype);
   }
 }

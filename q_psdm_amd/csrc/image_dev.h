@@ -329,6 +329,94 @@ struct TabU16X {
   __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
 };
 
+// Corner-quad bricks (QPSDM_U16Q): the U16X bricks, same headers and
+// codes, with the codes stored per cell: brick (bv, bu, bw) is the float4
+// header, then for cells v in [8 bv, 8 bv + 8), u in [8 bu, 8 bu + 8) and
+// depth nodes w in [8 bw, 8 bw + 8), in [v][u][w] order, the codes of the
+// cell's corners (v, u), (v, u + 1), (v + 1, u), (v + 1, u + 1) as one
+// 16-byte word.  A column reads one word per depth node (the corners share
+// a cache line with the next seven nodes) instead of four: the imaging
+// kernel's vector-memory pipeline, not its arithmetic, sets its pace.
+constexpr int kQBrickBytes = 16 + 8 * 8 * 8 * 16;
+
+// Bilinear weights of a cell in 16-bit fixed point: wa = round(65535 a),
+// wb = round(65535 b) for the u and v weights, then w11 = floor(wa wb /
+// 65535), w01 = wa - w11, w10 = wb - w11 and w00 = 65535 - wa - wb + w11,
+// all in [0, 65535] and summing to exactly 65535, so a weighted sum of
+// 16-bit codes fits 32 bits and, divided by 65535, is a convex combination
+// of them.  floor(x / 65535) = (x + (x >> 16) + 1) >> 16 holds for every
+// x <= 65535^2 without overflow (checked over that whole range).
+// Packed per row: (w00 | w01 << 16) and (w10 | w11 << 16).
+__host__ __device__ inline void cell_weights(float a, float b, unsigned *row0, unsigned *row1) {
+  const unsigned wa = (unsigned)fmaf(a, 65535.0f, 0.5f), wb = (unsigned)fmaf(b, 65535.0f, 0.5f);
+  const unsigned x = wa * wb;
+  const unsigned w11 = (x + (x >> 16) + 1u) >> 16;
+  *row0 = (65535u - wa - wb + w11) | ((wa - w11) << 16);
+  *row1 = (wb - w11) | (w11 << 16);
+}
+
+typedef unsigned short u16pair_t __attribute__((ext_vector_type(2)));
+
+// x.lo w.lo + x.hi w.hi + acc over unsigned 16-bit halves (v_dot2_u32_u16)
+__device__ inline unsigned dot2_u16(unsigned x, unsigned w, unsigned acc) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16pair_t, x), __builtin_bit_cast(u16pair_t, w), acc, false);
+}
+
+struct TabU16Q {
+  const unsigned char *p;
+  size_t nstride;  // bytes per node
+  int nbu, nbw;
+  struct Col {
+    __amdgpu_buffer_rsrc_t r;
+    int base;  // header of the column's brick at depth brick 0
+    int o;     // the cell's corner word at w = 0 of the brick
+    unsigned row0, row1;  // the cell's fixed-point bilinear weights
+    int hb;    // depth brick of the cached header
+    float4 h;  // its header, steps divided by 65535 (the weights' sum)
+  };
+  __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
+    Col c;
+    c.r = make_rsrc(p + (size_t)node * nstride, (unsigned)nstride);
+    c.base = (int)__umul24(__umul24((unsigned)(iv >> 3), (unsigned)nbu) + (unsigned)(iu >> 3),
+                           (unsigned)(nbw * kQBrickBytes));
+    c.o = c.base + 16 + ((iv & 7) * 8 + (iu & 7)) * 128;
+    cell_weights(a, b, &c.row0, &c.row1);
+    c.hb = -1;
+    return c;
+  }
+  typedef TabU16X::Raw Raw;
+  // header of the column's brick sb bytes below depth brick 0, as reduce takes it
+  __device__ static inline float4 header(const Col &c, int sb) {
+    float4 h = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.base, sb, 0));
+    h.y *= 1.0f / 65535.0f;
+    h.w *= 1.0f / 65535.0f;
+    return h;
+  }
+  __device__ inline Raw fetch(Col &c, int w) const {
+    const int bw = w >> 3;
+    const int sb = bw * kQBrickBytes;
+    if (bw != c.hb) {
+      c.h = header(c, sb);
+      c.hb = bw;
+    }
+    const uint4 q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.o, sb + (w & 7) * 16, 0));
+    return Raw{q.x, q.y, q.z, q.w, c.h};
+  }
+  // Each row's two t codes (low halves) and two t* codes (high halves) are
+  // paired by byte permutes and weighted by one 16-bit dot product each;
+  // the weighted code sum decodes as off + (step / 65535) * sum.
+  __device__ inline float2 reduce(const Col &c, const Raw &q) const {
+    const unsigned t0 = __builtin_amdgcn_perm(q.q01, q.q00, 0x05040100u);
+    const unsigned t1 = __builtin_amdgcn_perm(q.q11, q.q10, 0x05040100u);
+    const unsigned s0 = __builtin_amdgcn_perm(q.q01, q.q00, 0x07060302u);
+    const unsigned s1 = __builtin_amdgcn_perm(q.q11, q.q10, 0x07060302u);
+    const float t = (float)dot2_u16(t1, c.row1, dot2_u16(t0, c.row0, 0u));
+    const float s = (float)dot2_u16(s1, c.row1, dot2_u16(s0, c.row0, 0u));
+    return make_float2(fmaf(q.h.y, t, q.h.x), fmaf(q.h.w, s, q.h.z));
+  }
+  __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
+};
+
 inline void u16x_bricks(const qpsdm_tgeom &g, int *nbv, int *nbu, int *nbw) {
   *nbv = (g.nv - 2) / 8 + 1;
   *nbu = (g.nu - 2) / 8 + 1;
@@ -339,6 +427,24 @@ inline size_t u16x_node_bytes(const qpsdm_tgeom &g) {
   int nbv, nbu, nbw;
   u16x_bricks(g, &nbv, &nbu, &nbw);
   return (size_t)nbv * nbu * nbw * kXBrickBytes;
+}
+
+inline size_t u16q_node_bytes(const qpsdm_tgeom &g) {
+  int nbv, nbu, nbw;
+  u16x_bricks(g, &nbv, &nbu, &nbw);
+  return (size_t)nbv * nbu * nbw * kQBrickBytes;
+}
+
+// U16Q nodes are addressed with 32-bit byte offsets.
+inline bool u16q_fits(const qpsdm_tgeom &g) { return u16q_node_bytes(g) < (1ull << 31); }
+
+inline TabU16Q make_tab_u16q(const qpsdm_tgeom &g, const void *p) {
+  TabU16Q t;
+  int nbv;
+  u16x_bricks(g, &nbv, &t.nbu, &t.nbw);
+  t.p = (const unsigned char *)p;
+  t.nstride = u16q_node_bytes(g);
+  return t;
 }
 
 inline TabU16X make_tab_u16x(const qpsdm_tgeom &g, const void *p) {

@@ -321,7 +321,7 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
   QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16 || p->bank_dtype == QPSDM_F16,
                 "ctx_create: bank dtype");
   QPSDM_REQUIRE(p->table_dtype == QPSDM_F32 || p->table_dtype == QPSDM_F16B || p->table_dtype == QPSDM_U16B ||
-                    p->table_dtype == QPSDM_U16X,
+                    p->table_dtype == QPSDM_U16X || p->table_dtype == QPSDM_U16Q,
                 "ctx_create: table dtype");
   qpsdm_ctx *c = new qpsdm_ctx();
   c->p = *p;
@@ -549,8 +549,8 @@ extern "C" int qpsdm_ctx_set_gathers(qpsdm_ctx *c, const qpsdm_gather_params *g,
   if (ncol) {
     int rc = qpsdm_gather_bins(g, &nbx, &nby);
     if (rc) return rc;
-    QPSDM_REQUIRE(c->p.table_dtype == QPSDM_F32 || c->p.table_dtype == QPSDM_U16X,
-                  "ctx_set_gathers: gathers need QPSDM_F32 or QPSDM_U16X tables");
+    QPSDM_REQUIRE(c->p.table_dtype == QPSDM_F32 || c->p.table_dtype == QPSDM_U16X || c->p.table_dtype == QPSDM_U16Q,
+                  "ctx_set_gathers: gathers need QPSDM_F32, QPSDM_U16X or QPSDM_U16Q tables");
     for (int i = 0; i < ncol; ++i)
       QPSDM_REQUIRE(cols[2 * i] >= 0 && cols[2 * i] < c->p.image.nx && cols[2 * i + 1] >= 0 &&
                         cols[2 * i + 1] < c->p.image.ny,

@@ -87,7 +87,9 @@ __global__ void blend_kernel(const float *a, const float *b, size_t n, float c1,
 
 // One work-group per edge-sharing brick: min/max of t and t* over its
 // 9 x 9 x 8 nodes (indices clamped to the table), offset = min, step =
-// range / 65535, code = round((v - off) / step).
+// range / 65535, code = round((v - off) / step).  QUAD: the same header and
+// codes written as the U16Q corner words of the brick's 8 x 8 x 8 cells.
+template <bool QUAD>
 __global__ __launch_bounds__(256) void quant_x_kernel(qpsdm_tgeom g, int nbv, int nbu, int nbw, const float2 *tab,
                                                       unsigned char *out) {
   constexpr int N = qpsdm_dev::kXCodes;
@@ -123,13 +125,25 @@ __global__ __launch_bounds__(256) void quant_x_kernel(qpsdm_tgeom g, int nbv, in
   }
   const float t0 = red[0][0], s0 = red[2][0];
   const float tst = fmaxf(red[1][0] - t0, 1e-30f) / 65535.0f, sst = fmaxf(red[3][0] - s0, 1e-30f) / 65535.0f;
-  unsigned char *o = out + b * qpsdm_dev::kXBrickBytes;
-  if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+  auto code = [&](int k) {
     const float2 v = at(k);
     const unsigned ct = min(__float2uint_rn(fmaxf((v.x - t0) / tst, 0.0f)), 65535u);
     const unsigned cs = min(__float2uint_rn(fmaxf((v.y - s0) / sst, 0.0f)), 65535u);
-    *(unsigned *)(o + 16 + k * 4) = ct | (cs << 16);
+    return ct | (cs << 16);
+  };
+  if (!QUAD) {
+    unsigned char *o = out + b * qpsdm_dev::kXBrickBytes;
+    if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) *(unsigned *)(o + 16 + k * 4) = code(k);
+  } else {
+    unsigned char *o = out + b * qpsdm_dev::kQBrickBytes;
+    if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
+    // cell (lv, lu) at depth lw: corners k = ((lv [+1]) * 9 + lu [+1]) * 8 + lw of the 9 x 9 x 8 nodes
+    for (int c = threadIdx.x; c < 8 * 8 * 8; c += blockDim.x) {
+      const int lw = c & 7, lu = (c >> 3) & 7, lv = c >> 6;
+      const int k00 = (lv * 9 + lu) * 8 + lw;
+      *(uint4 *)(o + 16 + c * 16) = make_uint4(code(k00), code(k00 + 8), code(k00 + 72), code(k00 + 80));
+    }
   }
 }
 
@@ -171,6 +185,7 @@ extern "C" size_t qpsdm_tables_bytes(const qpsdm_tgeom *g, int dtype) {
     case QPSDM_F16B:
     case QPSDM_U16B: return qpsdm_dev::b16_node_bytes(*g) * g->nnx * g->nny;
     case QPSDM_U16X: return qpsdm_dev::u16x_node_bytes(*g) * g->nnx * g->nny;
+    case QPSDM_U16Q: return qpsdm_dev::u16q_node_bytes(*g) * g->nnx * g->nny;
     default: return 0;
   }
 }
@@ -183,15 +198,19 @@ extern "C" int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab, voi
   const size_t nb = (size_t)((g->nv + 7) / 8) * ((g->nu + 7) / 8) * ((g->nw + 7) / 8) * g->nnx *
                     g->nny;
   QPSDM_REQUIRE(nb < (1ull << 31), "quantize: too many bricks for one launch");
-  QPSDM_REQUIRE(code == QPSDM_F16B || code == QPSDM_U16B || code == QPSDM_U16X,
-                "quantize: code must be F16B, U16B or U16X");
-  if (code == QPSDM_U16X) {
+  QPSDM_REQUIRE(code == QPSDM_F16B || code == QPSDM_U16B || code == QPSDM_U16X || code == QPSDM_U16Q,
+                "quantize: code must be F16B, U16B, U16X or U16Q");
+  if (code == QPSDM_U16X || code == QPSDM_U16Q) {
     int nbv, nbu, nbw;
     qpsdm_dev::u16x_bricks(*g, &nbv, &nbu, &nbw);
     const size_t nx = (size_t)nbv * nbu * nbw * g->nnx * g->nny;
     QPSDM_REQUIRE(nx < (1ull << 31), "quantize: too many bricks for one launch");
-    hipLaunchKernelGGL(quant_x_kernel, dim3((unsigned)nx), dim3(256), 0, (hipStream_t)stream, *g, nbv, nbu, nbw,
-                       (const float2 *)tab, (unsigned char *)out);
+    if (code == QPSDM_U16X)
+      hipLaunchKernelGGL(quant_x_kernel<false>, dim3((unsigned)nx), dim3(256), 0, (hipStream_t)stream, *g, nbv,
+                         nbu, nbw, (const float2 *)tab, (unsigned char *)out);
+    else
+      hipLaunchKernelGGL(quant_x_kernel<true>, dim3((unsigned)nx), dim3(256), 0, (hipStream_t)stream, *g, nbv,
+                         nbu, nbw, (const float2 *)tab, (unsigned char *)out);
   } else if (code == QPSDM_U16B)
     hipLaunchKernelGGL(quant_kernel<true>, dim3((unsigned)nb), dim3(512), 0, (hipStream_t)stream, *g,
                        (const float2 *)tab, (unsigned char *)out);

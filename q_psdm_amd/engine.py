@@ -108,9 +108,10 @@ class Survey:
 
     def quantize_tables(self, tab, code="f16", stream=None):
         """16-bit brick tables; code "f16" (FP16 codes in 8^3 bricks), "u16"
-        (fixed point in 8^3 bricks) or "u16x" (fixed point in edge-sharing
-        9x9x8 bricks).  Returned as a uint8 tensor tagged with .qpsdm_code."""
-        c = {"f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B, "u16x": L.QPSDM_U16X}[code]
+        (fixed point in 8^3 bricks), "u16x" (fixed point in edge-sharing
+        9x9x8 bricks) or "u16q" (the u16x codes stored as per-cell corner
+        words).  Returned as a uint8 tensor tagged with .qpsdm_code."""
+        c = {"f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B, "u16x": L.QPSDM_U16X, "u16q": L.QPSDM_U16Q}[code]
         nbytes = L.lib().qpsdm_tables_bytes(C.byref(self.tg), c)
         out = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         check(L.lib().qpsdm_tables_quantize(C.byref(self.tg), _ptr(tab), _ptr(out), c, _stream(stream)))
@@ -402,7 +403,7 @@ class Pipeline:
         p.geom = geom_params
         p.bank_dtype = {"f32": L.QPSDM_F32, "bf16": L.QPSDM_BF16, "f16": L.QPSDM_F16}[bank_dtype]
         p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B,
-                         "u16x": L.QPSDM_U16X}[table_dtype]
+                         "u16x": L.QPSDM_U16X, "u16q": L.QPSDM_U16Q}[table_dtype]
         p.batch_traces = batch_traces
         if launch is not None:
             p.shape = L.Launch(*launch)

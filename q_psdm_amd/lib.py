@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqpsdm.so")
 
-QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X, QPSDM_F16 = 0, 1, 2, 3, 4, 5
+QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X, QPSDM_F16, QPSDM_U16Q = 0, 1, 2, 3, 4, 5, 6
 
 
 class Window(C.Structure):

@@ -102,7 +102,7 @@ def test_batch_matches_fp64_oracle_and_count(gpu):
 
 
 @pytest.mark.parametrize("bank_dtype,code", [(torch.bfloat16, "u16"), (torch.float16, "u16x"),
-                                             (torch.bfloat16, "u16x")])
+                                             (torch.bfloat16, "u16x"), (torch.float16, "u16q")])
 def test_batch_mixed_precision_matches_per_shot(gpu, bank_dtype, code):
     cfg = synth.tiny()
     sv, tab = _survey(cfg)

@@ -89,9 +89,13 @@ def test_gathers_mixed_storage_and_accumulation(gpu):
     sv.gathers_shot(ox, None, cols_d, g_d, 0, n, b32, tab, gpx)
     sv.gathers_shot(None, oy, cols_d, g_d, 0, n, b32, tab, gpy)
     torch.cuda.synchronize()
+    qq = sv.quantize_tables(tab, code="u16q")
+    qx_, qy_ = sv.new_gathers(len(cols), nbx), sv.new_gathers(len(cols), nby)
+    sv.gathers_shot(qx_, qy_, cols_d, g_d, 0, n, b16, qq, gp)
     e = rel_l2(mx.cpu().numpy(), gx.cpu().numpy()), rel_l2(my.cpu().numpy(), gy.cpu().numpy())
-    print(f"mixed storage vs FP32 gathers: {e[0]:.2e} / {e[1]:.2e}")
-    assert max(e) < 1e-3
+    eq = rel_l2(qx_.cpu().numpy(), gx.cpu().numpy()), rel_l2(qy_.cpu().numpy(), gy.cpu().numpy())
+    print(f"mixed storage vs FP32 gathers: {e[0]:.2e} / {e[1]:.2e} (U16Q tables {eq[0]:.2e} / {eq[1]:.2e})")
+    assert max(e) < 1e-3 and max(eq) < 1e-3
     assert rel_l2(ax.cpu().numpy(), gx.cpu().numpy()) < 1e-6 and rel_l2(ay.cpu().numpy(), gy.cpu().numpy()) < 1e-6
     assert torch.equal(ox, gx) and torch.equal(oy, gy)
     none = sv.gather_params(aspect="none")

@@ -178,7 +178,8 @@ def _dequant(sv, q, code="f16"):
 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
-@pytest.mark.parametrize("code,tol", [("f32", 2e-7), ("u16", 1.5e-6), ("u16x", 1.5e-6), ("f16", 4e-5)])
+@pytest.mark.parametrize("code,tol", [("f32", 2e-7), ("u16", 1.5e-6), ("u16x", 1.5e-6), ("u16q", 1.5e-6),
+                                      ("f16", 4e-5)])
 def test_table_lookup_matches_oracle(gpu, mk, code, tol):
     """The device lookup of each table storage type against the oracle's
     FP64 trilinear lookup of the FP32 tables, over whole image volumes from
@@ -259,16 +260,40 @@ def test_u16x_image_matches_f32_tables(gpu, mk):  # tables' reads are pinned by 
     bank = sv.qbank(torch.from_numpy(traces).cuda())
     g_d = torch.from_numpy(geom).cuda()
     qx = sv.quantize_tables(tab, code="u16x")
+    qq = sv.quantize_tables(tab, code="u16q")
     qb = sv.quantize_tables(tab, code="u16")
     imgs = {}
-    for name, t in (("f32", tab), ("u16x", qx), ("u16", qb)):
+    for name, t in (("f32", tab), ("u16x", qx), ("u16q", qq), ("u16", qb)):
         img = sv.new_image()
         for tr0, n in ranges:
             sv.image_shot(img, g_d, tr0, n, bank[tr0:tr0 + n], t)
         imgs[name] = img.cpu().numpy()
     ex, eb = rel_l2(imgs["u16x"], imgs["f32"]), rel_l2(imgs["u16"], imgs["f32"])
-    print(f"16-bit tables vs FP32 tables (diffractions): u16x {ex:.2e}, u16 {eb:.2e}")
-    assert ex < 1e-3 and eb < 1e-3
+    eq, eqx = rel_l2(imgs["u16q"], imgs["f32"]), rel_l2(imgs["u16q"], imgs["u16x"])
+    print(f"16-bit tables vs FP32 tables (diffractions): u16x {ex:.2e}, u16q {eq:.2e}, u16 {eb:.2e}; "
+          f"u16q vs u16x {eqx:.2e}")
+    assert ex < 1e-3 and eb < 1e-3 and eq < 1e-3
+    assert eqx < 1e-3  # same codes; only the bilinear weights differ (16-bit fixed point)
+
+
+@pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
+def test_u16q_holds_the_u16x_codes(gpu, mk):
+    """The corner-quad layout stores each U16X brick's header and codes: every
+    cell's word at every depth node holds the codes of its four corners, read
+    back from the U16X brick that owns the cell."""
+    cfg = mk()
+    sv, tab = _survey(cfg)
+    g = cfg.tgeom
+    nv, nu, nw = g["nv"], g["nu"], g["nw"]
+    nbv, nbu, nbw = (nv - 2) // 8 + 1, (nu - 2) // 8 + 1, (nw + 7) // 8
+    nn = g["nnx"] * g["nny"]
+    x = sv.quantize_tables(tab, code="u16x").cpu().numpy().reshape(nn, nbv, nbu, nbw, 16 + 648 * 4)
+    q = sv.quantize_tables(tab, code="u16q").cpu().numpy().reshape(nn, nbv, nbu, nbw, 16 + 512 * 16)
+    assert np.array_equal(x[..., :16], q[..., :16])
+    xc = x[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 9, 9, 8)
+    qc = q[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 8, 8, 8, 4)
+    for k, (dv, du) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
+        assert np.array_equal(qc[..., k], xc[..., dv:dv + 8, du:du + 8, :]), k
 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])

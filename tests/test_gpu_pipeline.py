@@ -102,3 +102,8 @@ def test_pipeline_mixed_dtypes_and_table_builder(gpu):
     pl.push_su(engine.su_encode(traces, wxy))
     assert rel_l2(pl.image(), ref) < 5e-3
     pl.close()
+    pl = engine.Pipeline(sv, _gp(cfg), bank_dtype="f16", table_dtype="u16q", batch_traces=8)
+    pl.build_tables()
+    pl.push_su(engine.su_encode(traces, wxy))
+    assert rel_l2(pl.image(), ref) < 5e-3
+    pl.close()
