@@ -248,7 +248,7 @@ int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch *shape, flo
  * (atan(Y / Z) in y), k in [0, 2 ndip]; updates beyond the outer bins are not
  * binned.  aspect: 1 both axes (the reference's "All"), 2 x only, 3 y only,
  * 4 none; an axis not asked for may be NULL.  Gathers accumulate (+=) as the
- * image does.  Tables QPSDM_F32 or QPSDM_U16X; any bank dtype. */
+ * image does.  Tables QPSDM_F32, QPSDM_U16X or QPSDM_U16Q; any bank dtype. */
 typedef struct {
   float dipx, dipy;   /* largest dip binned (degrees)  */
   float ddipx, ddipy; /* bin width (degrees)            */
@@ -393,7 +393,7 @@ typedef struct {
   qpsdm_bank_params bank;    /* windows and Q bank (nt, K, dtstar as image)   */
   qpsdm_geom_params geom;    /* SU coordinate scale, model extent, origin     */
   int bank_dtype;            /* QPSDM_F32, QPSDM_BF16 or QPSDM_F16            */
-  int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B or QPSDM_U16X */
+  int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X or QPSDM_U16Q */
   int batch_traces;          /* traces per staging slot                       */
   qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */
 } qpsdm_ctx_params;
@@ -429,7 +429,7 @@ int qpsdm_ctx_reset_image(qpsdm_ctx *c);
  * later batch: the reference's opcrp/cdps/aspect/dipx/dipy/ddipx/ddipy
  * options (src/q-psdm.cu:651-702, 820-835) and gthrxd/gthryd buffers
  * (src/q-psdm.cu:211-212).  Replaces an earlier set; ncol = 0 stops
- * gathering.  Needs QPSDM_F32 or QPSDM_U16X tables. */
+ * gathering.  Needs QPSDM_F32, QPSDM_U16X or QPSDM_U16Q tables. */
 int qpsdm_ctx_set_gathers(qpsdm_ctx *c, const qpsdm_gather_params *g, const int *cols, int ncol);
 /* The gathers to host memory, gx [ncol][nbx][nz] and gy [ncol][nby][nz] (an
  * axis not gathered may be NULL), divided by nsb when normalize != 0. */
