@@ -33,6 +33,24 @@ def test_library_exports_every_declared_symbol():
     assert set(L.EXPORTS) >= declared
 
 
+def test_storage_type_codes_and_table_sizes():
+    """The header's storage-type codes are the binding's, and the table
+    sizes follow the layouts: FP32 float2 per node, U16X 9x9x8-node bricks of
+    16 + 648 x 4 bytes, U16Q the same bricks as 8x8x8 cells of one 16-byte
+    corner word each (16 + 512 x 16 bytes)."""
+    text = open(os.path.join(INC, "qpsdm.h")).read()
+    defs = dict((k, int(v)) for k, v in re.findall(r"#define (QPSDM_(?:F32|BF16|F16B|U16B|U16X|F16|U16Q)) (\d+)", text))
+    assert defs == {k: getattr(L, k) for k in defs} and len(defs) == 7
+    for nv, nu, nw in ((321, 321, 121), (41, 41, 17), (10, 2, 2), (11, 9, 9)):
+        tg = L.TGeom(2, 3, 0.0, 0.0, 1.0, 1.0, nu, nv, nw, 0.0, 0.0, 1.0, 1.0, 1.0)
+        nbv, nbu, nbw = (nv - 2) // 8 + 1, (nu - 2) // 8 + 1, (nw + 7) // 8
+        size = lambda c: L.lib().qpsdm_tables_bytes(C.byref(tg), c)
+        assert size(L.QPSDM_F32) == 6 * nv * nu * nw * 8
+        assert size(L.QPSDM_U16X) == 6 * nbv * nbu * nbw * (16 + 648 * 4)
+        assert size(L.QPSDM_U16Q) == 6 * nbv * nbu * nbw * (16 + 512 * 16)
+    assert L.lib().qpsdm_tables_bytes(C.byref(tg), 99) == 0
+
+
 def test_npfar_window_gain_iaztab_match_oracle():
     for n in range(1, 5000, 13):
         assert engine.npfar(n) == O.lib().or_npfar(n)
