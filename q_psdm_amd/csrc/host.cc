@@ -177,11 +177,13 @@ extern "C" int qpsdm_read_tidx(const char *path, qpsdm_tidx_header *h, qpsdm_ttt
   h->nzi0 = raw[6];
   h->nxi0 = raw[7];
   int rc = QPSDM_OK;
-  if (d) {
-    const long n = (long)h->nxg * h->nzg;
-    if (n > maxdesc) {
-      rc = qpsdm::set_error(QPSDM_ERANGE, "index has %ld descriptors, room for %d", n, maxdesc);
-    } else if ((long)fread(d, sizeof(qpsdm_ttt_desc), n, fp) != n) {
+  if (h->nxg < 0 || h->nzg < 0 || h->nzi0 < 0 || h->nxi0 < 0) {
+    rc = qpsdm::set_error(QPSDM_EIO, "negative grid size in index file %s", path);
+  } else if (d) {
+    const size_t n = (size_t)h->nxg * (size_t)h->nzg;
+    if (maxdesc < 0 || n > (size_t)maxdesc) {
+      rc = qpsdm::set_error(QPSDM_ERANGE, "index has %zu descriptors, room for %d", n, maxdesc);
+    } else if (fread(d, sizeof(qpsdm_ttt_desc), n, fp) != n) {
       rc = qpsdm::set_error(QPSDM_EIO, "travel time table or qfield index size error %s", path);
     }
   }
@@ -192,6 +194,7 @@ extern "C" int qpsdm_read_tidx(const char *path, qpsdm_tidx_header *h, qpsdm_ttt
 extern "C" int qpsdm_write_tidx(const char *path, const qpsdm_tidx_header *h,
                                 const qpsdm_ttt_desc *d) {
   QPSDM_REQUIRE(path && h && d, "write_tidx: bad args");
+  QPSDM_REQUIRE(h->nxg >= 0 && h->nzg >= 0, "write_tidx: negative grid size");
   FILE *fp = fopen(path, "wb");
   if (!fp) return qpsdm::set_error(QPSDM_EIO, "cannot create %s", path);
   int32_t raw[8] = {h->cdp1, h->cdp2, 0, 0, h->nzg, h->nxg, h->nzi0, h->nxi0};
@@ -212,6 +215,7 @@ static long file_size(const char *p) {
 static long read_floats(const char *path, float *out, size_t maxfloat) {
   const long sz = file_size(path);
   if (sz < 0) return qpsdm::set_error(QPSDM_EIO, "missing %s", path);
+  if (sz % 4 != 0) return qpsdm::set_error(QPSDM_EIO, "%s: size %ld is not a whole number of floats", path, sz);
   if ((size_t)sz / 4 > maxfloat) return qpsdm::set_error(QPSDM_ERANGE, "%s larger than buffer", path);
   FILE *fp = fopen(path, "rb");
   if (!fp) return qpsdm::set_error(QPSDM_EIO, "cannot open %s", path);

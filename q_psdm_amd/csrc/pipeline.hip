@@ -379,31 +379,47 @@ extern "C" int qpsdm_ctx_destroy(qpsdm_ctx *c) {
 }
 
 // Tables in the FP32 layout (host or device); quantised on the device when
-// the context stores 16-bit bricks.
+// the context stores 16-bit bricks.  The new set is complete before the old
+// one is released: on any error the context keeps its previous tables (or
+// none) and nothing allocated here is left behind.
 extern "C" int qpsdm_ctx_set_tables(qpsdm_ctx *c, const float *tab, int on_device) {
   QPSDM_REQUIRE(c && tab, "ctx_set_tables: null args");
   const qpsdm_tgeom &g = c->p.image.tg;
   const size_t n32 = (size_t)g.nnx * g.nny * g.nv * g.nu * g.nw * 8;
   float *t32 = nullptr;
-  QPSDM_HIP(hipMalloc(&t32, n32));
-  QPSDM_HIP(hipMemcpyAsync(t32, tab, n32, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                           c->compute));
-  if (c->tab) {
-    QPSDM_HIP(hipStreamSynchronize(c->compute));
-    QPSDM_HIP(hipFree(c->tab));
-    c->tab = nullptr;
+  void *q = nullptr;
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(c->compute);
+    if (t32) (void)hipFree(t32);
+    if (q) (void)hipFree(q);
+    return rc;
+  };
+  if (hipMalloc(&t32, n32) != hipSuccess) {
+    t32 = nullptr;
+    return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_set_tables: hipMalloc of %zu bytes failed", n32));
   }
-  if (c->p.table_dtype == QPSDM_F32) {
-    c->tab = t32;
-  } else {
-    void *q = nullptr;
-    QPSDM_HIP(hipMalloc(&q, qpsdm_tables_bytes(&g, c->p.table_dtype)));
-    int rc = qpsdm_tables_quantize(&g, t32, q, c->p.table_dtype, c->compute);
-    QPSDM_HIP(hipStreamSynchronize(c->compute));
-    QPSDM_HIP(hipFree(t32));
-    if (rc) return rc;
-    c->tab = q;
+  if (hipMemcpyAsync(t32, tab, n32, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->compute) !=
+      hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_set_tables: table copy failed"));
+  void *fresh = t32;
+  if (c->p.table_dtype != QPSDM_F32) {
+    const size_t nq = qpsdm_tables_bytes(&g, c->p.table_dtype);
+    if (hipMalloc(&q, nq) != hipSuccess) {
+      q = nullptr;
+      return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_set_tables: hipMalloc of %zu bytes failed", nq));
+    }
+    const int rc = qpsdm_tables_quantize(&g, t32, q, c->p.table_dtype, c->compute);
+    if (rc) return fail(rc);
+    if (hipStreamSynchronize(c->compute) != hipSuccess)
+      return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_set_tables: quantize failed"));
+    (void)hipFree(t32);
+    t32 = nullptr;
+    fresh = q;
   }
+  if (hipStreamSynchronize(c->compute) != hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_set_tables: table copy failed"));
+  if (c->tab) (void)hipFree(c->tab);
+  c->tab = fresh;
   c->have_tab = true;
   return QPSDM_OK;
 }
