@@ -210,13 +210,25 @@ typedef struct {
   float taperzone;                  /* radians, src/q-psdm.cu:1287                */
   float tmin;                       /* floor on each leg's traveltime (s)         */
   qpsdm_tgeom tg;
+  /* Form of the sum.  0: the exact form -- every image point evaluates its
+   * own dip taper and amplitude weight w (tr + 1/tr).  1: the node form --
+   * below z_exact (and above the tables' last depth node) the four taper
+   * ramps and the amplitude are evaluated at the tables' depth nodes and
+   * interpolated linearly to the points between them, the ramps clamped per
+   * point; times and t* levels are exact in both forms (both legs are linear
+   * in z between the nodes).  The node form's difference from the exact
+   * form is an approximation error, not rounding: stated against the FP64
+   * oracle where it is used. */
+  int weight_nodes;
+  float z_exact;                    /* node form: depth (m) imaged exactly above  */
 } qpsdm_image_params;
 
 /* Launch shape picked by the autotuner; a NULL shape or 0 fields take the
- * default 8 x 8 columns x 6 depth points. */
+ * default 8 x 8 columns x 6 depth points.  In the node form tile_z counts
+ * table depth intervals per tile (2, 3 or 4); the exact body's share of
+ * the node form runs 8 depth points per tile. */
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
-  int trace_chunk;            /* reserved: traces per tile work item (0 = all)         */
 } qpsdm_launch;
 
 /* Table lookup as the imaging kernel does it (nearest node of the surface

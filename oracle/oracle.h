@@ -98,10 +98,31 @@ void or_ray_times(const or_model *m, double sx, double sy, double x, double y,
 void or_build_tables(const or_model *m, const or_tgeom *g, float *tab);
 void or_lookup(const or_tgeom *g, const float *tab, double px, double py,
                double x, double y, double z, double *t, double *tstar);
+/* Per-shot source table, in the layout of the library's per-shot source
+ * tables: hdr = {sx, sy, su0, sv0}; tab = (t, t*) [nvs][nus][nw][2] on the
+ * node tables' (du, dv, dw) spacing from the relative origin (su0, sv0).
+ * The source leg of a trace is read at (x - sx - su0, y - sy - sv0, z) with
+ * (sx, sy) from hdr; a trace whose source differs from hdr (bitwise) is not
+ * imaged. */
+typedef struct {
+  const float *hdr;
+  const float *tab;
+  int nus, nvs;
+} or_src;
+
 int64_t or_image_f64(const double *bank, const float *geom, int ntr,
                      const float *tab, const or_iparams *p, double *img);
 int64_t or_image_f32(const float *bank, const float *geom, int ntr,
                      const float *tab, const or_iparams *p, float *img);
+int64_t or_image_src_f64(const double *bank, const float *geom, int ntr, const float *tab,
+                         const or_src *src, const or_iparams *p, double *img);
+int64_t or_image_src_f32(const float *bank, const float *geom, int ntr, const float *tab,
+                         const or_src *src, const or_iparams *p, float *img);
+/* Node form (the library's weight_nodes = 1) of the FP32 path; the points
+ * [ps, pe) it interpolates from depth nodes come from or_node_range. */
+void or_node_range(const or_iparams *p, float z_exact, int *ps, int *pe);
+int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                           const or_iparams *p, float z_exact, float *img);
 int64_t or_count(const float *geom, int ntr, const or_iparams *p);
 int or_num_threads(void);
 void or_set_num_threads(int n);

@@ -49,6 +49,11 @@ class BParams(C.Structure):
                 ("dtstar", C.c_float), ("fc", C.c_float), ("threshold", C.c_float)]
 
 
+class Src(C.Structure):
+    _fields_ = [("hdr", C.POINTER(C.c_float)), ("tab", C.POINTER(C.c_float)), ("nus", C.c_int),
+                ("nvs", C.c_int)]
+
+
 _lib = None
 _ref = None
 
@@ -83,6 +88,13 @@ def lib():
         L.or_image_f64.argtypes = [dp, fp, C.c_int, fp, C.POINTER(IParams), dp]
         L.or_image_f32.restype = C.c_int64
         L.or_image_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(IParams), fp]
+        L.or_image_src_f64.restype = C.c_int64
+        L.or_image_src_f64.argtypes = [dp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), dp]
+        L.or_image_src_f32.restype = C.c_int64
+        L.or_image_src_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), fp]
+        L.or_image_nodes_f32.restype = C.c_int64
+        L.or_image_nodes_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, fp]
+        L.or_node_range.argtypes = [C.POINTER(IParams), C.c_float, ip, ip]
         L.or_count.restype = C.c_int64
         L.or_count.argtypes = [fp, C.c_int, C.POINTER(IParams)]
         L.or_num_threads.restype = C.c_int
@@ -241,19 +253,59 @@ def lookup(g, tab, px, py, x, y, z):
     return t.value, ts.value
 
 
-def image(bank, geom, tab, ip, fp32=False):
-    """Kirchhoff sum; returns (image[ny,nx,nz], update count)."""
+def image(bank, geom, tab, ip, fp32=False, src=None):
+    """Kirchhoff sum; returns (image[ny,nx,nz], update count).  src: the
+    traces' per-shot source table as (hdr[4] = (sx, sy, su0, sv0), table
+    [nvs][nus][nw][2]); traces whose source is not hdr's are not imaged."""
     geom = np.ascontiguousarray(geom, np.float32)
     tab = np.ascontiguousarray(tab, np.float32)
     ntr = geom.shape[0]
+    sp = None
+    if src is not None:
+        hdr = np.ascontiguousarray(src[0], np.float32).reshape(4)
+        st = np.ascontiguousarray(src[1], np.float32)
+        assert st.ndim == 4 and st.shape[2] == ip.tg.nw and st.shape[3] == 2
+        sp = Src(_f(hdr), _f(st), st.shape[1], st.shape[0])
+        keep = (hdr, st)
     if fp32:
         bank = np.ascontiguousarray(bank, np.float32)
         img = np.zeros((ip.ny, ip.nx, ip.nz), np.float32)
-        n = lib().or_image_f32(_f(bank), _f(geom), ntr, _f(tab), C.byref(ip), _f(img))
+        if sp is None:
+            n = lib().or_image_f32(_f(bank), _f(geom), ntr, _f(tab), C.byref(ip), _f(img))
+        else:
+            n = lib().or_image_src_f32(_f(bank), _f(geom), ntr, _f(tab), C.byref(sp), C.byref(ip), _f(img))
     else:
         bank = np.ascontiguousarray(bank, np.float64)
         img = np.zeros((ip.ny, ip.nx, ip.nz), np.float64)
-        n = lib().or_image_f64(_d(bank), _f(geom), ntr, _f(tab), C.byref(ip), _d(img))
+        if sp is None:
+            n = lib().or_image_f64(_d(bank), _f(geom), ntr, _f(tab), C.byref(ip), _d(img))
+        else:
+            n = lib().or_image_src_f64(_d(bank), _f(geom), ntr, _f(tab), C.byref(sp), C.byref(ip), _d(img))
+    if sp is not None:
+        del keep
+    return img, n
+
+
+def node_range(ip, z_exact):
+    """Points [ps, pe) the node form interpolates from depth nodes."""
+    a, b = C.c_int(), C.c_int()
+    lib().or_node_range(C.byref(ip), z_exact, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def image_nodes(bank, geom, tab, ip, z_exact, src=None):
+    """Node form of the FP32 path (weight_nodes = 1): (image, count)."""
+    geom = np.ascontiguousarray(geom, np.float32)
+    tab = np.ascontiguousarray(tab, np.float32)
+    bank = np.ascontiguousarray(bank, np.float32)
+    img = np.zeros((ip.ny, ip.nx, ip.nz), np.float32)
+    sp = None
+    if src is not None:
+        hdr = np.ascontiguousarray(src[0], np.float32).reshape(4)
+        st = np.ascontiguousarray(src[1], np.float32)
+        sp = Src(_f(hdr), _f(st), st.shape[1], st.shape[0])
+    n = lib().or_image_nodes_f32(_f(bank), _f(geom), geom.shape[0], _f(tab), C.byref(sp) if sp else None,
+                                 C.byref(ip), z_exact, _f(img))
     return img, n
 
 

@@ -453,26 +453,44 @@ static void frac_index(double q, int n, int *i0, double *a) {
 /* Nearest-node, shifted-table lookup: (t, t*) from position (px, py, 0) to
  * (x, y, z) = trilinear interpolation in the nearest node's table at the
  * relative position (x - px, y - py, z). */
-void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double x,
-               double y, double z, double *t, double *tstar) {
-  int in = nearest_node(px, g->n0x, g->ndx, g->nnx);
-  int jn = nearest_node(py, g->n0y, g->ndy, g->nny);
-  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+/* Trilinear (t, t*) of one relative-coordinate grid T [nv][nu][nw][2]
+ * (origin (u0, v0), spacing (du, dv, dw)) at the relative position
+ * (rx, ry, z), indices clamped to the grid. */
+static void lookup_grid(const float *T, int nu, int nv, int nw, double u0, double v0, double du,
+                        double dv, double dw, double rx, double ry, double z, double *t,
+                        double *tstar) {
   int iu, iv, iw;
   double a, b, c;
-  frac_index((x - px - g->u0) / g->du, g->nu, &iu, &a);
-  frac_index((y - py - g->v0) / g->dv, g->nv, &iv, &b);
-  frac_index(z / g->dw, g->nw, &iw, &c);
+  frac_index((rx - u0) / du, nu, &iu, &a);
+  frac_index((ry - v0) / dv, nv, &iv, &b);
+  frac_index(z / dw, nw, &iw, &c);
   double r[2] = {0, 0};
   for (int k = 0; k < 8; ++k) {
-    int du = k & 1, dv = (k >> 1) & 1, dz = (k >> 2) & 1;
-    double wgt = (du ? a : 1 - a) * (dv ? b : 1 - b) * (dz ? c : 1 - c);
-    size_t o = ((size_t)(iv + dv) * g->nu + (iu + du)) * g->nw + (iw + dz);
+    int ku = k & 1, kv = (k >> 1) & 1, kz = (k >> 2) & 1;
+    double wgt = (ku ? a : 1 - a) * (kv ? b : 1 - b) * (kz ? c : 1 - c);
+    size_t o = ((size_t)(iv + kv) * nu + (iu + ku)) * nw + (iw + kz);
     r[0] += wgt * T[2 * o];
     r[1] += wgt * T[2 * o + 1];
   }
   *t = r[0];
   *tstar = r[1];
+}
+
+/* Nearest-node, shifted-table lookup: (t, t*) from position (px, py, 0) to
+ * (x, y, z) = trilinear interpolation in the nearest node's table at the
+ * relative position (x - px, y - py, z). */
+void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double x,
+               double y, double z, double *t, double *tstar) {
+  int in = nearest_node(px, g->n0x, g->ndx, g->nnx);
+  int jn = nearest_node(py, g->n0y, g->ndy, g->nny);
+  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+  lookup_grid(T, g->nu, g->nv, g->nw, g->u0, g->v0, g->du, g->dv, g->dw, x - px, y - py, z, t, tstar);
+}
+
+/* A trace belongs to a per-shot source table when its source is the
+ * table's own, bit for bit (the table was built for that one position). */
+static int same_source(const float *geom_row, const or_src *src) {
+  return memcmp(geom_row, src->hdr, 2 * sizeof(float)) == 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -501,6 +519,7 @@ void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double
       REAL sx = geom[4 * itr], sy = geom[4 * itr + 1];                                     \
       REAL gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];                                 \
       if (sx != sx) continue; /* rejected by the extent check */                           \
+      if (src && !same_source(geom + 4 * itr, src)) continue;                              \
       const BANK *b = bank + (size_t)itr * p->K * p->nt;                                   \
       REAL y = (REAL)iy * p->dy;                                                           \
       for (int ix = 0; ix < p->nx; ++ix) {                                                 \
@@ -520,7 +539,11 @@ void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double
           if (tany < ty1) wt *= (tany - ty1c) / (ty1 - ty1c);                              \
           if (tany > ty2) wt *= (ty2c - tany) / (ty2c - ty2);                              \
           double a1, a2, b1, b2;                                                           \
-          or_lookup(g, tab, sx, sy, x, y, z, &a1, &a2);                                    \
+          if (src)                                                                         \
+            lookup_grid(src->tab, src->nus, src->nvs, g->nw, src->hdr[2], src->hdr[3], g->du,  \
+                        g->dv, g->dw, x - sx, y - sy, z, &a1, &a2);                          \
+          else                                                                             \
+            or_lookup(g, tab, sx, sy, x, y, z, &a1, &a2);                                  \
           or_lookup(g, tab, gx, gy, x, y, z, &b1, &b2);                                    \
           REAL ts = (REAL)a1, tg = (REAL)b1;                                               \
           if (ts < p->tmin) ts = p->tmin;                                                  \
@@ -545,10 +568,15 @@ void or_lookup(const or_tgeom *g, const float *tab, double px, double py, double
   }                                                                                        \
   return count;
 
-int64_t or_image_f64(const double *bank, const float *geom, int ntr, const float *tab,
-                     const or_iparams *p, double *img) {
+int64_t or_image_src_f64(const double *bank, const float *geom, int ntr, const float *tab,
+                         const or_src *src, const or_iparams *p, double *img) {
   typedef double BANK;
   IMAGE_BODY(double, sqrt)
+}
+
+int64_t or_image_f64(const double *bank, const float *geom, int ntr, const float *tab,
+                     const or_iparams *p, double *img) {
+  return or_image_src_f64(bank, geom, ntr, tab, NULL, p, img);
 }
 
 
@@ -557,27 +585,21 @@ int64_t or_image_f64(const double *bank, const float *geom, int ntr, const float
  * GPU's order of operations -- reciprocal spacings, bilinear (x, y) table
  * interpolation at two depth nodes then a lerp in z, 1/sqrt distances,
  * reciprocal weights. */
-static inline void lookup_f32(const or_tgeom *g, const float *tab, float px, float py, float x,
-                              float y, float z, float *t, float *ts) {
-  const float inx = 1.0f / g->ndx, iny = 1.0f / g->ndy, idu = 1.0f / g->du, idv = 1.0f / g->dv,
-              idw = 1.0f / g->dw;
-  int in = (int)floorf((px - g->n0x) * inx + 0.5f);
-  int jn = (int)floorf((py - g->n0y) * iny + 0.5f);
-  in = in < 0 ? 0 : (in > g->nnx - 1 ? g->nnx - 1 : in);
-  jn = jn < 0 ? 0 : (jn > g->nny - 1 ? g->nny - 1 : jn);
-  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
-  float q = fminf(fmaxf((x - px - g->u0) * idu, 0.0f), (float)(g->nu - 1));
-  int iu = (int)q < g->nu - 2 ? (int)q : g->nu - 2;
+static inline void lookup_f32_grid(const float *T, int nu, int nv, int nw, float u0, float v0, float idu,
+                                   float idv, float idw, float px, float py, float x, float y, float z,
+                                   float *t, float *ts) {
+  float q = fminf(fmaxf((x - px - u0) * idu, 0.0f), (float)(nu - 1));
+  int iu = (int)q < nu - 2 ? (int)q : nu - 2;
   const float a = q - (float)iu;
-  q = fminf(fmaxf((y - py - g->v0) * idv, 0.0f), (float)(g->nv - 1));
-  int iv = (int)q < g->nv - 2 ? (int)q : g->nv - 2;
+  q = fminf(fmaxf((y - py - v0) * idv, 0.0f), (float)(nv - 1));
+  int iv = (int)q < nv - 2 ? (int)q : nv - 2;
   const float b = q - (float)iv;
-  const float fw = fminf(z * idw, (float)(g->nw - 1));
-  int w = (int)fw < g->nw - 2 ? (int)fw : g->nw - 2;
+  const float fw = fminf(z * idw, (float)(nw - 1));
+  int w = (int)fw < nw - 2 ? (int)fw : nw - 2;
   const float c = fw - (float)w;
   float col[2][2];
   for (int k = 0; k < 2; ++k) {
-#define TV(dv_, du_, comp_) T[2 * (((size_t)(iv + dv_) * g->nu + (iu + du_)) * g->nw + (w + k)) + comp_]
+#define TV(dv_, du_, comp_) T[2 * (((size_t)(iv + dv_) * nu + (iu + du_)) * nw + (w + k)) + comp_]
     for (int comp = 0; comp < 2; ++comp) {
       const float lx = fmaf(a, TV(0, 1, comp) - TV(0, 0, comp), TV(0, 0, comp));
       const float hx = fmaf(a, TV(1, 1, comp) - TV(1, 0, comp), TV(1, 0, comp));
@@ -589,10 +611,47 @@ static inline void lookup_f32(const or_tgeom *g, const float *tab, float px, flo
   *ts = fmaf(c, col[1][1] - col[0][1], col[0][1]);
 }
 
+static inline void lookup_f32(const or_tgeom *g, const float *tab, float px, float py, float x,
+                              float y, float z, float *t, float *ts) {
+  const float inx = 1.0f / g->ndx, iny = 1.0f / g->ndy;
+  int in = (int)floorf((px - g->n0x) * inx + 0.5f);
+  int jn = (int)floorf((py - g->n0y) * iny + 0.5f);
+  in = in < 0 ? 0 : (in > g->nnx - 1 ? g->nnx - 1 : in);
+  jn = jn < 0 ? 0 : (jn > g->nny - 1 ? g->nny - 1 : jn);
+  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+  lookup_f32_grid(T, g->nu, g->nv, g->nw, g->u0, g->v0, 1.0f / g->du, 1.0f / g->dv, 1.0f / g->dw, px, py, x, y,
+                  z, t, ts);
+}
+
+/* Source leg of the FP32 path: the per-shot source table when given (read
+ * at the table's own source position), else the nearest node's table. */
+static inline void source_f32(const or_tgeom *g, const float *tab, const or_src *src, float sx, float sy,
+                              float x, float y, float z, float *t, float *ts) {
+  if (src)
+    lookup_f32_grid(src->tab, src->nus, src->nvs, g->nw, src->hdr[2], src->hdr[3], 1.0f / g->du, 1.0f / g->dv,
+                    1.0f / g->dw, sx, sy, x, y, z, t, ts);
+  else
+    lookup_f32(g, tab, sx, sy, x, y, z, t, ts);
+}
+
 static inline float sat01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
 
 int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float *tab,
                      const or_iparams *p, float *img) {
+  return or_image_src_f32(bank, geom, ntr, tab, NULL, p, img);
+}
+
+static int64_t image_f32_impl(const float *bank, const float *geom, int ntr, const float *tab,
+                              const or_src *src, const or_iparams *p, int skip_lo, int skip_hi, float *img);
+
+int64_t or_image_src_f32(const float *bank, const float *geom, int ntr, const float *tab,
+                         const or_src *src, const or_iparams *p, float *img) {
+  return image_f32_impl(bank, geom, ntr, tab, src, p, 0, 0, img);
+}
+
+/* FP32 path over the points outside [skip_lo, skip_hi). */
+static int64_t image_f32_impl(const float *bank, const float *geom, int ntr, const float *tab,
+                              const or_src *src, const or_iparams *p, int skip_lo, int skip_hi, float *img) {
   int64_t count = 0;
   const or_tgeom *g = &p->tg;
   const float tz = p->taperzone;
@@ -609,6 +668,7 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
       const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
       const float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
       if (sx != sx) continue;
+      if (src && !same_source(geom + 4 * itr, src)) continue;
       const float *bt = bank + (size_t)itr * p->K * p->nt;
       const float y = (float)iy * p->dy;
       for (int ix = 0; ix < p->nx; ++ix) {
@@ -616,10 +676,11 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
         const float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
         const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
         for (int iz = 1; iz < p->nz; ++iz) {
+          if (iz >= skip_lo && iz < skip_hi) continue;
           const float z = (float)iz * p->dz;
           /* source leg: t in samples, t* in levels + 1/2, bisector terms, ds^3/t */
           float ts, tss;
-          lookup_f32(g, tab, sx, sy, x, y, z, &ts, &tss);
+          source_f32(g, tab, src, sx, sy, x, y, z, &ts, &tss);
           ts = fmaxf(ts, p->tmin);
           const float d2s = hs2 + z * z;
           const float rds = 1.0f / sqrtf(d2s), ds = d2s * rds;
@@ -658,6 +719,183 @@ int64_t or_image_f32(const float *bank, const float *geom, int ntr, const float 
 
 /* Update count alone: pairs (trace, point) with z > 0 inside the widened
  * dip bounds, evaluated in FP32 with the FP32 path's order of operations. */
+/* ---------------------------------------------------------------------- */
+/* Node form of the FP32 path (the library's weight_nodes = 1): below      */
+/* z_exact and within the tables' depth range, the four dip-taper ramps,   */
+/* the amplitude (tr + 1/tr), the time in samples and the level are        */
+/* evaluated at the tables' depth nodes and interpolated linearly to the   */
+/* points between, the ramps clamped per point; elsewhere the exact FP32   */
+/* path.  The FP32 restatement of the device's node body, in its order of  */
+/* operations (this build's own method: the reference's kernel evaluates   */
+/* every point, src/q-psdm.cu:2303-2378).                                   */
+/* ---------------------------------------------------------------------- */
+static int nf_interval(int iz, float dz, float inv_dw, int nw, float *frac) {
+  const float z = (float)iz * dz;
+  float fw = z * inv_dw;
+  if (fw > (float)(nw - 1)) fw = (float)(nw - 1);
+  int w = (int)fw;
+  if (w > nw - 2) w = nw - 2;
+  *frac = fw - (float)w;
+  return w;
+}
+
+static int nf_first_point(int W, int nz, float dz, float inv_dw, int nw) {
+  float f;
+  int iz = (int)((float)W / (dz * inv_dw));
+  iz = iz < 0 ? 0 : (iz > nz ? nz : iz);
+  while (iz < nz && nf_interval(iz, dz, inv_dw, nw, &f) < W) ++iz;
+  while (iz > 0 && nf_interval(iz - 1, dz, inv_dw, nw, &f) >= W) --iz;
+  return iz;
+}
+
+void or_node_range(const or_iparams *p, float z_exact, int *ps_out, int *pe_out) {
+  const int nz = p->nz, nw = p->tg.nw;
+  const float dz = p->dz, inv_dw = 1.0f / p->tg.dw;
+  int ws = (int)ceil((double)z_exact / (double)p->tg.dw);
+  if (ws < 1) ws = 1;
+  int pe = nz;
+  for (int iz = 0; iz < nz; ++iz) {
+    const float z = (float)iz * dz;
+    if (z * inv_dw > (float)(nw - 1)) {
+      pe = iz;
+      break;
+    }
+  }
+  int ps = pe;
+  if (ws <= nw - 2) {
+    ps = nf_first_point(ws, nz, dz, inv_dw, nw);
+    if (ps > pe) ps = pe;
+  }
+  *ps_out = ps;
+  *pe_out = pe;
+}
+
+/* Bilinear (t, t*) at depth node w of grid T (no depth interpolation). */
+static void lookup_f32_node(const float *T, int nu, int nv, int nw, float u0, float v0, float idu, float idv,
+                            float px, float py, float x, float y, int w, float *t, float *ts) {
+  float q = fminf(fmaxf((x - px - u0) * idu, 0.0f), (float)(nu - 1));
+  int iu = (int)q < nu - 2 ? (int)q : nu - 2;
+  const float a = q - (float)iu;
+  q = fminf(fmaxf((y - py - v0) * idv, 0.0f), (float)(nv - 1));
+  int iv = (int)q < nv - 2 ? (int)q : nv - 2;
+  const float b = q - (float)iv;
+  float r[2];
+#define TV(dv_, du_, comp_) T[2 * (((size_t)(iv + dv_) * nu + (iu + du_)) * nw + w) + comp_]
+  for (int comp = 0; comp < 2; ++comp) {
+    const float lx = fmaf(a, TV(0, 1, comp) - TV(0, 0, comp), TV(0, 0, comp));
+    const float hx = fmaf(a, TV(1, 1, comp) - TV(1, 0, comp), TV(1, 0, comp));
+    r[comp] = fmaf(b, hx - lx, lx);
+  }
+#undef TV
+  *t = r[0];
+  *ts = r[1];
+}
+
+static void node_table_f32(const or_tgeom *g, const float *tab, const or_src *src, int is_source, float px,
+                           float py, float x, float y, int w, float *t, float *ts) {
+  if (is_source && src) {
+    lookup_f32_node(src->tab, src->nus, src->nvs, g->nw, src->hdr[2], src->hdr[3], 1.0f / g->du, 1.0f / g->dv, px,
+                    py, x, y, w, t, ts);
+    return;
+  }
+  int in = (int)floorf((px - g->n0x) * (1.0f / g->ndx) + 0.5f);
+  int jn = (int)floorf((py - g->n0y) * (1.0f / g->ndy) + 0.5f);
+  in = in < 0 ? 0 : (in > g->nnx - 1 ? g->nnx - 1 : in);
+  jn = jn < 0 ? 0 : (jn > g->nny - 1 ? g->nny - 1 : jn);
+  const float *T = tab + 2 * (size_t)(jn * g->nnx + in) * g->nv * g->nu * g->nw;
+  lookup_f32_node(T, g->nu, g->nv, g->nw, g->u0, g->v0, 1.0f / g->du, 1.0f / g->dv, px, py, x, y, w, t, ts);
+}
+
+typedef struct {
+  float F, L, u[4], a;
+} nf_node;
+
+int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                           const or_iparams *p, float z_exact, float *img) {
+  int ps, pe;
+  or_node_range(p, z_exact, &ps, &pe);
+  int64_t count = image_f32_impl(bank, geom, ntr, tab, src, p, ps, pe, img);
+  if (ps >= pe) return count;
+  const or_tgeom *g = &p->tg;
+  const float tz = p->taperzone;
+  const float tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;
+  const float tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);
+  const float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
+  const float rx1 = 1.0f / (tx1 - tx1c), rx2 = 1.0f / (tx2c - tx2);
+  const float ry1 = 1.0f / (ty1 - ty1c), ry2 = 1.0f / (ty2c - ty2);
+  const float cx1 = -tx1c * rx1, cx2 = tx2c * rx2, cy1 = -ty1c * ry1, cy2 = ty2c * ry2;
+  const float inv_dt = 1.0f / p->dt, inv_dts = 1.0f / p->dtstar, inv_dw = 1.0f / g->dw;
+  const float jmax = (float)(p->K - 1);
+  int64_t ncount = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : ncount)
+  for (int iy = 0; iy < p->ny; ++iy) {
+    for (int itr = 0; itr < ntr; ++itr) {
+      const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
+      const float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
+      if (sx != sx) continue;
+      if (src && !same_source(geom + 4 * itr, src)) continue;
+      const float *bt = bank + (size_t)itr * p->K * p->nt;
+      const float y = (float)iy * p->dy;
+      for (int ix = 0; ix < p->nx; ++ix) {
+        const float x = (float)ix * p->dx;
+        const float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
+        const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
+        int wcur = -1;
+        nf_node nv[2];
+        for (int iz = ps; iz < pe; ++iz) {
+          float c;
+          const int w = nf_interval(iz, p->dz, inv_dw, g->nw, &c);
+          if (w != wcur) {
+            for (int k = 0; k < 2; ++k) {
+              const int wn = w + k;
+              const float zn = (float)wn * g->dw;
+              float ts, tss, tg, tsg;
+              node_table_f32(g, tab, src, 1, sx, sy, x, y, wn, &ts, &tss);
+              node_table_f32(g, tab, src, 0, gx, gy, x, y, wn, &tg, &tsg);
+              ts = fmaxf(ts, p->tmin);
+              const float d2 = fmaf(zn, zn, hs2);
+              const float ds = d2 * (1.0f / sqrtf(d2));
+              const float tsn = ts * inv_dt, tsl = fmaf(tss, inv_dts, 0.5f), its = 1.0f / ts;
+              const float rdg = 1.0f / sqrtf(fmaf(zn, zn, hg2));
+              const float q = ds * rdg;
+              const float X = fmaf(dgx, q, dsx), Y = fmaf(dgy, q, dsy), Z = fmaf(zn, q, zn);
+              const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
+              const float t = fmaxf(tg, p->tmin);
+              const float tr = (q * q) * q * (t * its);
+              nv[k].u[0] = fmaf(tanx, rx1, cx1);
+              nv[k].u[1] = fmaf(tanx, -rx2, cx2);
+              nv[k].u[2] = fmaf(tany, ry1, cy1);
+              nv[k].u[3] = fmaf(tany, -ry2, cy2);
+              nv[k].a = tr + 1.0f / tr;
+              nv[k].F = fmaf(t, inv_dt, tsn);
+              nv[k].L = fmaf(tsg, inv_dts, tsl);
+            }
+            wcur = w;
+          }
+          const nf_node *n0 = &nv[0], *n1 = &nv[1];
+          const float f = fmaf(c, n1->F - n0->F, n0->F);
+          const float wx = fminf(sat01(fmaf(c, n1->u[0] - n0->u[0], n0->u[0])),
+                                 sat01(fmaf(c, n1->u[1] - n0->u[1], n0->u[1])));
+          const float wy = fminf(sat01(fmaf(c, n1->u[2] - n0->u[2], n0->u[2])),
+                                 sat01(fmaf(c, n1->u[3] - n0->u[3], n0->u[3])));
+          const float wt = (wx * wy) * fmaf(c, n1->a - n0->a, n0->a);
+          const int i0 = (int)f;
+          float lv = fmaf(c, n1->L - n0->L, n0->L);
+          lv = lv < 0.0f ? 0.0f : (lv > jmax ? jmax : lv);
+          const int j = (int)lv;
+          if (!((unsigned)(i0 - 1) < (unsigned)(p->nt - 2))) continue;
+          if (wt > 0.0f) ++ncount;
+          const float fr = f - floorf(f);
+          const float *b = bt + (size_t)j * p->nt + i0;
+          float *o = &img[((size_t)iy * p->nx + ix) * p->nz + iz];
+          *o = fmaf(wt, fmaf(fr, b[1] - b[0], b[0]), *o);
+        }
+      }
+    }
+  }
+  return count + ncount;
+}
+
 int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
   int64_t count = 0;
   float tz = p->taperzone;

@@ -87,6 +87,6 @@ def _resident(sv, geom, bank, tab, shots, launch):
     import ctypes as C
     out = C.c_int(0)
     bcode, tcode = sv._codes(bank, tab)
-    lp = L.Launch(*launch)
+    lp = L.Launch(*launch[:3])
     L.check(L.lib().qpsdm_image_batch_grid(C.byref(sv.ip), C.byref(lp), bcode, tcode, C.byref(out)))
     return out.value

@@ -16,8 +16,10 @@
 
 #include <mutex>
 
+#include <type_traits>
+
 #include "common.h"
-#include "image_tile.h"
+#include "image_nodes.h"
 
 namespace qk = qpsdm_dev;
 
@@ -50,12 +52,14 @@ struct ShotRuns {
     run.brow0 = run.tr0;
     run.dip = qk::uniform_dip(qk::make_dip(sf[2], sf[3], sf[4], sf[5], tz));
     run.src = nullptr;
-    run.su0 = run.sv0 = 0.f;
+    run.ssx = run.ssy = run.su0 = run.sv0 = 0.f;
     run.snu = nus;
     run.snv = nvs;
     if (src_hdr) {
       const cfp h = (cfp)(src_hdr + r);
       run.src = src_tab + (size_t)r * src_stride;
+      run.ssx = h[0];
+      run.ssy = h[1];
       run.su0 = h[2];
       run.sv0 = h[3];
     }
@@ -125,6 +129,17 @@ __global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab
   qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
 }
 
+// Node form of one shot (weight_nodes = 1): one work-group per node tile
+// (blockIdx.x = column tile, blockIdx.y = node group).
+template <int NI, int PM, typename BankT, typename Tab>
+__global__ __launch_bounds__(64) void image_batch_nodes_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+                                                               const float4 *__restrict__ geom,
+                                                               const BankT *__restrict__ bank, ShotRuns runs,
+                                                               int shot) {
+  const OneShot one{runs(shot)};
+  qk::image_tile_nodes<NI, PM>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, one, 1);
+}
+
 template <int ZP>
 __global__ __launch_bounds__(64) void count_batch_kernel(qk::ImgDev P, const float4 *__restrict__ geom,
                                                          ShotRuns runs, int nshot,
@@ -153,7 +168,7 @@ template <int ZP, typename BankT, typename Tab>
 int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   auto kern = image_batch_kernel<ZP, BankT, Tab>;
   const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
-  const int nzg = (P.nz + ZP - 1) / ZP;
+  const int nzg = (P.ze - P.zb + ZP - 1) / ZP;
   if (go.nwg == -1) {
     if (go.nwg_out) *go.nwg_out = -1;
     if (!go.run) return QPSDM_OK;
@@ -184,8 +199,62 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
   return QPSDM_OK;
 }
 
+// Node form of every shot, in shot order: per shot the exact body (one
+// work-group per tile of 8 points) above the node range and beyond the
+// tables' depth range, then the node body (one work-group per node tile).
+template <int NI, int PM, typename BankT, typename Tab>
+int launch_batch_nodes_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
+  if (go.nwg_out) *go.nwg_out = -1;
+  if (!go.run) return QPSDM_OK;
+  const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
+  qk::ImgDev E = P;
+  E.tile_z = 8;
+  const int n1 = (P.nf_ps + 7) / 8, n2 = (P.nz - P.nf_pe + 7) / 8;
+  for (int r = 0; r < go.nshot; ++r) {
+    if (n1 > 0) {
+      E.zb = 0;
+      E.ze = P.nf_ps;
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab>), dim3(ncol * n1), dim3(64), 0, go.s, E, tab,
+                         go.img, go.geom, bank, go.runs, r, ncol);
+    }
+    if (n2 > 0) {
+      E.zb = P.nf_pe;
+      E.ze = P.nz;
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab>), dim3(ncol * n2), dim3(64), 0, go.s, E, tab,
+                         go.img, go.geom, bank, go.runs, r, ncol);
+    }
+    hipLaunchKernelGGL((image_batch_nodes_kernel<NI, PM, BankT, Tab>), dim3(ncol, P.nf_ngroups), dim3(64), 0, go.s,
+                       P, tab, go.img, go.geom, bank, go.runs, r);
+    QPSDM_HIP(hipGetLastError());
+  }
+  return QPSDM_OK;
+}
+
+template <typename BankT, typename Tab>
+int launch_batch_nodes(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
+  constexpr bool ok = (std::is_same<BankT, float>::value || std::is_same<BankT, __half>::value) &&
+                      (std::is_same<Tab, qk::TabF32>::value || std::is_same<Tab, qk::TabU16Q>::value);
+  if constexpr (!ok) {
+    return qpsdm::set_error(QPSDM_EINVAL,
+                            "image_batch: the node form takes FP32 or FP16 banks and FP32 or U16Q tables");
+  } else {
+    switch (P.tile_z * 4 + P.nf_pm) {
+      case 2 * 4 + 2: return launch_batch_nodes_t<2, 2>(P, tab, bank, go);
+      case 2 * 4 + 3: return launch_batch_nodes_t<2, 3>(P, tab, bank, go);
+      case 3 * 4 + 2: return launch_batch_nodes_t<3, 2>(P, tab, bank, go);
+      case 3 * 4 + 3: return launch_batch_nodes_t<3, 3>(P, tab, bank, go);
+      case 4 * 4 + 2: return launch_batch_nodes_t<4, 2>(P, tab, bank, go);
+      case 4 * 4 + 3: return launch_batch_nodes_t<4, 3>(P, tab, bank, go);
+      default:
+        return qpsdm::set_error(QPSDM_EINVAL, "image_batch: node form with %d intervals x %d points per tile",
+                                P.tile_z, P.nf_pm);
+    }
+  }
+}
+
 template <typename BankT, typename Tab>
 int launch_batch_tz(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
+  if (P.nf_ngroups > 0) return launch_batch_nodes(P, tab, bank, go);
   switch (P.tile_z) {
     case 2: return launch_batch_t<2>(P, tab, bank, go);
     case 3: return launch_batch_t<3>(P, tab, bank, go);
@@ -257,7 +326,7 @@ extern "C" int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch
 
 extern "C" int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom, const qpsdm_shot *shots,
                                  int nshot, unsigned long long *count, void *stream) {
-  qpsdm_launch sh = {8, 8, 8, 1};  // count kernel shape
+  qpsdm_launch sh = {8, 8, 8};  // count kernel shape
   qk::ImgDev P;
   int rc = qk::make_dev(p, &sh, 0, 0, &P);
   if (rc) return rc;

@@ -6,10 +6,58 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
 #include "common.h"
-#include "image_tile.h"
+#include "image_nodes.h"
 
 namespace qk = qpsdm_dev;
+
+namespace qpsdm_dev {
+
+// Node range of the node form (ImgDev nf_*), with the device's float rule
+// for a point's interval: intervals from the first node at or below z_exact
+// (node 1 at least: the surface node has no bisector) to the last interval
+// whose points lie within the tables' depth range; ni intervals per tile.
+int node_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
+  const int nz = p->nz, nw = p->tg.nw;
+  const float dz = p->dz, inv_dw = o->inv_dw;
+  QPSDM_REQUIRE(p->z_exact >= 0.0f, "image: z_exact must be >= 0");
+  QPSDM_REQUIRE(ni >= 2 && ni <= 4, "image: the node form takes tile_z = 2, 3 or 4 depth intervals per tile");
+  const int ws = std::max(1, (int)std::ceil((double)p->z_exact / (double)p->tg.dw));
+  int pe = nz;
+  for (int iz = 0; iz < nz; ++iz) {
+    const float z = (float)iz * dz;
+    if (z * inv_dw > (float)(nw - 1)) {
+      pe = iz;
+      break;
+    }
+  }
+  if (ws > nw - 2) return QPSDM_OK;
+  const int ps = first_point(ws, nz, dz, inv_dw, nw);
+  if (ps >= pe) return QPSDM_OK;
+  float f;
+  const int we = point_interval(pe - 1, dz, inv_dw, nw, &f) + 1;
+  int pm = 0;
+  for (int w = ws; w < we; ++w) {
+    const int a = std::max(first_point(w, nz, dz, inv_dw, nw), ps);
+    const int b = std::min(first_point(w + 1, nz, dz, inv_dw, nw), pe);
+    pm = std::max(pm, b - a);
+  }
+  QPSDM_REQUIRE(pm <= 3, "image: the node form takes at most 3 image points per table depth interval (dw <= 3 dz), "
+                         "found %d", pm);
+  o->nf_ws = ws;
+  o->nf_we = we;
+  o->nf_ps = ps;
+  o->nf_pe = pe;
+  o->nf_pm = std::max(pm, 2);
+  o->nf_ngroups = (we - ws + ni - 1) / ni;
+  return QPSDM_OK;
+}
+
+}  // namespace qpsdm_dev
 
 namespace {
 
@@ -26,7 +74,7 @@ inline OneRun one_run(const qk::ImgDev &P) {
   r.run.brow0 = 0;
   r.run.dip = P.dip;
   r.run.src = nullptr;
-  r.run.su0 = r.run.sv0 = 0.f;
+  r.run.ssx = r.run.ssy = r.run.su0 = r.run.sv0 = 0.f;
   r.run.snu = r.run.snv = 0;
   return r;
 }
@@ -37,6 +85,14 @@ __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float 
                                                    const float4 *__restrict__ geom,
                                                    const BankT *__restrict__ bank, OneRun run) {
   qk::image_tile<ZP>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
+}
+
+// Node form: blockIdx.x = column tile, blockIdx.y = node group.
+template <int NI, int PM, typename BankT, typename Tab>
+__global__ __launch_bounds__(64) void image_nodes_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
+                                                         const float4 *__restrict__ geom,
+                                                         const BankT *__restrict__ bank, OneRun run) {
+  qk::image_tile_nodes<NI, PM>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
 }
 
 template <int ZP>
@@ -58,7 +114,8 @@ template <int ZP, typename BankT, typename Tab>
 int launch_t(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geom, const BankT *bank,
              hipStream_t s) {
   const int nxt = (P.nx + P.tile_x - 1) / P.tile_x, nyt = (P.ny + P.tile_y - 1) / P.tile_y;
-  const dim3 grid(nxt * nyt, (P.nz + ZP - 1) / ZP);
+  if (P.ze <= P.zb) return QPSDM_OK;
+  const dim3 grid(nxt * nyt, (P.ze - P.zb + ZP - 1) / ZP);
   hipLaunchKernelGGL((image_kernel<ZP, BankT, Tab>), grid, dim3(64), 0, s, P, tab, img, geom, bank,
                      one_run(P));
   QPSDM_HIP(hipGetLastError());
@@ -82,13 +139,63 @@ int launch_tz(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geo
   }
 }
 
+template <int NI, int PM, typename BankT, typename Tab>
+int launch_nodes_t(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geom, const BankT *bank,
+                   hipStream_t s) {
+  const int nxt = (P.nx + P.tile_x - 1) / P.tile_x, nyt = (P.ny + P.tile_y - 1) / P.tile_y;
+  hipLaunchKernelGGL((image_nodes_kernel<NI, PM, BankT, Tab>), dim3(nxt * nyt, P.nf_ngroups), dim3(64), 0, s, P,
+                     tab, img, geom, bank, one_run(P));
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+// The node form's storage types: FP32 or FP16 banks, FP32 or U16Q tables.
+template <typename BankT, typename Tab>
+constexpr bool node_form_types() {
+  return (std::is_same<BankT, float>::value || std::is_same<BankT, __half>::value) &&
+         (std::is_same<Tab, qk::TabF32>::value || std::is_same<Tab, qk::TabU16Q>::value);
+}
+
+// One run in the form the parameters ask for: the exact body over the whole
+// depth range, or the node form -- the exact body (8 points per tile) above
+// the node range and beyond the tables' last node, the node body between.
+template <typename BankT, typename Tab>
+int launch_form(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geom, const BankT *bank,
+                hipStream_t s) {
+  if (P.nf_ngroups == 0) return launch_tz(P, tab, img, geom, bank, s);
+  if constexpr (!node_form_types<BankT, Tab>()) {
+    return qpsdm::set_error(QPSDM_EINVAL, "image: the node form takes FP32 or FP16 banks and FP32 or U16Q tables");
+  } else {
+    qk::ImgDev E = P;
+    E.tile_z = 8;
+    E.zb = 0;
+    E.ze = P.nf_ps;
+    int rc = launch_t<8>(E, tab, img, geom, bank, s);
+    E.zb = P.nf_pe;
+    E.ze = P.nz;
+    if (!rc) rc = launch_t<8>(E, tab, img, geom, bank, s);
+    if (rc) return rc;
+    switch (P.tile_z * 4 + P.nf_pm) {
+      case 2 * 4 + 2: return launch_nodes_t<2, 2>(P, tab, img, geom, bank, s);
+      case 2 * 4 + 3: return launch_nodes_t<2, 3>(P, tab, img, geom, bank, s);
+      case 3 * 4 + 2: return launch_nodes_t<3, 2>(P, tab, img, geom, bank, s);
+      case 3 * 4 + 3: return launch_nodes_t<3, 3>(P, tab, img, geom, bank, s);
+      case 4 * 4 + 2: return launch_nodes_t<4, 2>(P, tab, img, geom, bank, s);
+      case 4 * 4 + 3: return launch_nodes_t<4, 3>(P, tab, img, geom, bank, s);
+      default:
+        return qpsdm::set_error(QPSDM_EINVAL, "image: node form with %d intervals x %d points per tile", P.tile_z,
+                                P.nf_pm);
+    }
+  }
+}
+
 template <typename Tab>
 int launch_bank(const qk::ImgDev &P, const Tab &T, float *img, const float4 *g4, const void *bank, int bank_dtype,
                 hipStream_t s) {
   switch (bank_dtype) {
-    case QPSDM_F32: return launch_tz(P, T, img, g4, (const float *)bank, s);
-    case QPSDM_BF16: return launch_tz(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
-    case QPSDM_F16: return launch_tz(P, T, img, g4, (const __half *)bank, s);
+    case QPSDM_F32: return launch_form(P, T, img, g4, (const float *)bank, s);
+    case QPSDM_BF16: return launch_form(P, T, img, g4, (const __hip_bfloat16 *)bank, s);
+    case QPSDM_F16: return launch_form(P, T, img, g4, (const __half *)bank, s);
     default: return qpsdm::set_error(QPSDM_EINVAL, "image: bank dtype %d", bank_dtype);
   }
 }
@@ -165,7 +272,7 @@ extern "C" int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab,
 
 extern "C" int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom, int tr0, int ntr,
                                 unsigned long long *count, void *stream) {
-  qpsdm_launch sh = {8, 8, 8, 1};  // count kernel shape
+  qpsdm_launch sh = {8, 8, 8};  // count kernel shape
   qk::ImgDev P;
   int rc = qk::make_dev(p, &sh, tr0, ntr, &P);
   if (rc) return rc;

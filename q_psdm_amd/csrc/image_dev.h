@@ -85,7 +85,17 @@ struct ImgDev {
   float u0, v0, inv_du, inv_dv, inv_dw;
   int tile_x, tile_y, tile_z;
   int tr0, ntr;
+  float dw;
+  // the exact body's point range [zb, ze) (all of [0, nz) unless the node
+  // form splits it)
+  int zb, ze;
+  // node form (weight_nodes = 1): node intervals [nf_ws, nf_we) and their
+  // points [nf_ps, nf_pe), nf_ngroups tiles of tile_z intervals per column
+  // group, at most nf_pm points per interval; nf_ngroups = 0 without it
+  int nf_ws, nf_we, nf_ps, nf_pe, nf_ngroups, nf_pm;
 };
+
+int node_range(const qpsdm_image_params *p, int ni, ImgDev *o);
 
 inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0, int ntr,
                     ImgDev *o) {
@@ -143,6 +153,14 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->tile_z = tzp;
   o->tr0 = tr0;
   o->ntr = ntr;
+  o->dw = g.dw;
+  o->zb = 0;
+  o->ze = p->nz;
+  o->nf_ws = o->nf_we = o->nf_ngroups = o->nf_pm = 0;
+  o->nf_ps = o->nf_pe = p->nz;
+  QPSDM_REQUIRE(p->weight_nodes == 0 || p->weight_nodes == 1,
+                "image: weight_nodes must be 0 (exact form) or 1 (node form)");
+  if (p->weight_nodes) return node_range(p, tzp, o);
   return QPSDM_OK;
 }
 

@@ -25,16 +25,24 @@ namespace qpsdm_dev {
 
 // A run of traces [tr0, tr0 + ntr) of geom; bank row of trace t is
 // t - tr0 + brow0; dip bounds of the run.  When src is set, the run's
-// source leg comes from its own table (one node at the source, relative
-// origin (su0, sv0), snv x snu x nw nodes at the node tables' spacing)
-// instead of the nearest node's table.
+// source leg comes from its own table (built for the source (ssx, ssy),
+// relative origin (su0, sv0), snv x snu x nw nodes at the node tables'
+// spacing) instead of the nearest node's table, and only the run's traces
+// whose source is (ssx, ssy), bit for bit, are imaged.
 struct Run {
   int tr0, ntr, brow0;
   Dip dip;
   const float2 *src;
-  float su0, sv0;
+  float ssx, ssy, su0, sv0;
   int snu, snv;
 };
+
+// A trace of a source-table run whose source is not the table's own
+// (compared bitwise, on the scalar unit).
+__device__ inline bool foreign_source(const Run &run, const float4 &g) {
+  return run.src && ((__float_as_int(g.x) != __float_as_int(run.ssx)) |
+                     (__float_as_int(g.y) != __float_as_int(run.ssy)));
+}
 
 // Source-leg cache of the tile points, per source position: t in samples,
 // t* in bank levels plus the rounding half, the distance ds and ds^3 / t
@@ -114,8 +122,8 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
   const int ntx = (P.nx + P.tile_x - 1) / P.tile_x;
   const int tx0 = (tile_col % ntx) * P.tile_x;
   const int ty0 = (tile_col / ntx) * P.tile_y;
-  const int z0 = zgroup * ZP;
-  const int nzv = min(ZP, P.nz - z0);
+  const int z0 = P.zb + zgroup * ZP;
+  const int nzv = min(ZP, P.ze - z0);
   const int cx = tx0 + lane % P.tile_x, cy = ty0 + lane / P.tile_x;
   const bool valid = cx < P.nx && cy < P.ny;
   const float x = (float)cx * P.dx, y = (float)cy * P.dy;
@@ -174,6 +182,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
     };
     float4 g;
     for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
+      if (foreign_source(run, g)) continue;
       set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;

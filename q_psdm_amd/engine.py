@@ -95,6 +95,20 @@ class Survey:
         self.ip = ip
         self.tg = ip.tg
 
+    def set_form(self, form="exact", z_exact=400.0):
+        """Form of the imaging sum: "exact" (every point evaluates its taper
+        and amplitude) or "nodes" (below z_exact metres, the taper ramps and
+        the amplitude are evaluated at the tables' depth nodes and
+        interpolated; times and levels stay exact).  Returns self."""
+        assert form in ("exact", "nodes")
+        self.ip.weight_nodes = 1 if form == "nodes" else 0
+        self.ip.z_exact = float(z_exact)
+        return self
+
+    @property
+    def form(self):
+        return "nodes" if self.ip.weight_nodes else "exact"
+
     # -- tables ----------------------------------------------------------
     def table_shape(self):
         g = self.cfg.tgeom
@@ -185,7 +199,7 @@ class Survey:
         bcode, tcode = self._codes(bank, tab)
         lp = None
         if launch is not None:
-            lp = L.Launch(*launch)
+            lp = L.Launch(*launch[:3])
         check(L.lib().qpsdm_image_shot(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img),
                                        _ptr(geom), tr0, ntr, _ptr(bank), bcode, _ptr(tab), tcode,
                                        _stream(stream)))
@@ -253,7 +267,7 @@ class Survey:
         bcode, tcode = self._codes(bank, tab)
         if work is None:
             work = torch.empty(1, dtype=torch.int32, device="cuda")
-        lp = L.Launch(*launch) if launch is not None else None
+        lp = L.Launch(*launch[:3]) if launch is not None else None
         if src is not None:
             assert src.numel() >= L.lib().qpsdm_surrogate_src_bytes(C.byref(self.ip), shots.n)
         check(L.lib().qpsdm_image_batch_src(C.byref(self.ip), C.byref(lp) if lp else None, _ptr(img), _ptr(geom),
@@ -311,7 +325,7 @@ class BatchGraph:
             src = surrogate.new_src(survey, shots.n)
         self.src = src
         self._keep = (traces, bank, img, geom, tab, shots, surrogate, tab_f32, mask, src)
-        lp = L.Launch(*launch) if launch is not None else None
+        lp = L.Launch(*launch[:3]) if launch is not None else None
         h = C.c_void_p()
         opt = lambda t: _ptr(t) if t is not None else None
         torch.cuda.synchronize()
@@ -406,7 +420,7 @@ class Pipeline:
                          "u16x": L.QPSDM_U16X, "u16q": L.QPSDM_U16Q}[table_dtype]
         p.batch_traces = batch_traces
         if launch is not None:
-            p.shape = L.Launch(*launch)
+            p.shape = L.Launch(*launch[:3])
         self.survey = survey
         self.params = p
         self.ctx = C.c_void_p()

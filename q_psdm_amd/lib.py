@@ -62,12 +62,12 @@ class ImageParams(C.Structure):
                 ("dy", C.c_float), ("dz", C.c_float), ("nt", C.c_int), ("dt", C.c_float),
                 ("K", C.c_int), ("dtstar", C.c_float), ("tanx1", C.c_float),
                 ("tanx2", C.c_float), ("tany1", C.c_float), ("tany2", C.c_float),
-                ("taperzone", C.c_float), ("tmin", C.c_float), ("tg", TGeom)]
+                ("taperzone", C.c_float), ("tmin", C.c_float), ("tg", TGeom),
+                ("weight_nodes", C.c_int), ("z_exact", C.c_float)]
 
 
 class Launch(C.Structure):
-    _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
-                ("trace_chunk", C.c_int)]
+    _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int)]
 
 
 class GatherParams(C.Structure):

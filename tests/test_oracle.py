@@ -11,6 +11,7 @@ from scipy import integrate
 
 from oracle import oracle as O
 from q_psdm_amd import synth
+from tests.helpers import rel_l2
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 VEC = json.load(open(os.path.join(GOLD, "reference_vectors.json")))
@@ -376,3 +377,125 @@ def test_point_scatterer_focuses():
     iy, ix, iz = np.unravel_index(np.argmax(env), env.shape)
     assert abs(ix * cfg.dx - 120.0) <= cfg.dx and abs(iy * cfg.dy - 90.0) <= cfg.dy
     assert abs(iz * cfg.dz - 190.0) <= 2 * cfg.dz
+
+
+# ------------------------------------------------------------ node form
+def _deep_case():
+    """A 1 km deep v(z) case: 8 x 6 columns x 100 depths at 10 m, 20 m tables."""
+    cfg = synth.tiny()
+    cfg.nx, cfg.ny, cfg.nz = 8, 6, 100
+    cfg.nt = 1201
+    cfg.tgeom = dict(cfg.tgeom, nw=52)
+    tab, ip = _tiny_case(cfg)
+    return cfg, tab, ip
+
+
+def test_node_form_with_everything_exact_is_the_fp32_path():
+    """z_exact below the image: the node form images every point exactly,
+    bit for bit the FP32 path."""
+    cfg = synth.tiny()
+    tab, ip = _tiny_case(cfg)
+    geom, _ = synth.geometry(synth.shots_tiny(2, 5))
+    bank = np.random.default_rng(3).standard_normal((geom.shape[0], cfg.K, cfg.nt)).astype(np.float32)
+    a, _ = O.image(bank, geom, tab, ip, fp32=True)
+    b, _ = O.image_nodes(bank, geom, tab, ip, 1e9)
+    assert O.node_range(ip, 1e9) == (cfg.nz, cfg.nz)
+    assert np.array_equal(a, b)
+
+
+def test_node_range_follows_the_interval_rule():
+    """[ps, pe): the first point whose table interval reaches z_exact, and
+    the first point below the tables' last node, by the float rule
+    fw = min(z / dw, nw - 1) the kernels use."""
+    cfg, tab, ip = _deep_case()
+    dw = np.float32(ip.tg.dw)
+    inv = np.float32(1.0) / dw
+    for ze in (0.0, 100.0, 333.0, 980.0):
+        ps, pe = O.node_range(ip, ze)
+        ws = max(1, int(np.ceil(ze / float(dw))))
+        fw = (np.arange(cfg.nz, dtype=np.float32) * np.float32(cfg.dz)) * inv
+        w = np.minimum(np.minimum(fw, np.float32(ip.tg.nw - 1)).astype(np.int64), ip.tg.nw - 2)
+        beyond = np.nonzero(fw > ip.tg.nw - 1)[0]
+        assert pe == (beyond[0] if beyond.size else cfg.nz)
+        inside = np.nonzero(w[:pe] >= ws)[0]
+        assert ps == (inside[0] if inside.size else pe)
+
+
+def test_node_form_is_exact_at_the_nodes_and_converges_with_depth():
+    """At points on a table depth node (interpolation fraction 0) the node
+    form evaluates the exact formula, so it agrees with the FP32 path to
+    rounding; between nodes its error against the FP64 sum falls as the
+    nodes get relatively closer (the ramps and the amplitude are smooth on
+    the scale z)."""
+    cfg, tab, ip = _deep_case()
+    rng = np.random.default_rng(5)
+    geom, _ = synth.geometry(synth.shots_tiny(2, 6, extent=(300.0, 300.0)))
+    bank = rng.standard_normal((geom.shape[0], cfg.K, cfg.nt)).astype(np.float32)
+    e32, _ = O.image(bank, geom, tab, ip, fp32=True)
+    e64, _ = O.image(bank.astype(np.float64), geom, tab, ip)
+    nimg, _ = O.image_nodes(bank, geom, tab, ip, 0.0)
+    ps, pe = O.node_range(ip, 0.0)
+    at_node = [iz for iz in range(ps, pe) if (iz * cfg.dz) % ip.tg.dw == 0]
+    assert len(at_node) > 10
+    assert rel_l2(nimg[..., at_node], e32[..., at_node]) < 2e-6
+    errs = []
+    for ze in (0.0, 100.0, 300.0, 600.0):
+        img, _ = O.image_nodes(bank, geom, tab, ip, ze)
+        ps, pe = O.node_range(ip, ze)
+        errs.append(rel_l2(img[..., ps:pe], e64[..., ps:pe]))
+        floor = rel_l2(e32[..., ps:pe], e64[..., ps:pe])
+    # ~4e-2, 3e-3, 2e-4, then the FP32 path's own rounding (~4e-5)
+    assert errs[0] > 10 * errs[1] > 100 * errs[2]
+    assert errs[0] > 1e-2 and errs[3] < 1.5 * floor
+
+
+# ------------------------------------------------------ per-shot source tables
+def _cut_source_table(tab, g, sx, sy, nus, nvs):
+    """A per-shot source table holding the nearest node's own values on the
+    sub-grid that starts at the node table's relative node floor((-sx -
+    u0)/du) (the layout of the library's per-shot source tables)."""
+    inode = int(np.clip(np.floor((sx - g.n0x) / g.ndx + 0.5), 0, g.nnx - 1))
+    jnode = int(np.clip(np.floor((sy - g.n0y) / g.ndy + 0.5), 0, g.nny - 1))
+    iu0 = int(np.clip(np.floor((-sx - g.u0) / g.du), 0, g.nu - nus))
+    iv0 = int(np.clip(np.floor((-sy - g.v0) / g.dv), 0, g.nv - nvs))
+    T = tab[jnode * g.nnx + inode]
+    hdr = np.array([sx, sy, g.u0 + iu0 * g.du, g.v0 + iv0 * g.dv], np.float32)
+    return hdr, np.ascontiguousarray(T[iv0:iv0 + nvs, iu0:iu0 + nus])
+
+
+def test_source_table_cut_from_the_node_table_reproduces_the_node_image():
+    """A shot's source table that holds its nearest node's values at the same
+    relative nodes reads the same (t, t*) as the node table: same image (to
+    rounding of the shifted origin); a trace of another source is skipped."""
+    cfg = synth.tiny_layers()
+    g = O.tgeom(**cfg.tgeom)
+    m = O.model_layers(cfg.model["vl"], cfg.model["ql"], cfg.model["layer_dz"],
+                       (cfg.model["ax"], cfg.model["ay"], cfg.model["az"], cfg.model["asig"], cfg.model["av"],
+                        cfg.model["aq"]))
+    tab = O.build_tables(m, g)
+    ip = O.IParams()
+    ip.nx, ip.ny, ip.nz, ip.dx, ip.dy, ip.dz = cfg.nx, cfg.ny, cfg.nz, cfg.dx, cfg.dy, cfg.dz
+    ip.nt, ip.dt, ip.K = cfg.nt, cfg.dt, cfg.K
+    ip.dtstar = 2 * float(tab[..., 1].max()) / (cfg.K - 1)
+    ip.tanx1, ip.tanx2, ip.tany1, ip.tany2, ip.taperzone = cfg.dip_bounds()
+    ip.tmin = cfg.tmin
+    ip.tg = g
+    shots = synth.shots_tiny(1, 6, extent=(190.0, 150.0))
+    geom, _ = synth.geometry(shots)
+    sx, sy = float(geom[0, 0]), float(geom[0, 1])
+    nus = min(g.nu, int(np.ceil((cfg.nx - 1) * cfg.dx / g.du)) + 2)
+    nvs = min(g.nv, int(np.ceil((cfg.ny - 1) * cfg.dy / g.dv)) + 2)
+    src = _cut_source_table(tab, g, sx, sy, nus, nvs)
+    bank = np.random.default_rng(8).standard_normal((geom.shape[0], cfg.K, cfg.nt))
+    for fp32 in (False, True):
+        b = bank.astype(np.float32) if fp32 else bank
+        a, na = O.image(b, geom, tab, ip, fp32=fp32)
+        s, ns = O.image(b, geom, tab, ip, fp32=fp32, src=src)
+        assert na == ns and rel_l2(s, a) < (2e-5 if fp32 else 1e-12)
+    # a trace whose source is not the table's is not imaged
+    other = geom.copy()
+    other[2, :2] += np.float32(5.0)
+    keep = np.array([i for i in range(geom.shape[0]) if i != 2])
+    s1, _ = O.image(bank, other, tab, ip, src=src)
+    s2, _ = O.image(bank[keep], geom[keep], tab, ip, src=src)
+    assert np.array_equal(s1, s2)
