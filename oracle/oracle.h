@@ -123,6 +123,9 @@ int64_t or_image_src_f32(const float *bank, const float *geom, int ntr, const fl
 void or_node_range(const or_iparams *p, float z_exact, int *ps, int *pe);
 int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
                            const or_iparams *p, float z_exact, float *img);
+/* Test-only: the FP64 sum's level rule (0 round, the product's; 1 floor;
+ * 2 always level 0), for planting a wrong pick in known-answer tests. */
+void or_set_level_rule(int rule);
 int64_t or_count(const float *geom, int ntr, const or_iparams *p);
 int or_num_threads(void);
 void or_set_num_threads(int n);

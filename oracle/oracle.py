@@ -95,6 +95,7 @@ def lib():
         L.or_image_nodes_f32.restype = C.c_int64
         L.or_image_nodes_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, fp]
         L.or_node_range.argtypes = [C.POINTER(IParams), C.c_float, ip, ip]
+        L.or_set_level_rule.argtypes = [C.c_int]
         L.or_count.restype = C.c_int64
         L.or_count.argtypes = [fp, C.c_int, C.POINTER(IParams)]
         L.or_num_threads.restype = C.c_int
@@ -307,6 +308,12 @@ def image_nodes(bank, geom, tab, ip, z_exact, src=None):
     n = lib().or_image_nodes_f32(_f(bank), _f(geom), geom.shape[0], _f(tab), C.byref(sp) if sp else None,
                                  C.byref(ip), z_exact, _f(img))
     return img, n
+
+
+def set_level_rule(rule):
+    """Test-only: the FP64 sum's level rule ("round", the product's; "floor";
+    "zero": always level 0)."""
+    lib().or_set_level_rule({"round": 0, "floor": 1, "zero": 2}[rule])
 
 
 def count(geom, ip):

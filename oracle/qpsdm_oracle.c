@@ -506,6 +506,19 @@ static int same_source(const float *geom_row, const or_src *src) {
 /* marks a trace the extent check rejected (src/q-psdm.cu:1379-1383).       */
 /* Returns the update count (pairs inside the dip aperture).                */
 /* ---------------------------------------------------------------------- */
+/* Bank level of a two-leg t* (in units of the level spacing): the nearest
+ * level, round(t* / dt*) (this build's pick; the reference's kernel takes
+ * the t* table qqq and the spacing arguments but is a stub,
+ * src/q-psdm.cu:2246-2259).  The other rules exist only so that tests can
+ * plant a wrong pick in the FP64 sum and show that a known answer fails. */
+static int g_level_rule = 0; /* 0 round (the product's), 1 floor, 2 level 0 */
+void or_set_level_rule(int rule) { g_level_rule = rule; }
+static inline int level_pick(double x) {
+  if (g_level_rule == 1) return (int)floor(x);
+  if (g_level_rule == 2) return 0;
+  return (int)(x + 0.5);
+}
+
 #define IMAGE_BODY(REAL, SQRT)                                                             \
   int64_t count = 0;                                                                       \
   const or_tgeom *g = &p->tg;                                                              \
@@ -549,7 +562,7 @@ static int same_source(const float *geom_row, const or_src *src) {
           if (ts < p->tmin) ts = p->tmin;                                                  \
           if (tg < p->tmin) tg = p->tmin;                                                  \
           REAL t = ts + tg, tst = (REAL)a2 + (REAL)b2;                                     \
-          int j = (int)(tst / p->dtstar + (REAL)0.5);                                      \
+          int j = level_pick((double)tst / (double)p->dtstar);                             \
           if (j < 0) j = 0;                                                                \
           if (j > p->K - 1) j = p->K - 1;                                                  \
           REAL f = t / p->dt;                                                              \
