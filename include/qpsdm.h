@@ -195,6 +195,21 @@ size_t qpsdm_bank_row_bytes(int K, int nt, int dtype);
 int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces_dev, int ntr, void *bank_dev,
                 int bank_dtype, void *stream);
 
+/* qpsdm_qbank for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and output as
+ * qpsdm_qbank; instead of the per-parameter-set coefficient buffers that
+ * qpsdm_qbank allocates on a set's first call, the FP64 filter coefficients
+ * are generated on the device, on `stream`, into a fixed library-owned
+ * device array (1.1 MB, one per GPU, held until the process ends) that the
+ * bank kernel then reads.  Limits: nsb <= 8192, K * nww <= 131072,
+ * nww <= 4096 (QPSDM_ERANGE beyond).  Calls no hipMalloc, hipFree or
+ * allocating library routine and copies nothing from the host.  Every call
+ * on a GPU rewrites that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race. */
+int qpsdm_qbank_noalloc(const qpsdm_bank_params *p, const float *traces_dev, int ntr, void *bank_dev,
+                        int bank_dtype, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Imaging (kernel b).  Replaces image_depth_gpu (src/q-psdm.cu:2246-2380) */
 /* and its per-trace launch (src/q-psdm.cu:2228-2243).                    */

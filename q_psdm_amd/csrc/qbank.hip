@@ -391,7 +391,7 @@ bool make_plan(int n, Plan *pl) {
   return n == 1;
 }
 
-double gain_limit(double x, double G) {
+__host__ __device__ inline double gain_limit(double x, double G) {
   const double L = log(G);
   if (x <= L) return exp(x);
   if (x <= L + 0.2) {
@@ -400,6 +400,53 @@ double gain_limit(double x, double G) {
   }
   return 1.1 * G;
 }
+
+// Arguments of the filter coefficients, shared by the host generator (the
+// cached coefficients) and the device one (the no-allocation entry point).
+struct CoefArgs {
+  int N, K, nww, nf1, nf3, nf4;
+  double dwx, wr, dtstar, threshold;
+};
+
+inline CoefArgs coef_args(const qpsdm_bank_params *p) {
+  const qpsdm_window &w = p->win;
+  CoefArgs a;
+  a.N = w.nsb;
+  a.K = p->K;
+  a.nww = w.nww;
+  a.nf1 = w.nf1;
+  a.nf3 = w.nf3;
+  a.nf4 = w.nf4;
+  a.dwx = 2.0 * M_PI / ((double)w.nsb * p->dt);
+  a.wr = 2.0 * M_PI * p->fc;
+  a.dtstar = (double)p->dtstar;
+  a.threshold = (double)p->threshold;
+  return a;
+}
+
+// Filter of level j at kept bin k, in FP64: high cut (the reference's
+// derivative filter shape, src/q-psdm.cu:1236-1247), gain limiter,
+// dispersion.
+__host__ __device__ inline float2 filter_coef(const CoefArgs &a, int j, int k) {
+  const int m = a.nf1 + k;
+  double hc = 1.0;
+  if (m > a.nf4) {
+    hc = 0.0;
+  } else if (m >= a.nf3 && a.nf4 > a.nf3) {
+    const double c = cos((double)(m - a.nf3) / (a.nf4 - a.nf3) * M_PI * 0.5);
+    hc = c * c;
+  }
+  const double om = m * a.dwx, ts = j * a.dtstar;
+  const double amp = hc * gain_limit(om * ts * 0.5, a.threshold);
+  const double ph = -om * ts * log(om / a.wr) / M_PI;
+  return make_float2((float)(amp * cos(ph)), (float)(amp * sin(ph)));
+}
+
+__host__ __device__ inline float2 twiddle(int a, int N) {
+  return make_float2((float)cos(2.0 * M_PI * a / N), (float)(-sin(2.0 * M_PI * a / N)));
+}
+
+__host__ __device__ inline float taper_coef(int i) { return 0.5 - 0.5 * cosf(M_PI * i / 200.0); }
 
 int get_coefs(const qpsdm_bank_params *p, Coefs **out) {
   const qpsdm_window &w = p->win;
@@ -418,29 +465,14 @@ int get_coefs(const qpsdm_bank_params *p, Coefs **out) {
   const int N = w.nsb, K = p->K, nww = w.nww;
   std::vector<float2> tw(N), H((size_t)K * nww);
   std::vector<float> wt(nww), tap(200);
-  for (int a = 0; a < N; ++a)
-    tw[a] = make_float2((float)cos(2.0 * M_PI * a / N), (float)(-sin(2.0 * M_PI * a / N)));
-  // coefficient generation in FP64: high cut (the reference's derivative
-  // filter shape, src/q-psdm.cu:1236-1247), gain limiter, dispersion
-  const double dwx = 2.0 * M_PI / ((double)N * p->dt), wr = 2.0 * M_PI * p->fc;
+  for (int a = 0; a < N; ++a) tw[a] = twiddle(a, N);
+  const CoefArgs ca = coef_args(p);
   for (int j = 0; j < K; ++j)
-    for (int k = 0; k < nww; ++k) {
-      const int m = w.nf1 + k;
-      double hc = 1.0;
-      if (m > w.nf4) hc = 0.0;
-      else if (m >= w.nf3 && w.nf4 > w.nf3) {
-        const double c = cos((double)(m - w.nf3) / (w.nf4 - w.nf3) * M_PI * 0.5);
-        hc = c * c;
-      }
-      const double om = m * dwx, ts = j * (double)p->dtstar;
-      const double amp = hc * gain_limit(om * ts * 0.5, p->threshold);
-      const double ph = -om * ts * log(om / wr) / M_PI;
-      H[(size_t)j * nww + k] = make_float2((float)(amp * cos(ph)), (float)(amp * sin(ph)));
-    }
+    for (int k = 0; k < nww; ++k) H[(size_t)j * nww + k] = filter_coef(ca, j, k);
   // the reference's float accumulation of omega (src/q-psdm.cu:2186-2191)
   float wv = w.nf1 * w.dw;
   for (int k = 0; k < nww; ++k, wv += w.dw) wt[k] = wv;
-  for (int i = 0; i < 200; ++i) tap[i] = 0.5 - 0.5 * cosf(M_PI * i / 200.0);
+  for (int i = 0; i < 200; ++i) tap[i] = taper_coef(i);
   QPSDM_HIP(hipMalloc(&c.tw, sizeof(float2) * N));
   QPSDM_HIP(hipMalloc(&c.H, sizeof(float2) * H.size()));
   QPSDM_HIP(hipMalloc(&c.wtab, sizeof(float) * nww));
@@ -501,6 +533,9 @@ int fit_lds(size_t lds, int nsb, bool spectra, int dtype) {
   }
 }
 
+int launch_coefs(const qpsdm_bank_params *p, const Coefs *c, const float *traces, int ntr, void *bank, int dtype,
+                 float *re, float *im, void *stream);
+
 int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank, int dtype,
            float *re, float *im, void *stream) {
   int rc = check_params(p);
@@ -510,6 +545,12 @@ int launch(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank,
   Coefs *c = nullptr;
   rc = get_coefs(p, &c);
   if (rc) return rc;
+  return launch_coefs(p, c, traces, ntr, bank, dtype, re, im, stream);
+}
+
+int launch_coefs(const qpsdm_bank_params *p, const Coefs *c, const float *traces, int ntr, void *bank, int dtype,
+                 float *re, float *im, void *stream) {
+  int rc = QPSDM_OK;
   BankDev P;
   P.nt = p->nt;
   P.dt = p->dt;
@@ -586,4 +627,98 @@ extern "C" int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces, int 
                            int dtype, void *stream) {
   QPSDM_REQUIRE(bank || ntr == 0, "bank: null output");
   return launch(p, traces, ntr, bank, dtype, nullptr, nullptr, stream);
+}
+
+// ---- the entry point that allocates nothing ------------------------------
+// qpsdm_qbank keeps each parameter set's coefficients in device memory it
+// allocates on the set's first call.  qpsdm_qbank_noalloc instead generates
+// them on the device, in FP64, into `scratch` (a file-scope device array of
+// fixed size, one copy per GPU), on the call's stream before the bank
+// kernel reads them: no allocation, no host-to-device copy, so the call can
+// be captured into a graph.  The array keeps the last call's coefficients;
+// calls that overlap on one GPU race for it.
+namespace {
+
+constexpr int kScrN = 8192;      // twiddles: nsb <= 8192 (the bank's own limit)
+constexpr int kScrKW = 131072;   // filters: K * nww <= 131072
+constexpr int kScrW = 4096;      // omega per kept bin: nww <= 4096
+constexpr int kScrTaper = 200;   // raised-cosine taper
+constexpr size_t kScrFloat2 = (size_t)kScrN + kScrKW + (kScrW + kScrTaper + 1) / 2;
+
+__device__ __align__(256) float2 scratch[kScrFloat2];
+
+std::mutex g_scr_mu;
+void *g_scr_ptr[64] = {};
+
+// The calling device's copy of scratch, looked up once per device with that
+// device current (a host-side name of a __device__ array is not a device
+// address).
+int scratch_base(float2 **out) {
+  int dev = 0;
+  QPSDM_HIP(hipGetDevice(&dev));
+  QPSDM_REQUIRE(dev >= 0 && dev < 64, "qbank_noalloc: device %d beyond the 64 this library tracks", dev);
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (!g_scr_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return qpsdm::set_error(QPSDM_EHIP, "qbank_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                              hipGetErrorString(e));
+    g_scr_ptr[dev] = ptr;
+  }
+  *out = (float2 *)g_scr_ptr[dev];
+  return QPSDM_OK;
+}
+
+__global__ void coef_kernel(CoefArgs a, float nf1dw, float dw, float2 *tw, float2 *H, float *wtab,
+                            float *taper) {
+  const int nh = a.K * a.nww;
+  const int total = a.N + nh + kScrTaper;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    if (i < a.N) {
+      tw[i] = twiddle(i, a.N);
+    } else if (i < a.N + nh) {
+      const int e = i - a.N;
+      H[e] = filter_coef(a, e / a.nww, e % a.nww);
+    } else {
+      taper[i - a.N - nh] = taper_coef(i - a.N - nh);
+    }
+  }
+  // the reference's float accumulation of omega, in order (one lane)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    float wv = nf1dw;
+    for (int k = 0; k < a.nww; ++k, wv += dw) wtab[k] = wv;
+  }
+}
+
+}  // namespace
+
+extern "C" int qpsdm_qbank_noalloc(const qpsdm_bank_params *p, const float *traces, int ntr, void *bank, int dtype,
+                                   void *stream) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  QPSDM_REQUIRE(bank || ntr == 0, "bank: null output");
+  QPSDM_REQUIRE((traces || ntr == 0) && ntr >= 0, "bank: bad traces");
+  const qpsdm_window &w = p->win;
+  if (w.nsb > kScrN || (size_t)p->K * w.nww > (size_t)kScrKW || w.nww > kScrW)
+    return qpsdm::set_error(QPSDM_ERANGE,
+                            "qbank_noalloc: nsb=%d, K*nww=%lld, nww=%d beyond the scratch limits %d, %d, %d", w.nsb,
+                            (long long)p->K * w.nww, w.nww, kScrN, kScrKW, kScrW);
+  if (ntr == 0) return QPSDM_OK;
+  Coefs c;
+  if (!make_plan(w.nsb, &c.plan))
+    return qpsdm::set_error(QPSDM_EINVAL, "nsb=%d has a prime factor above 13", w.nsb);
+  float2 *base = nullptr;
+  rc = scratch_base(&base);
+  if (rc) return rc;
+  c.tw = base;
+  c.H = base + kScrN;
+  c.wtab = (float *)(base + kScrN + kScrKW);
+  c.taper = c.wtab + kScrW;
+  const CoefArgs a = coef_args(p);
+  const int total = a.N + a.K * a.nww + kScrTaper;
+  hipLaunchKernelGGL(coef_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, a,
+                     (float)(w.nf1 * w.dw), w.dw, c.tw, c.H, c.wtab, c.taper);
+  QPSDM_HIP(hipGetLastError());
+  return launch_coefs(p, &c, traces, ntr, bank, dtype, nullptr, nullptr, stream);
 }

@@ -102,7 +102,8 @@ EXPORTS = [
     "qpsdm_window_params", "qpsdm_clamp_band", "qpsdm_npfar", "qpsdm_gain_coefs", "qpsdm_iaztab",
     "qpsdm_trace_geometry", "qpsdm_tables_build", "qpsdm_tables_f16b_bytes",
     "qpsdm_tables_quantize", "qpsdm_tables_blend", "qpsdm_read_tidx", "qpsdm_write_tidx",
-    "qpsdm_load_line_table", "qpsdm_trace_spectra", "qpsdm_bank_row_bytes", "qpsdm_qbank", "qpsdm_image_shot",
+    "qpsdm_load_line_table", "qpsdm_trace_spectra", "qpsdm_bank_row_bytes", "qpsdm_qbank", "qpsdm_qbank_noalloc",
+    "qpsdm_image_shot",
     "qpsdm_count_shot", "qpsdm_image_scale", "qpsdm_last_error", "qpsdm_version",
     "qpsdm_su_fgettr", "qpsdm_su_fputtr", "qpsdm_su_sgettr", "qpsdm_su_sputtr", "qpsdm_su_decode",
     "getscale", "get_vel_coor", "time_offset_mute",
@@ -172,6 +173,7 @@ def lib():
                                             vp, C.c_size_t, ip, ip]
         L.qpsdm_trace_spectra.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, vp, vp]
         L.qpsdm_qbank.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, C.c_int, vp]
+        L.qpsdm_qbank_noalloc.argtypes = [C.POINTER(BankParams), vp, C.c_int, vp, C.c_int, vp]
         L.qpsdm_bank_row_bytes.restype = C.c_size_t
         L.qpsdm_bank_row_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
         L.qpsdm_image_shot.argtypes = [C.POINTER(ImageParams), C.POINTER(Launch), vp, vp, C.c_int,
