@@ -322,9 +322,11 @@ int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom_dev, const 
                       int nshot, unsigned long long *count_dev, void *stream);
 
 /* As qpsdm_image_batch, with each shot's source leg read from its own
- * source table (src_dev as written by qpsdm_surrogate_src_tables for the
- * same shots); the receiver legs use tab_dev.  src_dev == NULL is
- * qpsdm_image_batch. */
+ * source table (src_dev as written by qpsdm_surrogate_src_tables or
+ * qpsdm_src_tables_nodes for the same shots); the receiver legs use tab_dev.
+ * A shot has one source: its table is built for its first valid trace's
+ * (sx, sy), and traces of the shot with another source are not imaged.
+ * src_dev == NULL is qpsdm_image_batch. */
 int qpsdm_image_batch_src(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                           const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
                           int table_dtype, const qpsdm_shot *shots_dev, int nshot, const void *src_dev,
@@ -373,13 +375,23 @@ long long qpsdm_surrogate_mask_bricks(const qpsdm_tgeom *g);
 int qpsdm_surrogate_mask(const qpsdm_surrogate *s, const qpsdm_tgeom *g, const float *tab_dev, float tol,
                          unsigned char *mask_dev, float *err_dev, void *stream);
 /* Per-shot source tables on the relative sub-grid that reaches the image:
- * header float4 {sx, sy, u0, v0} per shot, then [nshot][nvs][nus][nw]
- * float2 (t, t*) from the surrogate, or, where the mask (optional) marks the
- * nearest node's brick invalid, from that node's table tab_dev (FP32). */
+ * header float4 {sx, sy, u0, v0} per shot (the shot's first valid trace's
+ * source), then [nshot][nvs][nus][nw] float2 (t, t*) from the surrogate,
+ * or, where the mask (optional) marks the nearest node's brick invalid,
+ * from the table path: the FP32 node tables tab_dev of the (up to) four
+ * surface nodes around the source, each read at the same relative node and
+ * blended bilinearly in the source position. */
 size_t qpsdm_surrogate_src_bytes(const qpsdm_image_params *ip, int nshot);
 int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_image_params *ip, const float *geom_dev,
                                const qpsdm_shot *shots_dev, int nshot, const float *tab_dev,
                                const unsigned char *mask_dev, void *src_dev, void *stream);
+/* The same per-shot source tables from the table path alone (no network):
+ * every value the bilinear blend of the four surrounding node tables at
+ * the shot's source.  Replaces the nearest-node source leg of the node
+ * tables (the reference's tables carry per-cell source grids for the same
+ * purpose, src/q-psdm.cu:103-109); qpsdm_surrogate_src_bytes sizes it. */
+int qpsdm_src_tables_nodes(const qpsdm_image_params *ip, const float *geom_dev, const qpsdm_shot *shots_dev,
+                           int nshot, const float *tab_dev, void *src_dev, void *stream);
 
 /* A captured hipGraph of one batch: Q bank of traces_dev [ntr][nt] into
  * bank_dev, then (with a surrogate) the shots' source tables into src_dev,

@@ -9,9 +9,10 @@ count, so it is measured: every candidate images a slice of a
 representative shot, timed with HIP events, and the fastest is kept.
 
 Candidates: column tiles 8x8, 16x4 and 4x16 (one wave of columns) times
-depth points per lane ZP in {2, 3, 4, 6, 8}; for the batched kernel also
-the grid: persistent at the resident capacity or twice it, or one
-work-group per tile (nwg = -1).  Results are cached per (geometry, storage
+depth points per lane ZP in {2, 3, 4, 6, 8} (in the node form: table depth
+intervals per tile in {2, 3, 4}); for the batched kernel also the grid:
+persistent at the resident capacity or twice it, or one work-group per tile
+(nwg = -1; the node form always runs one work-group per tile).  Results are cached per (geometry, storage
 types, kernel) in memory for the process.
 """
 import itertools
@@ -20,6 +21,7 @@ import torch
 
 TILES = ((8, 8), (16, 4), (4, 16))
 DEPTHS = (2, 3, 4, 6, 8)
+NODE_INTERVALS = (2, 3, 4)  # node form: table depth intervals per tile
 GRIDS = (0, -2, -1)  # batched kernel: resident, twice resident (-2), one group per tile (-1)
 _cache = {}
 
@@ -27,7 +29,7 @@ _cache = {}
 def _key(sv, bank, tab, kind):
     c = sv.cfg
     return (c.nx, c.ny, c.nz, c.K, c.nt, tuple(sorted(c.tgeom.items())), str(bank.dtype),
-            getattr(tab, "qpsdm_code", 0), kind)
+            getattr(tab, "qpsdm_code", 0), kind, sv.form, float(sv.ip.z_exact))
 
 
 def _time(fn, reps=2):
@@ -57,13 +59,16 @@ def tune(sv, geom, bank, tab, shots=None, ntr=400, kind="batch", log=None, reps=
     img = sv.new_image()
     n = min(ntr, geom.shape[0])
     results = []
+    depths = DEPTHS
     if kind == "batch":
         tr0, n0, *tans = shots.host[0]
         one = engine.ShotList([(tr0, min(n, n0)) + tuple(tans)])
         grids = GRIDS
     else:
         grids = (0,)
-    for (tx, ty), zp, g in itertools.product(TILES, DEPTHS, grids):
+    if sv.form == "nodes":  # node tiles: tile_z intervals, one work-group per tile
+        depths, grids = NODE_INTERVALS, ((-1,) if kind == "batch" else (0,))
+    for (tx, ty), zp, g in itertools.product(TILES, depths, grids):
         launch = (tx, ty, zp, 0)
         if kind == "batch":
             nwg = g if g != -2 else 2 * _resident(sv, geom, bank, tab, one, launch)

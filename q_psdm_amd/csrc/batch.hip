@@ -328,7 +328,9 @@ extern "C" int qpsdm_count_batch(const qpsdm_image_params *p, const float *geom,
                                  int nshot, unsigned long long *count, void *stream) {
   qpsdm_launch sh = {8, 8, 8};  // count kernel shape
   qk::ImgDev P;
-  int rc = qk::make_dev(p, &sh, 0, 0, &P);
+  QPSDM_REQUIRE(p, "count_batch: null params");
+  const qpsdm_image_params pe = qk::exact_form(p);
+  int rc = qk::make_dev(&pe, &sh, 0, 0, &P);
   if (rc) return rc;
   QPSDM_REQUIRE(geom && shots && count && nshot >= 0, "count_batch: null buffer");
   if (nshot == 0) return QPSDM_OK;

@@ -224,7 +224,9 @@ extern "C" int qpsdm_gathers_shot(const qpsdm_image_params *p, const qpsdm_gathe
                                   const void *bank_dev, int bank_dtype, const void *tab_dev, int table_dtype,
                                   void *stream) {
   qk::ImgDev P;
-  int rc = qk::make_dev(p, nullptr, tr0, ntr, &P);
+  QPSDM_REQUIRE(p, "gathers: null params");
+  const qpsdm_image_params pe = qk::exact_form(p);
+  int rc = qk::make_dev(&pe, nullptr, tr0, ntr, &P);
   if (rc) return rc;
   GatherDev G;
   rc = qpsdm_gather_bins(g, &G.nbx, &G.nby);

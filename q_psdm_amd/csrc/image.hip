@@ -252,7 +252,9 @@ int launch_lookup(const qk::ImgDev &P, const Tab &tab, float px, float py, const
 extern "C" int qpsdm_tables_lookup(const qpsdm_image_params *p, const void *tab, int table_dtype, float px,
                                    float py, const float *xyz, int n, float *out, void *stream) {
   qk::ImgDev P;
-  int rc = qk::make_dev(p, nullptr, 0, 0, &P);
+  QPSDM_REQUIRE(p, "tables_lookup: null params");
+  const qpsdm_image_params pe = qk::exact_form(p);
+  int rc = qk::make_dev(&pe, nullptr, 0, 0, &P);
   if (rc) return rc;
   QPSDM_REQUIRE(tab && xyz && out && n >= 0, "tables_lookup: null args");
   if (n == 0) return QPSDM_OK;
@@ -274,7 +276,9 @@ extern "C" int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom, 
                                 unsigned long long *count, void *stream) {
   qpsdm_launch sh = {8, 8, 8};  // count kernel shape
   qk::ImgDev P;
-  int rc = qk::make_dev(p, &sh, tr0, ntr, &P);
+  QPSDM_REQUIRE(p, "count: null params");
+  const qpsdm_image_params pe = qk::exact_form(p);
+  int rc = qk::make_dev(&pe, &sh, tr0, ntr, &P);
   if (rc) return rc;
   QPSDM_REQUIRE(geom && count, "count: null buffer");
   if (ntr == 0) return QPSDM_OK;

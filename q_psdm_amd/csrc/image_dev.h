@@ -97,6 +97,15 @@ struct ImgDev {
 
 int node_range(const qpsdm_image_params *p, int ni, ImgDev *o);
 
+// The parameters with the exact form selected: for the kernels that evaluate
+// single points (update count, table lookup, dip-angle gathers), which have
+// no node form.
+inline qpsdm_image_params exact_form(const qpsdm_image_params *p) {
+  qpsdm_image_params q = *p;
+  q.weight_nodes = 0;
+  return q;
+}
+
 inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0, int ntr,
                     ImgDev *o) {
   QPSDM_REQUIRE(p, "image: null params");

@@ -715,9 +715,56 @@ __global__ void shot_src_kernel(SrcGeo S, const float4 *__restrict__ geom, const
   hdr[r] = make_float4(sx, sy, su0, sv0);
 }
 
+// (t, t*) of a source at (sx, sy) at relative node (jv, ju, jw): the node
+// tables of the (up to) four surface nodes around the source, each read at
+// the same relative node, blended bilinearly in the source position (one
+// node per axis where the node grid has one).  Exact for a laterally
+// invariant model; elsewhere the lateral variation between nodes is
+// interpolated instead of frozen at the nearest node.
+__device__ inline float2 node_blend(const qpsdm_tgeom &g, const float2 *__restrict__ tab, float sx, float sy, int jv,
+                                    int ju, int jw) {
+  const float fx = fminf(fmaxf((sx - g.n0x) / g.ndx, 0.0f), (float)(g.nnx - 1));
+  const float fy = fminf(fmaxf((sy - g.n0y) / g.ndy, 0.0f), (float)(g.nny - 1));
+  const int ix = min((int)fx, max(g.nnx - 2, 0)), iy = min((int)fy, max(g.nny - 2, 0));
+  const float a = g.nnx > 1 ? fx - (float)ix : 0.0f, b = g.nny > 1 ? fy - (float)iy : 0.0f;
+  const int ix1 = min(ix + 1, g.nnx - 1), iy1 = min(iy + 1, g.nny - 1);
+  const long long per = (long long)g.nv * g.nu * g.nw;
+  const long long cell = ((long long)jv * g.nu + ju) * g.nw + jw;
+  const float2 v00 = tab[(long long)(iy * g.nnx + ix) * per + cell];
+  const float2 v01 = tab[(long long)(iy * g.nnx + ix1) * per + cell];
+  const float2 v10 = tab[(long long)(iy1 * g.nnx + ix) * per + cell];
+  const float2 v11 = tab[(long long)(iy1 * g.nnx + ix1) * per + cell];
+  const float lx = fmaf(a, v01.x - v00.x, v00.x), ly = fmaf(a, v01.y - v00.y, v00.y);
+  const float hx = fmaf(a, v11.x - v10.x, v10.x), hy = fmaf(a, v11.y - v10.y, v10.y);
+  return make_float2(fmaf(b, hx - lx, lx), fmaf(b, hy - ly, ly));
+}
+
+// Table path of the per-shot source tables: every value from node_blend.
+__global__ void src_nodes_kernel(SrcGeo S, const float4 *__restrict__ hdr, int nshot, const float2 *__restrict__ tab,
+                                 float2 *out) {
+  const qpsdm_tgeom &g = S.g;
+  const long long total = S.stride * nshot;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int shot = (int)(q / S.stride);
+    long long r = q - (long long)shot * S.stride;
+    const int jw = (int)(r % g.nw);
+    r /= g.nw;
+    const int iu = (int)(r % S.nus), iv = (int)(r / S.nus);
+    const float4 hd = hdr[shot];
+    if (!(hd.x == hd.x)) {  // a shot with no valid trace: never read
+      out[q] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const int ju = min(max((int)rintf((hd.z - g.u0) / g.du) + iu, 0), g.nu - 1);
+    const int jv = min(max((int)rintf((hd.w - g.v0) / g.dv) + iv, 0), g.nv - 1);
+    out[q] = node_blend(g, tab, hd.x, hd.y, jv, ju, jw);
+  }
+}
+
 // Surrogate (t, t*) on each shot's sub-grid [nvs][nus][nw]; where the
-// nearest node's brick failed the validity check, the node table's value at
-// the same relative node (the nearest-node table of the table path).
+// nearest node's brick failed the validity check, the table path's value
+// (node_blend: the four surrounding node tables blended at the source).
 __global__ __launch_bounds__(kThreads) void src_table_kernel(FwdArgs A, SrcGeo S, const float4 *__restrict__ hdr,
                                                              int nshot, const float2 *__restrict__ tab,
                                                              const unsigned char *__restrict__ mask,
@@ -773,7 +820,7 @@ __global__ __launch_bounds__(kThreads) void src_table_kernel(FwdArgs A, SrcGeo S
         float2 val;
         const long long brick = (((long long)node[gg] * nbv + jv[gg] / 8) * nbu + ju[gg] / 8) * nbw + jw[gg] / 8;
         if (mask && tab && !mask[brick]) {
-          val = tab[(((long long)node[gg] * g.nv + jv[gg]) * g.nu + ju[gg]) * g.nw + jw[gg]];
+          val = node_blend(g, tab, v[gg][0], v[gg][1], jv[gg], ju[gg], jw[gg]);
         } else {
           float bt, bs, ba;
           base_leg(A.B, v[gg][0], v[gg][1], v[gg][2], v[gg][3], v[gg][4], &bt, &bs, &ba);
@@ -1115,6 +1162,26 @@ extern "C" int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_
   const long long total = S.stride * nshot;
   hipLaunchKernelGGL(src_table_kernel, dim3(grid_for((total + 63) / 64)), dim3(kThreads), 0, st, fwd_args(s), S,
                      (const float4 *)hdr, nshot, (const float2 *)tab, mask, tabs);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}
+
+extern "C" int qpsdm_src_tables_nodes(const qpsdm_image_params *ip, const float *geom, const qpsdm_shot *shots,
+                                      int nshot, const float *tab, void *src, void *stream) {
+  QPSDM_REQUIRE(geom && shots && tab && src && nshot >= 0, "src_tables_nodes: null args");
+  SrcGeo S;
+  int rc = src_geo(ip, &S);
+  if (rc) return rc;
+  if (nshot == 0) return QPSDM_OK;
+  const hipStream_t st = (hipStream_t)stream;
+  float4 *hdr = (float4 *)src;
+  float2 *tabs = (float2 *)((char *)src + (size_t)nshot * 16);
+  hipLaunchKernelGGL(shot_src_kernel, dim3((nshot + 63) / 64), dim3(64), 0, st, S, (const float4 *)geom, shots,
+                     nshot, hdr);
+  const long long total = S.stride * nshot;
+  const int grid = (int)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(src_nodes_kernel, dim3(grid), dim3(256), 0, st, S, (const float4 *)hdr, nshot,
+                     (const float2 *)tab, tabs);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }

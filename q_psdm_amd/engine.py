@@ -275,6 +275,22 @@ class Survey:
                                             _ptr(src) if src is not None else None, _ptr(work), nwg,
                                             _stream(stream)))
 
+    def src_bytes(self, nshot):
+        return L.lib().qpsdm_surrogate_src_bytes(C.byref(self.ip), nshot)
+
+    def src_tables_nodes(self, geom, shots, tab, out=None, stream=None):
+        """Per-shot source tables from the FP32 node tables alone: each value
+        the bilinear blend of the four surface nodes around the shot's
+        source (uint8 buffer: headers + tables, as Surrogate.src_tables)."""
+        assert tab.dtype == torch.float32 and tuple(tab.shape) == self.table_shape()
+        assert geom.dim() == 2 and geom.shape[1] == 4 and shots.rows <= geom.shape[0]
+        if out is None:
+            out = torch.empty(self.src_bytes(shots.n), dtype=torch.uint8, device="cuda")
+        assert out.numel() >= self.src_bytes(shots.n)
+        check(L.lib().qpsdm_src_tables_nodes(C.byref(self.ip), _ptr(geom), _ptr(shots.dev), shots.n, _ptr(tab),
+                                             _ptr(out), _stream(stream)))
+        return out
+
     def count_batch(self, geom, shots, stream=None):
         counter = torch.zeros(1, dtype=torch.int64, device="cuda")
         check(L.lib().qpsdm_count_batch(C.byref(self.ip), _ptr(geom), _ptr(shots.dev), shots.n, _ptr(counter),

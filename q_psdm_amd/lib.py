@@ -120,7 +120,8 @@ EXPORTS = [
     "qpsdm_surrogate_nparams", "qpsdm_surrogate_create", "qpsdm_surrogate_destroy",
     "qpsdm_surrogate_get_params", "qpsdm_surrogate_set_params", "qpsdm_surrogate_samples",
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
-    "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables", "qpsdm_copy_f32",
+    "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables",
+    "qpsdm_src_tables_nodes", "qpsdm_copy_f32",
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
     "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
     "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
@@ -257,6 +258,7 @@ def lib():
         L.qpsdm_surrogate_src_bytes.restype = C.c_size_t
         L.qpsdm_surrogate_src_bytes.argtypes = [C.POINTER(ImageParams), C.c_int]
         L.qpsdm_surrogate_src_tables.argtypes = [vp, C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp, vp]
+        L.qpsdm_src_tables_nodes.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp]
         L.qpsdm_graph_launch.argtypes = [vp, vp]
         L.qpsdm_graph_destroy.argtypes = [vp]
         _lib = L

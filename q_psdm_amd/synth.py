@@ -268,3 +268,35 @@ def reflectivity_traces(cfg, ntr, seed, qeff=100.0, noise=0.05, nwin=8):
     rms = np.sqrt(np.mean(out ** 2)) + 1e-30
     out += noise * rms * rng.standard_normal(out.shape)
     return out.astype(np.float32)
+
+
+def reflectivity_traces_gpu(cfg, ntr, seed, qeff=100.0, noise=0.05, nwin=8, device="cuda"):
+    """reflectivity_traces on the GPU with torch's seeded generator (another
+    random stream than the numpy version, same construction): sparse Laplace
+    reflectivity, band-limited 5-60 Hz, attenuated per time window with
+    t* = t_window/qeff and Kolsky dispersion, plus 5 % Gaussian noise.
+    float32 [ntr][nt] on the device."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) & 0x7FFFFFFFFFFFFFFF)
+    nt = cfg.nt
+    u = torch.rand((ntr, nt), generator=g, device=device, dtype=torch.float64) - 0.5
+    lap = -torch.sign(u) * torch.log1p(-2.0 * u.abs())
+    keep = torch.rand((ntr, nt), generator=g, device=device, dtype=torch.float64) < 0.05
+    r = torch.where(keep, lap, torch.zeros_like(lap))
+    nfft = 1 << int(math.ceil(math.log2(2 * nt)))
+    f = torch.fft.rfftfreq(nfft, cfg.dt, device=device, dtype=torch.float64)
+    band = torch.clamp((f - 3.0) / 4.0, 0, 1) * torch.clamp((65.0 - f) / 10.0, 0, 1)
+    w = 2 * math.pi * f
+    lnw = torch.log(torch.where(w > 0, w, torch.ones_like(w)) / (2 * math.pi * cfg.fc))
+    out = torch.zeros((ntr, nt), dtype=torch.float64, device=device)
+    edges = np.linspace(0, nt, nwin + 1).astype(int)
+    for k in range(nwin):
+        seg = torch.zeros((ntr, nfft), dtype=torch.float64, device=device)
+        seg[:, edges[k]:edges[k + 1]] = r[:, edges[k]:edges[k + 1]]
+        tst = 0.5 * (edges[k] + edges[k + 1]) * cfg.dt / qeff
+        H = band * torch.exp(-w * tst / 2) * torch.exp(1j * w * tst * lnw / math.pi)
+        out += torch.fft.irfft(torch.fft.rfft(seg, dim=1) * H[None, :], nfft, dim=1)[:, :nt]
+    rms = torch.sqrt(torch.mean(out ** 2)) + 1e-30
+    out += noise * rms * torch.randn(out.shape, generator=g, device=device, dtype=torch.float64)
+    return out.to(torch.float32).contiguous()

@@ -142,20 +142,27 @@ def test_src_tables_fallback_and_surrogate(gpu):
     g_d = torch.from_numpy(geom).cuda()
     nb = engine.L.lib().qpsdm_surrogate_mask_bricks(engine.C.byref(sv.tg))
     g = cfg.tgeom
-    # every brick invalid: the node tables' values at the same relative nodes
+    # every brick invalid: the table path -- the four node tables around the
+    # source, each at the same relative nodes, blended bilinearly in (sx, sy)
     src = s.src_tables(sv, g_d, shots, tab=tab, mask=torch.zeros(nb, dtype=torch.uint8, device="cuda"))
     hdr, tabs, (nus, nvs, nw) = _src_layout(s, sv, shots.n, src)
-    t_all = tab.cpu().numpy()
+    t_all = tab.cpu().numpy().astype(np.float64)
     for k, (tr0, n, *_) in enumerate(shots.host):
         sx, sy = geom[tr0, 0], geom[tr0, 1]
         assert hdr[k, 0] == sx and hdr[k, 1] == sy
         iu0 = int(round((hdr[k, 2] - g["u0"]) / g["du"]))
         iv0 = int(round((hdr[k, 3] - g["v0"]) / g["dv"]))
-        node = (min(max(int(np.floor(np.float32(sx - g["n0x"]) * np.float32(1.0 / g["ndx"]) + 0.5)), 0), g["nnx"] - 1)
-                + g["nnx"] * min(max(int(np.floor(np.float32(sy - g["n0y"]) * np.float32(1.0 / g["ndy"]) + 0.5)), 0),
-                                 g["nny"] - 1))
-        ref = t_all[node, iv0:iv0 + nvs, iu0:iu0 + nus]
-        assert np.array_equal(tabs[k].cpu().numpy(), ref)
+        fx = min(max((float(sx) - g["n0x"]) / g["ndx"], 0.0), g["nnx"] - 1.0)
+        fy = min(max((float(sy) - g["n0y"]) / g["ndy"], 0.0), g["nny"] - 1.0)
+        ix, iy = min(int(fx), g["nnx"] - 2), min(int(fy), g["nny"] - 2)
+        a, b = fx - ix, fy - iy
+        node = lambda j, i: t_all[j * g["nnx"] + i, iv0:iv0 + nvs, iu0:iu0 + nus]
+        ref = ((1 - b) * ((1 - a) * node(iy, ix) + a * node(iy, ix + 1))
+               + b * ((1 - a) * node(iy + 1, ix) + a * node(iy + 1, ix + 1)))
+        np.testing.assert_allclose(tabs[k].cpu().numpy(), ref, rtol=2e-6, atol=1e-7)
+    # the table path alone writes the same tables, bit for bit
+    src_t = sv.src_tables_nodes(g_d, shots, tab)
+    assert torch.equal(src_t, src)
     # every brick valid: the surrogate at those points
     src1 = s.src_tables(sv, g_d, shots, tab=tab, mask=torch.ones(nb, dtype=torch.uint8, device="cuda"))
     hdr1, tabs1, _ = _src_layout(s, sv, shots.n, src1)
@@ -170,11 +177,18 @@ def test_src_tables_fallback_and_surrogate(gpu):
     assert np.abs(got - out[:, :2]).max() < 2e-6
 
 
-def test_image_with_src_tables_matches_node_path_under_full_fallback(gpu):
+def test_image_with_src_tables_matches_node_path_for_shots_on_nodes(gpu):
+    """Shots whose sources sit on surface nodes: the table path's source
+    tables (the surrogate fully fallen back) hold the node's own values, and
+    the image equals the node-table path's."""
     cfg = synth.tiny_layers()
     sv, tab = _survey(cfg)
     s = _sur(cfg)
     geom, shots = _shots(cfg, n=4, seed=3)
+    g = cfg.tgeom
+    for k, (tr0, n, *_) in enumerate(shots.host):
+        geom[tr0:tr0 + n, 0] = g["n0x"] + (k % g["nnx"]) * g["ndx"]
+        geom[tr0:tr0 + n, 1] = g["n0y"] + ((k + 1) % g["nny"]) * g["ndy"]
     g_d = torch.from_numpy(geom).cuda()
     traces = torch.from_numpy(np.random.default_rng(1).standard_normal((geom.shape[0], cfg.nt))
                               .astype(np.float32)).cuda()
@@ -224,10 +238,24 @@ def test_surrogate_gain_over_nearest_node_tables(gpu, trained):
     _, tn, _ = _src_layout(s, sv, shots.n, src_n)
     _, tx, _ = _src_layout(s, sv, shots.n, src_x)
     es = (ts - tx)[..., 0].abs()
-    en = (tn - tx)[..., 0].abs()
+    en = (tn - tx)[..., 0].abs()     # the table path (node blend) at off-node sources
+    # the nearest node's own values at the same relative nodes
+    hdr, _, (nus, nvs, nw) = _src_layout(s, sv, shots.n, src_s)
+    g = cfg.tgeom
+    t_all = tab.cpu().numpy()
+    near = np.empty((shots.n, nvs, nus, nw))
+    for k in range(shots.n):
+        iu0 = int(round((hdr[k, 2] - g["u0"]) / g["du"]))
+        iv0 = int(round((hdr[k, 3] - g["v0"]) / g["dv"]))
+        i = min(max(int(np.floor((hdr[k, 0] - g["n0x"]) / g["ndx"] + 0.5)), 0), g["nnx"] - 1)
+        j = min(max(int(np.floor((hdr[k, 1] - g["n0y"]) / g["ndy"] + 0.5)), 0), g["nny"] - 1)
+        near[k] = t_all[j * g["nnx"] + i, iv0:iv0 + nvs, iu0:iu0 + nus, :, 0]
+    eq = np.abs(near - tx[..., 0].cpu().numpy())
     print(f"mask: {frac_bad * 100:.1f}% bricks fall back; source-leg |dt| vs exact: surrogate mean "
-          f"{float(es.mean()) * 1e3:.4f} ms max {float(es.max()) * 1e3:.3f} ms; nearest node mean "
-          f"{float(en.mean()) * 1e3:.4f} ms max {float(en.max()) * 1e3:.3f} ms")
+          f"{float(es.mean()) * 1e3:.4f} ms max {float(es.max()) * 1e3:.3f} ms; node blend mean "
+          f"{float(en.mean()) * 1e3:.4f} ms max {float(en.max()) * 1e3:.3f} ms; nearest node mean "
+          f"{eq.mean() * 1e3:.4f} ms max {eq.max() * 1e3:.3f} ms")
+    assert float(en.mean()) < eq.mean()
     # this small model's anomaly is sharp on its domain (sigma 80 m over 200 m)
     # and one failed point sends a whole 8^3 brick back to the table, so the
     # gain here is a lower bound on the survey-scale one
@@ -236,10 +264,12 @@ def test_surrogate_gain_over_nearest_node_tables(gpu, trained):
                               .astype(np.float32)).cuda()
     bank = sv.qbank(traces)
     imgs = {}
-    for name, src in (("nodes", None), ("surrogate", src_s), ("exact", src_x)):
+    for name, src in (("nodes", None), ("blend", src_n), ("surrogate", src_s), ("exact", src_x)):
         img = sv.new_image()
         sv.image_batch(img, g_d, bank, tab, shots, src=src)
         imgs[name] = img.cpu().numpy()
     e_s, e_n = rel_l2(imgs["surrogate"], imgs["exact"]), rel_l2(imgs["nodes"], imgs["exact"])
-    print(f"image rel-L2 vs exact source legs: surrogate {e_s:.2e}, nearest-node tables {e_n:.2e}")
+    e_b = rel_l2(imgs["blend"], imgs["exact"])
+    print(f"image rel-L2 vs exact source legs: surrogate {e_s:.2e}, node blend {e_b:.2e}, "
+          f"nearest-node tables {e_n:.2e}")
     assert e_s < 0.75 * e_n
