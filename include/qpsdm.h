@@ -152,6 +152,46 @@ typedef struct {
   uint16_t nys, nxs, dys, dxs;
 } qpsdm_ttt_desc;
 
+/* A reference-layout table set over the image, as this build writes and
+ * reads it (the layout of src/q-psdm.cu:103-109, 1158-1159, 2382-2622).  The
+ * reference leaves the order inside a cell and the origin of a cell's
+ * source grid unstated (its kernel computes a cell index and then uses
+ * fixed traveltimes, src/q-psdm.cu:2343-2352); this build's convention:
+ *   - columns cdp1 .. cdp1 + ncol - 1 sit at image-relative
+ *     x = x0 + (c - cdp1) dx, line file L<line1 + l> at y = y0 + l dy,
+ *     depth node k at z = k dz (the index header's dx and dz);
+ *   - cell (ixg, izg), descriptor ixg nzg + izg, holds columns
+ *     cdp1 + ixg nxi0 + [0, nxi0) and depth nodes izg nzi0 + [0, nzi0)
+ *     (past the last column or node: the last one's values);
+ *   - its payload at float offset idx is [iys][ixs][column][depth], depth
+ *     fastest ("z contiguous", src/q-psdm.cu:1158-1159);
+ *   - every cell's source grid is the surface-node grid of the node tables:
+ *     nys = nny, nxs = nnx, dys = ndy, dxs = ndx, source node (iys, ixs) at
+ *     (n0x + ixs ndx, n0y + iys ndy).
+ * L<n>.ttt and Lxxx.tidx hold t, L<n>.qqq and Lxxx.qidx hold t*. */
+typedef struct {
+  int line1, nlines;  /* first line number and number of lines   */
+  int cdp1, ncol;     /* first column index and number of columns */
+  int nz;             /* depth nodes                              */
+  int nxi0, nzi0;     /* columns and depth nodes per cell         */
+  float x0, y0;       /* image-relative x of column cdp1, y of line line1 */
+  float dx, dy, dz;   /* column, line and depth-node spacing      */
+} qpsdm_ref_layout;
+
+/* Table builder output in the reference layout: writes Lxxx.tidx,
+ * Lxxx.qidx and L<n>.ttt / L<n>.qqq for every line of the layout into dir,
+ * from FP32 node tables on the device.  The layout's lattice must be the node
+ * tables' (dx = du, dy = dv, dz = dw, aligned to the relative grid): each
+ * payload value is a node table entry. */
+int qpsdm_tables_write_ref(const qpsdm_tgeom *g, const float *tab_dev, const qpsdm_ref_layout *L, const char *dir);
+/* The inverse: reads the index files and each line's payload (a missing
+ * line blended from the nearest existing lines around it, as get_ttt_all
+ * does) and fills the FP32 node tables on the device; a node-table entry
+ * whose point lies outside the layout's columns, lines or depths takes the
+ * nearest covered point's value.  Allocates a staging buffer for the
+ * payload. */
+int qpsdm_tables_read_ref(const qpsdm_tgeom *g, const qpsdm_ref_layout *L, const char *dir, float *tab_dev);
+
 /* Read an index file; desc may be NULL to query the header only. */
 int qpsdm_read_tidx(const char *path, qpsdm_tidx_header *hdr, qpsdm_ttt_desc *desc, int maxdesc);
 int qpsdm_write_tidx(const char *path, const qpsdm_tidx_header *hdr, const qpsdm_ttt_desc *desc);

@@ -3,6 +3,10 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "image_dev.h"
@@ -241,4 +245,211 @@ extern "C" int qpsdm_copy_f32(const float *src, float *dst, size_t n, void *stre
                      (float4 *)dst, n / 4);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
+}
+
+// ---- reference table layout (Lxxx.tidx|qidx + L<n>.ttt|qqq) -----------------
+namespace {
+
+struct RefGeo {
+  qpsdm_tgeom g;
+  qpsdm_ref_layout L;
+  int nxg, nzg;
+  long long per_cell;  // nny nnx nxi0 nzi0 floats
+  long long per_line;  // nxg nzg per_cell
+};
+
+int ref_geo(const qpsdm_tgeom *g, const qpsdm_ref_layout *L, RefGeo *o) {
+  QPSDM_REQUIRE(g && L, "tables_ref: null args");
+  QPSDM_REQUIRE(L->nlines >= 1 && L->ncol >= 1 && L->nz >= 1 && L->nxi0 >= 1 && L->nzi0 >= 1,
+                "tables_ref: empty layout");
+  QPSDM_REQUIRE(L->dx > 0 && L->dy > 0 && L->dz > 0, "tables_ref: bad spacing");
+  QPSDM_REQUIRE(g->ndx >= 1 && g->ndx <= 65535 && g->ndy >= 1 && g->ndy <= 65535 && g->ndx == rintf(g->ndx) &&
+                    g->ndy == rintf(g->ndy),
+                "tables_ref: node spacing must be whole metres below 65536 (16-bit source-grid steps)");
+  QPSDM_REQUIRE(g->nnx <= 65535 && g->nny <= 65535, "tables_ref: too many nodes for 16-bit source grids");
+  o->g = *g;
+  o->L = *L;
+  o->nxg = (L->ncol + L->nxi0 - 1) / L->nxi0;
+  o->nzg = (L->nz + L->nzi0 - 1) / L->nzi0;
+  o->per_cell = (long long)g->nny * g->nnx * L->nxi0 * L->nzi0;
+  o->per_line = (long long)o->nxg * o->nzg * o->per_cell;
+  return QPSDM_OK;
+}
+
+// Payload of line l: element o -> node table entry (component comp); a
+// column, line or depth outside the node table's relative range is clamped.
+__global__ void ref_pack_kernel(RefGeo R, const float2 *__restrict__ tab, int l, int comp, float *out) {
+  const qpsdm_tgeom &g = R.g;
+  const qpsdm_ref_layout &L = R.L;
+  for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < R.per_line;
+       o += (long long)gridDim.x * blockDim.x) {
+    const long long cell = o / R.per_cell;
+    long long r = o - cell * R.per_cell;
+    const int izc = (int)(r % L.nzi0);
+    r /= L.nzi0;
+    const int ixc = (int)(r % L.nxi0);
+    r /= L.nxi0;
+    const int ixs = (int)(r % g.nnx), iys = (int)(r / g.nnx);
+    const int ixg = (int)(cell / R.nzg), izg = (int)(cell % R.nzg);
+    const int c = min(ixg * L.nxi0 + ixc, L.ncol - 1);
+    const int k = min(izg * L.nzi0 + izc, L.nz - 1);
+    const float x = L.x0 + (float)c * L.dx, y = L.y0 + (float)l * L.dy, z = (float)k * L.dz;
+    const float px = g.n0x + (float)ixs * g.ndx, py = g.n0y + (float)iys * g.ndy;
+    const int iu = min(max((int)rintf((x - px - g.u0) / g.du), 0), g.nu - 1);
+    const int iv = min(max((int)rintf((y - py - g.v0) / g.dv), 0), g.nv - 1);
+    const int iw = min(max((int)rintf(z / g.dw), 0), g.nw - 1);
+    const float2 v = tab[(((long long)(iys * g.nnx + ixs) * g.nv + iv) * g.nu + iu) * g.nw + iw];
+    out[o] = comp ? v.y : v.x;
+  }
+}
+
+// Node table entries (component comp) from the lines' payloads (all lines
+// [nlines][per_line] on the device), each at its point's nearest covered
+// column, line and depth node.
+__global__ void ref_unpack_kernel(RefGeo R, const float *__restrict__ lines, int comp, float2 *tab) {
+  const qpsdm_tgeom &g = R.g;
+  const qpsdm_ref_layout &L = R.L;
+  const long long per = (long long)g.nv * g.nu * g.nw;
+  const long long total = per * g.nnx * g.nny;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int node = (int)(i / per);
+    long long r = i - node * per;
+    const int iw = (int)(r % g.nw);
+    r /= g.nw;
+    const int iu = (int)(r % g.nu), iv = (int)(r / g.nu);
+    const int ixs = node % g.nnx, iys = node / g.nnx;
+    const float px = g.n0x + (float)ixs * g.ndx, py = g.n0y + (float)iys * g.ndy;
+    const float x = px + g.u0 + (float)iu * g.du, y = py + g.v0 + (float)iv * g.dv, z = (float)iw * g.dw;
+    const int c = min(max((int)rintf((x - L.x0) / L.dx), 0), L.ncol - 1);
+    const int l = min(max((int)rintf((y - L.y0) / L.dy), 0), L.nlines - 1);
+    const int k = min(max((int)rintf(z / L.dz), 0), L.nz - 1);
+    const int ixg = c / L.nxi0, ixc = c % L.nxi0, izg = k / L.nzi0, izc = k % L.nzi0;
+    const long long o = (long long)l * R.per_line + (long long)(ixg * R.nzg + izg) * R.per_cell +
+                        (((long long)iys * g.nnx + ixs) * L.nxi0 + ixc) * L.nzi0 + izc;
+    float2 v = tab[i];
+    if (comp)
+      v.y = lines[o];
+    else
+      v.x = lines[o];
+    tab[i] = v;
+  }
+}
+
+const char *kIdxSuffix[2] = {"tidx", "qidx"};
+const char *kDatSuffix[2] = {"ttt", "qqq"};
+
+}  // namespace
+
+extern "C" int qpsdm_tables_write_ref(const qpsdm_tgeom *g, const float *tab, const qpsdm_ref_layout *L,
+                                      const char *dir) {
+  RefGeo R;
+  int rc = ref_geo(g, L, &R);
+  if (rc) return rc;
+  QPSDM_REQUIRE(tab && dir, "tables_write_ref: null args");
+  qpsdm_tidx_header h;
+  h.cdp1 = L->cdp1;
+  h.cdp2 = L->cdp1 + L->ncol - 1;
+  h.dx = L->dx;
+  h.dz = L->dz;
+  h.nzg = R.nzg;
+  h.nxg = R.nxg;
+  h.nzi0 = L->nzi0;
+  h.nxi0 = L->nxi0;
+  std::vector<qpsdm_ttt_desc> d((size_t)R.nxg * R.nzg);
+  for (int ixg = 0; ixg < R.nxg; ++ixg)
+    for (int izg = 0; izg < R.nzg; ++izg) {
+      qpsdm_ttt_desc &e = d[(size_t)ixg * R.nzg + izg];
+      e.idx = (int64_t)(ixg * R.nzg + izg) * R.per_cell;
+      e.nys = (uint16_t)g->nny;
+      e.nxs = (uint16_t)g->nnx;
+      e.dys = (uint16_t)g->ndy;
+      e.dxs = (uint16_t)g->ndx;
+    }
+  float *dev = nullptr;
+  QPSDM_HIP(hipMalloc(&dev, sizeof(float) * R.per_line));
+  std::vector<float> host(R.per_line);
+  char path[4096];
+  for (int comp = 0; comp < 2 && !rc; ++comp) {
+    snprintf(path, sizeof(path), "%s/Lxxx.%s", dir, kIdxSuffix[comp]);
+    rc = qpsdm_write_tidx(path, &h, d.data());
+    for (int l = 0; l < L->nlines && !rc; ++l) {
+      const int grid = (int)std::min<long long>((R.per_line + 255) / 256, 65536);
+      hipLaunchKernelGGL(ref_pack_kernel, dim3(grid), dim3(256), 0, 0, R, (const float2 *)tab, l, comp, dev);
+      if (hipMemcpy(host.data(), dev, sizeof(float) * R.per_line, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = qpsdm::set_error(QPSDM_EHIP, "tables_write_ref: copy of line %d failed", L->line1 + l);
+        break;
+      }
+      snprintf(path, sizeof(path), "%s/L%d.%s", dir, L->line1 + l, kDatSuffix[comp]);
+      FILE *fp = fopen(path, "wb");
+      if (!fp) {
+        rc = qpsdm::set_error(QPSDM_EIO, "cannot create %s", path);
+        break;
+      }
+      const bool ok = fwrite(host.data(), sizeof(float), R.per_line, fp) == (size_t)R.per_line;
+      fclose(fp);
+      if (!ok) rc = qpsdm::set_error(QPSDM_EIO, "short write %s", path);
+    }
+  }
+  (void)hipFree(dev);
+  return rc;
+}
+
+extern "C" int qpsdm_tables_read_ref(const qpsdm_tgeom *g, const qpsdm_ref_layout *L, const char *dir, float *tab) {
+  RefGeo R;
+  int rc = ref_geo(g, L, &R);
+  if (rc) return rc;
+  QPSDM_REQUIRE(tab && dir, "tables_read_ref: null args");
+  char path[4096];
+  std::vector<qpsdm_ttt_desc> d((size_t)R.nxg * R.nzg);
+  // both index files must describe this layout with the node grid as every cell's source grid
+  for (int comp = 0; comp < 2; ++comp) {
+    snprintf(path, sizeof(path), "%s/Lxxx.%s", dir, kIdxSuffix[comp]);
+    qpsdm_tidx_header h;
+    rc = qpsdm_read_tidx(path, &h, nullptr, 0);
+    if (rc) return rc;
+    if (h.cdp1 != L->cdp1 || h.cdp2 != L->cdp1 + L->ncol - 1 || h.nxi0 != L->nxi0 || h.nzi0 != L->nzi0 ||
+        h.nxg != R.nxg || h.nzg != R.nzg || h.dx != L->dx || h.dz != L->dz)
+      return qpsdm::set_error(QPSDM_EINVAL, "%s: header (cdp %d-%d, cells %d x %d of %d x %d) is not the layout's",
+                              path, h.cdp1, h.cdp2, h.nxg, h.nzg, h.nxi0, h.nzi0);
+    rc = qpsdm_read_tidx(path, &h, d.data(), (int)d.size());
+    if (rc) return rc;
+    for (size_t i = 0; i < d.size(); ++i)
+      if (d[i].nys != g->nny || d[i].nxs != g->nnx || d[i].dys != (uint16_t)g->ndy || d[i].dxs != (uint16_t)g->ndx ||
+          d[i].idx != (int64_t)i * R.per_cell)
+        return qpsdm::set_error(QPSDM_EINVAL,
+                                "%s: cell %zu's source grid %u x %u at %u x %u m (offset %lld) is not the surface-node "
+                                "grid %d x %d at %g x %g m",
+                                path, i, d[i].nys, d[i].nxs, d[i].dys, d[i].dxs, (long long)d[i].idx, g->nny, g->nnx,
+                                g->ndy, g->ndx);
+  }
+  const size_t nfl = (size_t)L->nlines * R.per_line;
+  float *dev = nullptr;
+  QPSDM_HIP(hipMalloc(&dev, sizeof(float) * nfl));
+  std::vector<float> host(R.per_line);
+  for (int comp = 0; comp < 2 && !rc; ++comp) {
+    for (int l = 0; l < L->nlines && !rc; ++l) {
+      const long n = qpsdm_load_line_table(dir, kDatSuffix[comp], L->line1 + l, L->line1,
+                                           L->line1 + L->nlines - 1, host.data(), host.size(), nullptr, nullptr);
+      if (n < 0) {
+        rc = (int)n;
+        break;
+      }
+      if (n != R.per_line) {
+        rc = qpsdm::set_error(QPSDM_EIO, "L%d.%s holds %ld floats, the layout %lld", L->line1 + l, kDatSuffix[comp],
+                              n, R.per_line);
+        break;
+      }
+      if (hipMemcpy(dev + (size_t)l * R.per_line, host.data(), sizeof(float) * R.per_line, hipMemcpyHostToDevice) !=
+          hipSuccess)
+        rc = qpsdm::set_error(QPSDM_EHIP, "tables_read_ref: upload of line %d failed", L->line1 + l);
+    }
+    if (rc) break;
+    const long long total = (long long)g->nv * g->nu * g->nw * g->nnx * g->nny;
+    const int grid = (int)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(ref_unpack_kernel, dim3(grid), dim3(256), 0, 0, R, dev, comp, (float2 *)tab);
+    if (hipDeviceSynchronize() != hipSuccess) rc = qpsdm::set_error(QPSDM_EHIP, "tables_read_ref: unpack failed");
+  }
+  (void)hipFree(dev);
+  return rc;
 }

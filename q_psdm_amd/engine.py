@@ -120,6 +120,36 @@ class Survey:
                                          _ptr(tab), _stream(stream)))
         return tab
 
+    def ref_layout(self, nxi0=16, nzi0=8, line1=1):
+        """The reference-layout lattice of this survey's node tables: the
+        (du, dv, dw) lattice on every column, line and depth node that some
+        node's relative table reaches (the union of their footprints, so the
+        files hold every table entry), nxi0 columns x nzi0 depth nodes per
+        cell.  The node spacing must be a multiple of du and dv."""
+        g = self.cfg.tgeom
+        kx, ky = g["ndx"] / g["du"], g["ndy"] / g["dv"]
+        assert g["nnx"] == 1 or abs(kx - round(kx)) < 1e-6, "node spacing not on the du lattice"
+        assert g["nny"] == 1 or abs(ky - round(ky)) < 1e-6, "node spacing not on the dv lattice"
+        ncol = (g["nnx"] - 1) * int(round(kx)) + g["nu"]
+        nlines = (g["nny"] - 1) * int(round(ky)) + g["nv"]
+        return L.RefLayout(line1, nlines, 0, ncol, g["nw"], nxi0, nzi0, g["n0x"] + g["u0"], g["n0y"] + g["v0"],
+                           g["du"], g["dv"], g["dw"])
+
+    def write_ref_tables(self, tab, layout, directory):
+        """Write FP32 node tables as Lxxx.tidx/qidx + L<n>.ttt/qqq (the
+        reference's table files) in the layout's convention."""
+        assert tab.dtype == torch.float32 and tuple(tab.shape) == self.table_shape()
+        torch.cuda.synchronize()
+        check(L.lib().qpsdm_tables_write_ref(C.byref(self.tg), _ptr(tab), C.byref(layout), directory.encode()))
+
+    def read_ref_tables(self, layout, directory, out=None):
+        """FP32 node tables from reference-layout table files."""
+        if out is None:
+            out = torch.zeros(self.table_shape(), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        check(L.lib().qpsdm_tables_read_ref(C.byref(self.tg), C.byref(layout), directory.encode(), _ptr(out)))
+        return out
+
     def quantize_tables(self, tab, code="f16", stream=None):
         """16-bit brick tables; code "f16" (FP16 codes in 8^3 bricks), "u16"
         (fixed point in 8^3 bricks), "u16x" (fixed point in edge-sharing

@@ -52,6 +52,12 @@ class TttDesc(C.Structure):
                 ("dys", C.c_uint16), ("dxs", C.c_uint16)]
 
 
+class RefLayout(C.Structure):
+    _fields_ = [("line1", C.c_int), ("nlines", C.c_int), ("cdp1", C.c_int), ("ncol", C.c_int), ("nz", C.c_int),
+                ("nxi0", C.c_int), ("nzi0", C.c_int), ("x0", C.c_float), ("y0", C.c_float), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz", C.c_float)]
+
+
 class BankParams(C.Structure):
     _fields_ = [("nt", C.c_int), ("dt", C.c_float), ("win", Window), ("K", C.c_int),
                 ("dtstar", C.c_float), ("fc", C.c_float), ("threshold", C.c_float)]
@@ -121,7 +127,7 @@ EXPORTS = [
     "qpsdm_surrogate_get_params", "qpsdm_surrogate_set_params", "qpsdm_surrogate_samples",
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
     "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables",
-    "qpsdm_src_tables_nodes", "qpsdm_copy_f32",
+    "qpsdm_src_tables_nodes", "qpsdm_copy_f32", "qpsdm_tables_write_ref", "qpsdm_tables_read_ref",
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
     "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
     "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
@@ -258,6 +264,8 @@ def lib():
         L.qpsdm_surrogate_src_bytes.restype = C.c_size_t
         L.qpsdm_surrogate_src_bytes.argtypes = [C.POINTER(ImageParams), C.c_int]
         L.qpsdm_surrogate_src_tables.argtypes = [vp, C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp, vp]
+        L.qpsdm_tables_write_ref.argtypes = [C.POINTER(TGeom), vp, C.POINTER(RefLayout), C.c_char_p]
+        L.qpsdm_tables_read_ref.argtypes = [C.POINTER(TGeom), C.POINTER(RefLayout), C.c_char_p, vp]
         L.qpsdm_src_tables_nodes.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp]
         L.qpsdm_graph_launch.argtypes = [vp, vp]
         L.qpsdm_graph_destroy.argtypes = [vp]
