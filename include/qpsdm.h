@@ -408,9 +408,11 @@ int qpsdm_surrogate_train(qpsdm_surrogate *s, const float *X_dev, const float *T
                           int steps, int batch, float lr0, float lr1, float *loss_host, void *stream);
 /* Judge: the surrogate at each node's own source position against that
  * node's FP32 table, at the table points inside the input box (the only
- * points source tables query); mask[brick] = 1 where max |dt| <= tol over
- * the 8^3 brick (node-major, [node][bv][bu][bw]); err_dev [bricks] gets the
- * max. */
+ * points source tables query), and at off-node sources inside the node's
+ * cell (displaced by 0.45 of the node spacing in eight directions) against
+ * the FP64 ray integrals at the brick's corners; mask[brick] = 1 where
+ * max |dt| <= tol over all of them (node-major, [node][bv][bu][bw]);
+ * err_dev [bricks] gets the max. */
 long long qpsdm_surrogate_mask_bricks(const qpsdm_tgeom *g);
 int qpsdm_surrogate_mask(const qpsdm_surrogate *s, const qpsdm_tgeom *g, const float *tab_dev, float tol,
                          unsigned char *mask_dev, float *err_dev, void *stream);
@@ -418,9 +420,10 @@ int qpsdm_surrogate_mask(const qpsdm_surrogate *s, const qpsdm_tgeom *g, const f
  * header float4 {sx, sy, u0, v0} per shot (the shot's first valid trace's
  * source), then [nshot][nvs][nus][nw] float2 (t, t*) from the surrogate,
  * or, where the mask (optional) marks the nearest node's brick invalid,
- * from the table path: the FP32 node tables tab_dev of the (up to) four
- * surface nodes around the source, each read at the same relative node and
- * blended bilinearly in the source position. */
+ * from the table builder at the shot's own source (the FP64 straight-ray
+ * integrals the node tables and the training samples come from).  tab_dev
+ * is not read (may be NULL): the node tables judge the network in
+ * qpsdm_surrogate_mask. */
 size_t qpsdm_surrogate_src_bytes(const qpsdm_image_params *ip, int nshot);
 int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_image_params *ip, const float *geom_dev,
                                const qpsdm_shot *shots_dev, int nshot, const float *tab_dev,

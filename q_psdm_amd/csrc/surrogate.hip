@@ -677,6 +677,82 @@ __global__ __launch_bounds__(kThreads) void mask_err_kernel(FwdArgs A, qpsdm_tge
   }
 }
 
+// The same judge at off-node sources: for every brick, the network against
+// the FP64 ray integrals at the brick's eight corner points from sources
+// displaced from the node by (+-0.45 ndx, +-0.45 ndy) and (+-0.45 ndx, 0),
+// (0, +-0.45 ndy) (inside the node's cell, where its shots lie); max |dt|
+// per brick into err (atomic max on the float bits, as mask_err_kernel).
+constexpr int kOffSrc = 8, kCorners = 8;
+
+__global__ __launch_bounds__(kThreads) void mask_offnode_kernel(FwdArgs A, qk::ModelDev M, qpsdm_tgeom g,
+                                                                unsigned *err) {
+  __shared__ __attribute__((aligned(16))) bf16 lw[kNF];
+  __shared__ float lb[kNBias];
+  load_weights(lw, lb, A.wf, kNF, A.bias);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nbv = (g.nv + 7) / 8, nbu = (g.nu + 7) / 8, nbw = (g.nw + 7) / 8;
+  const long long nbrick = (long long)g.nnx * g.nny * nbv * nbu * nbw;
+  const long long total = nbrick * kOffSrc * kCorners;
+  const long long ngroups = (total + 63) / 64;
+  const float ox[kOffSrc] = {0.45f, -0.45f, 0.45f, -0.45f, 0.45f, -0.45f, 0.0f, 0.0f};
+  const float oy[kOffSrc] = {0.45f, 0.45f, -0.45f, -0.45f, 0.0f, 0.0f, 0.45f, -0.45f};
+  const TrainStore st{};
+  for (long long grp = (long long)blockIdx.x * 4 + wave; grp < ngroups; grp += (long long)gridDim.x * 4) {
+    bf16x8 x[2];
+    float v[2][kIn];
+    long long idx[2], brick[2];
+#pragma unroll
+    for (int gg = 0; gg < 2; ++gg) {
+      long long q = grp * 64 + 32 * gg + (lane & 31);
+      idx[gg] = q;
+      if (q >= total) q = total - 1;
+      const int corner = (int)(q % kCorners);
+      const int pos = (int)((q / kCorners) % kOffSrc);
+      const long long b = q / (kCorners * kOffSrc);
+      brick[gg] = b;
+      long long r = b;
+      const int bw = (int)(r % nbw);
+      r /= nbw;
+      const int bu = (int)(r % nbu);
+      r /= nbu;
+      const int bv = (int)(r % nbv);
+      const int node = (int)(r / nbv);
+      const int iu = min(bu * 8 + ((corner & 1) ? 7 : 0), g.nu - 1);
+      const int iv = min(bv * 8 + ((corner & 2) ? 7 : 0), g.nv - 1);
+      const int iw = min(bw * 8 + ((corner & 4) ? 7 : 0), g.nw - 1);
+      const float sx = g.n0x + (float)(node % g.nnx) * g.ndx + ox[pos] * g.ndx;
+      const float sy = g.n0y + (float)(node / g.nnx) * g.ndy + oy[pos] * g.ndy;
+      v[gg][0] = sx;
+      v[gg][1] = sy;
+      v[gg][2] = sx + g.u0 + (float)iu * g.du;
+      v[gg][3] = sy + g.v0 + (float)iv * g.dv;
+      v[gg][4] = (float)iw * g.dw;
+      x[gg] = feature_operand(A.N, v[gg]);
+    }
+    float r[2][kOut];
+    forward64<false>(lw, lb, x, r, st);
+    if (lane < 32) {
+#pragma unroll
+      for (int gg = 0; gg < 2; ++gg) {
+        if (idx[gg] >= total) continue;
+        bool inside = true;
+#pragma unroll
+        for (int f = 0; f < kIn; ++f) {
+          const float u = (v[gg][f] - A.N.lo[f]) * A.N.sc[f] - 1.0f;
+          inside = inside && u >= -1.0001f && u <= 1.0001f;
+        }
+        if (!inside) continue;
+        float bt, bs, ba;
+        base_leg(A.B, v[gg][0], v[gg][1], v[gg][2], v[gg][3], v[gg][4], &bt, &bs, &ba);
+        double t, ts;
+        qk::ray_integrals(M, v[gg][0], v[gg][1], v[gg][2], v[gg][3], v[gg][4], &t, &ts, nullptr);
+        const float e = fabsf(bt + r[gg][0] * A.N.osc[0] - (float)t);
+        atomicMax(err + brick[gg], __float_as_uint(e));
+      }
+    }
+  }
+}
+
 __global__ void mask_threshold_kernel(const unsigned *err, long long nb, float tol, unsigned char *mask) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (long long)gridDim.x * blockDim.x)
     mask[i] = __uint_as_float(err[i]) <= tol ? 1 : 0;
@@ -763,10 +839,12 @@ __global__ void src_nodes_kernel(SrcGeo S, const float4 *__restrict__ hdr, int n
 }
 
 // Surrogate (t, t*) on each shot's sub-grid [nvs][nus][nw]; where the
-// nearest node's brick failed the validity check, the table path's value
-// (node_blend: the four surrounding node tables blended at the source).
-__global__ __launch_bounds__(kThreads) void src_table_kernel(FwdArgs A, SrcGeo S, const float4 *__restrict__ hdr,
-                                                             int nshot, const float2 *__restrict__ tab,
+// nearest node's brick failed the validity check, the table builder's value
+// at the shot's own source (the FP64 straight-ray integrals the node tables
+// and the training samples come from): a node table read off-node is off by
+// the lateral variation between nodes, far more than the network.
+__global__ __launch_bounds__(kThreads) void src_table_kernel(FwdArgs A, qk::ModelDev M, SrcGeo S,
+                                                             const float4 *__restrict__ hdr, int nshot,
                                                              const unsigned char *__restrict__ mask,
                                                              float2 *out) {
   __shared__ __attribute__((aligned(16))) bf16 lw[kNF];
@@ -819,8 +897,10 @@ __global__ __launch_bounds__(kThreads) void src_table_kernel(FwdArgs A, SrcGeo S
         if (q >= total) continue;
         float2 val;
         const long long brick = (((long long)node[gg] * nbv + jv[gg] / 8) * nbu + ju[gg] / 8) * nbw + jw[gg] / 8;
-        if (mask && tab && !mask[brick]) {
-          val = node_blend(g, tab, v[gg][0], v[gg][1], jv[gg], ju[gg], jw[gg]);
+        if (mask && !mask[brick]) {
+          double t, ts;
+          qk::ray_integrals(M, v[gg][0], v[gg][1], v[gg][2], v[gg][3], v[gg][4], &t, &ts, nullptr);
+          val = make_float2((float)t, (float)ts);
         } else {
           float bt, bs, ba;
           base_leg(A.B, v[gg][0], v[gg][1], v[gg][2], v[gg][3], v[gg][4], &bt, &bs, &ba);
@@ -1134,6 +1214,9 @@ extern "C" int qpsdm_surrogate_mask(const qpsdm_surrogate *s, const qpsdm_tgeom 
   const long long total = (long long)g->nv * g->nu * g->nw * g->nnx * g->nny;
   hipLaunchKernelGGL(mask_err_kernel, dim3(grid_for((total + 63) / 64)), dim3(kThreads), 0, st, fwd_args(s), *g,
                      (const float2 *)tab, (unsigned *)err_dev);
+  const long long off = nb * kOffSrc * kCorners;
+  hipLaunchKernelGGL(mask_offnode_kernel, dim3(grid_for((off + 63) / 64)), dim3(kThreads), 0, st, fwd_args(s),
+                     s->model, *g, (unsigned *)err_dev);
   hipLaunchKernelGGL(mask_threshold_kernel, dim3(1024), dim3(256), 0, st, (const unsigned *)err_dev, nb, tol, mask);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
@@ -1149,7 +1232,7 @@ extern "C" int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_
                                           const qpsdm_shot *shots, int nshot, const float *tab,
                                           const unsigned char *mask, void *src, void *stream) {
   QPSDM_REQUIRE(s && geom && shots && src && nshot >= 0, "surrogate_src_tables: null args");
-  QPSDM_REQUIRE(!mask || tab, "surrogate_src_tables: a mask needs the FP32 node tables");
+  (void)tab;  // the fallback is computed at the source (the node tables are the judge's, qpsdm_surrogate_mask)
   SrcGeo S;
   int rc = src_geo(ip, &S);
   if (rc) return rc;
@@ -1160,8 +1243,8 @@ extern "C" int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_
   hipLaunchKernelGGL(shot_src_kernel, dim3((nshot + 63) / 64), dim3(64), 0, st, S, (const float4 *)geom, shots,
                      nshot, hdr);
   const long long total = S.stride * nshot;
-  hipLaunchKernelGGL(src_table_kernel, dim3(grid_for((total + 63) / 64)), dim3(kThreads), 0, st, fwd_args(s), S,
-                     (const float4 *)hdr, nshot, (const float2 *)tab, mask, tabs);
+  hipLaunchKernelGGL(src_table_kernel, dim3(grid_for((total + 63) / 64)), dim3(kThreads), 0, st, fwd_args(s),
+                     s->model, S, (const float4 *)hdr, nshot, mask, tabs);
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
 }

@@ -142,27 +142,25 @@ def test_src_tables_fallback_and_surrogate(gpu):
     g_d = torch.from_numpy(geom).cuda()
     nb = engine.L.lib().qpsdm_surrogate_mask_bricks(engine.C.byref(sv.tg))
     g = cfg.tgeom
-    # every brick invalid: the table path -- the four node tables around the
-    # source, each at the same relative nodes, blended bilinearly in (sx, sy)
+    # every brick invalid: the table builder at each shot's own source (FP64
+    # straight rays), against the oracle's ray integrals
     src = s.src_tables(sv, g_d, shots, tab=tab, mask=torch.zeros(nb, dtype=torch.uint8, device="cuda"))
     hdr, tabs, (nus, nvs, nw) = _src_layout(s, sv, shots.n, src)
-    t_all = tab.cpu().numpy().astype(np.float64)
+    m = O.model_layers(cfg.model["vl"], cfg.model["ql"], cfg.model["layer_dz"],
+                       (cfg.model["ax"], cfg.model["ay"], cfg.model["az"], cfg.model["asig"], cfg.model["av"],
+                        cfg.model["aq"]))
+    rng = np.random.default_rng(9)
     for k, (tr0, n, *_) in enumerate(shots.host):
         sx, sy = geom[tr0, 0], geom[tr0, 1]
         assert hdr[k, 0] == sx and hdr[k, 1] == sy
-        iu0 = int(round((hdr[k, 2] - g["u0"]) / g["du"]))
-        iv0 = int(round((hdr[k, 3] - g["v0"]) / g["dv"]))
-        fx = min(max((float(sx) - g["n0x"]) / g["ndx"], 0.0), g["nnx"] - 1.0)
-        fy = min(max((float(sy) - g["n0y"]) / g["ndy"], 0.0), g["nny"] - 1.0)
-        ix, iy = min(int(fx), g["nnx"] - 2), min(int(fy), g["nny"] - 2)
-        a, b = fx - ix, fy - iy
-        node = lambda j, i: t_all[j * g["nnx"] + i, iv0:iv0 + nvs, iu0:iu0 + nus]
-        ref = ((1 - b) * ((1 - a) * node(iy, ix) + a * node(iy, ix + 1))
-               + b * ((1 - a) * node(iy + 1, ix) + a * node(iy + 1, ix + 1)))
-        np.testing.assert_allclose(tabs[k].cpu().numpy(), ref, rtol=2e-6, atol=1e-7)
-    # the table path alone writes the same tables, bit for bit
-    src_t = sv.src_tables_nodes(g_d, shots, tab)
-    assert torch.equal(src_t, src)
+        got = tabs[k].cpu().numpy()
+        for _ in range(30):
+            iv, iu, iw = int(rng.integers(0, nvs)), int(rng.integers(0, nus)), int(rng.integers(0, nw))
+            x = np.float32(sx + hdr[k, 2] + np.float32(iu * g["du"]))
+            y = np.float32(sy + hdr[k, 3] + np.float32(iv * g["dv"]))
+            t, ts = O.ray_times(m, float(sx), float(sy), float(x), float(y), iw * g["dw"])
+            assert abs(got[iv, iu, iw, 0] - t) <= 2e-6 * t + 1e-7
+            assert abs(got[iv, iu, iw, 1] - ts) <= 2e-6 * ts + 1e-8
     # every brick valid: the surrogate at those points
     src1 = s.src_tables(sv, g_d, shots, tab=tab, mask=torch.ones(nb, dtype=torch.uint8, device="cuda"))
     hdr1, tabs1, _ = _src_layout(s, sv, shots.n, src1)
@@ -231,14 +229,13 @@ def test_surrogate_gain_over_nearest_node_tables(gpu, trained):
     geom, shots = _shots(cfg, n=5, seed=44)
     g_d = torch.from_numpy(geom).cuda()
     src_s = s.src_tables(sv, g_d, shots, tab=tab, mask=m)
-    nb = m.numel()
-    src_n = s.src_tables(sv, g_d, shots, tab=tab, mask=torch.zeros(nb, dtype=torch.uint8, device="cuda"))
     src_x = _exact_src(cfg, sv, s, g_d, shots, src_s)
     _, ts, _ = _src_layout(s, sv, shots.n, src_s)
-    _, tn, _ = _src_layout(s, sv, shots.n, src_n)
     _, tx, _ = _src_layout(s, sv, shots.n, src_x)
+    src_b = sv.src_tables_nodes(g_d, shots, tab)   # the table path: four node tables blended at the source
+    _, tb, _ = _src_layout(s, sv, shots.n, src_b)
     es = (ts - tx)[..., 0].abs()
-    en = (tn - tx)[..., 0].abs()     # the table path (node blend) at off-node sources
+    en = (tb - tx)[..., 0].abs()
     # the nearest node's own values at the same relative nodes
     hdr, _, (nus, nvs, nw) = _src_layout(s, sv, shots.n, src_s)
     g = cfg.tgeom
@@ -255,16 +252,14 @@ def test_surrogate_gain_over_nearest_node_tables(gpu, trained):
           f"{float(es.mean()) * 1e3:.4f} ms max {float(es.max()) * 1e3:.3f} ms; node blend mean "
           f"{float(en.mean()) * 1e3:.4f} ms max {float(en.max()) * 1e3:.3f} ms; nearest node mean "
           f"{eq.mean() * 1e3:.4f} ms max {eq.max() * 1e3:.3f} ms")
-    assert float(en.mean()) < eq.mean()
-    # this small model's anomaly is sharp on its domain (sigma 80 m over 200 m)
-    # and one failed point sends a whole 8^3 brick back to the table, so the
-    # gain here is a lower bound on the survey-scale one
-    assert float(es.mean()) < 0.6 * float(en.mean())
+    # the learned source leg against both table-interpolation methods (the
+    # nearest node's table, and the four surrounding node tables blended)
+    assert float(es.mean()) < 0.6 * min(float(en.mean()), eq.mean())
     traces = torch.from_numpy(np.random.default_rng(2).standard_normal((geom.shape[0], cfg.nt))
                               .astype(np.float32)).cuda()
     bank = sv.qbank(traces)
     imgs = {}
-    for name, src in (("nodes", None), ("blend", src_n), ("surrogate", src_s), ("exact", src_x)):
+    for name, src in (("nodes", None), ("blend", src_b), ("surrogate", src_s), ("exact", src_x)):
         img = sv.new_image()
         sv.image_batch(img, g_d, bank, tab, shots, src=src)
         imgs[name] = img.cpu().numpy()
@@ -272,4 +267,4 @@ def test_surrogate_gain_over_nearest_node_tables(gpu, trained):
     e_b = rel_l2(imgs["blend"], imgs["exact"])
     print(f"image rel-L2 vs exact source legs: surrogate {e_s:.2e}, node blend {e_b:.2e}, "
           f"nearest-node tables {e_n:.2e}")
-    assert e_s < 0.75 * e_n
+    assert e_s < 0.75 * min(e_n, e_b)
