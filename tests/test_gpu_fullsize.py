@@ -135,3 +135,115 @@ def test_fullsize_mixed_form_within_budget(full):
     print(f"full volume, 256 traces, rel-L2 vs FP32: {e}")
     assert e["f16_u16x"] < 1e-3
     assert e["bf16_u16x"] > e["f16_u16x"]
+
+
+def _shot_list(ranges, tans):
+    return engine.ShotList([(tr0, n) + tuple(tans) for tr0, n in ranges])
+
+
+def test_fullsize_fast_form_against_fp64_and_surrogate_gain(full):
+    """The surrogate variant at full size: a surrogate trained on the device
+    (6000 Adam steps on 8M samples), judged (0.5 ms) and used for one
+    2000-receiver shot's source table.
+    * The fast form -- FP16 bank, U16Q tables, node form below 400 m, the
+      shot's own source table -- against the FP64 oracle fed that same
+      source table, on 32 traces of the shot over the whole volume: <= 1e-3.
+    * Gain over the table methods without the network: the source leg's mean
+      |dt| against FP64 rays at the true source below 0.6x that of the
+      nearest node's table and of the four node tables blended, and the
+      whole shot's image error against the exact-source image below 0.75x
+      both."""
+    cfg, sv, tab, geom = full
+    shots = synth.shots_config2(nshots=1)
+    g1, ranges = synth.geometry(shots)
+    n = ranges[0][1]
+    tans = cfg.dip_bounds()[:4]
+    g_d = torch.from_numpy(g1).cuda()
+    x = synth.reflectivity_traces_gpu(cfg, n, seed=77)
+    sur = engine.Surrogate.for_survey(cfg, out_scale=(0.02, 0.004, 0.05))
+    X, T = sur.samples(8_000_000, seed=0x5EED5)
+    sur.train(X, T, steps=6000, batch=65536, lr0=4e-3, lr1=4e-5)
+    del X, T
+    mask, _ = sur.mask(sv.tg, tab, tol=0.5e-3)
+    sl = _shot_list(ranges, tans)
+    src = sur.src_tables(sv, g_d, sl, tab=tab, mask=mask)
+    nus, nvs, nw = sur.src_grid(sv)
+    hdr = src[:16].view(torch.float32).cpu().numpy()
+    body = src[16:].view(torch.float32).view(nvs, nus, nw, 2)
+    # fast form vs the FP64 oracle fed the same source table
+    m = 32
+    sub = np.ascontiguousarray(g1[:m])
+    svm = engine.Survey(cfg, sv.tstar_max).set_form("nodes", 400.0)
+    q = svm.quantize_tables(tab, code="u16q")
+    img = svm.new_image()
+    svm.image_batch(img, torch.from_numpy(sub).cuda(), svm.qbank(x[:m].contiguous(), dtype=torch.float16), q,
+                    _shot_list([(0, m)], tans), launch=(8, 8, 4), nwg=-1, src=src)
+    ref, _ = O.image(O.qbank(x[:m].cpu().numpy(), o_bparams(sv)), sub, tab.cpu().numpy(), o_iparams(sv),
+                     src=(hdr, body.cpu().numpy()))
+    e = rel_l2(img.cpu().numpy(), ref)
+    print(f"fast form vs FP64 oracle with its own source table: {e:.2e}")
+    assert e <= 1e-3
+    # gain over the table methods
+    exact = src.clone()
+    engine.build_tables_into(cfg.model, dict(cfg.tgeom, nnx=1, nny=1, n0x=float(hdr[0]), n0y=float(hdr[1]), nu=nus,
+                                             nv=nvs, u0=float(hdr[2]), v0=float(hdr[3])),
+                             exact[16:].view(torch.float32))
+    blend = sv.src_tables_nodes(g_d, sl, tab)
+    tx = exact[16:].view(torch.float32).view(nvs, nus, nw, 2)[..., 0]
+    g = cfg.tgeom
+    iu0 = int(round((hdr[2] - g["u0"]) / g["du"]))
+    iv0 = int(round((hdr[3] - g["v0"]) / g["dv"]))
+    i = min(max(int(np.floor((hdr[0] - g["n0x"]) / g["ndx"] + 0.5)), 0), g["nnx"] - 1)
+    j = min(max(int(np.floor((hdr[1] - g["n0y"]) / g["ndy"] + 0.5)), 0), g["nny"] - 1)
+    near = tab[j * g["nnx"] + i, iv0:iv0 + nvs, iu0:iu0 + nus, :, 0]
+    es = float((body[..., 0] - tx).abs().mean())
+    eb = float((blend[16:].view(torch.float32).view(nvs, nus, nw, 2)[..., 0] - tx).abs().mean())
+    en = float((near - tx).abs().mean())
+    print(f"source leg mean |dt|: surrogate {es * 1e3:.3f} ms, node blend {eb * 1e3:.3f} ms, nearest node "
+          f"{en * 1e3:.3f} ms; bricks falling back {100 * (1 - float(mask.float().mean())):.1f} %")
+    assert es < 0.6 * min(eb, en)
+    bank = sv.qbank(x)
+    imgs = {}
+    for name, s in (("exact", exact), ("surrogate", src), ("blend", blend), ("nearest", None)):
+        im = sv.new_image()
+        sv.image_batch(im, g_d, bank, tab, sl, src=s)
+        imgs[name] = im.double()
+    err = {k: float((v - imgs["exact"]).norm() / imgs["exact"].norm()) for k, v in imgs.items() if k != "exact"}
+    print(f"whole shot, image rel-L2 vs exact source legs: {err}")
+    assert err["surrogate"] < 0.75 * min(err["blend"], err["nearest"])
+
+
+@pytest.mark.parametrize("form", ["exact", "nodes"])
+def test_fullsize_config4_graph_replays_equal_eager(full, form):
+    """Four shot gathers of the batching variant (uneven trace counts,
+    per-shot apertures) over the full image through a captured graph of
+    [Q bank -> batched imaging]: each of three replays, with eager work and
+    synchronisations between them, equals the eager batched call bit for
+    bit; in the node form too."""
+    cfg, sv0, tab, _ = full
+    sv = engine.Survey(cfg, sv0.tstar_max).set_form(form, 400.0)
+    shots = synth.shots_config4(nshots=4, seed=0x5EED0044)
+    geom, ranges = synth.geometry(shots)
+    desc = [(tr0, n) + synth.dip_tans(a) for (tr0, n), (*_, a) in zip(ranges, shots)]
+    x = synth.reflectivity_traces_gpu(cfg, geom.shape[0], seed=91)
+    g_d = torch.from_numpy(geom).cuda()
+    q = sv.quantize_tables(tab, code="u16q")
+    sl = engine.ShotList(desc)
+    launch = (8, 8, 4) if form == "nodes" else (8, 8, 8)
+    eager = sv.new_image()
+    sv.image_batch(eager, g_d, sv.qbank(x, dtype=torch.float16), q, sl, launch=launch, nwg=-1)
+    img = sv.new_image()
+    bank = torch.empty(sv.bank_shape(x.shape[0], torch.float16), dtype=torch.float16, device="cuda")
+    gr = engine.BatchGraph(sv, x, bank, img, g_d, q, sl, launch=launch, nwg=-1, stages=5)
+    try:
+        for _ in range(3):
+            img.zero_()
+            torch.cuda.synchronize()
+            gr.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(img, eager)
+            _ = (eager * 2).sum().item()   # eager work between replays
+    finally:
+        gr.close()
+    print(f"{form}: gathers of {[n for _, n in ranges]} traces, apertures {[round(s[3], 1) for s in shots]} deg")
+    assert float(eager.abs().sum()) > 0
