@@ -119,6 +119,12 @@ typedef struct {
  * generator): straight-ray t and t* into the FP32 layout, on the device. */
 int qpsdm_tables_build(const qpsdm_model *m, const qpsdm_tgeom *g, float *tab_dev, void *stream);
 
+/* The table builder's straight-ray integrals at arbitrary points, in FP64
+ * on the device: pts [n][5] = (sx, sy, x, y, z) (source at the surface),
+ * out [n][3] = (t, t*, sigma = int v ds).  For modelling data and checking
+ * tables off their nodes. */
+int qpsdm_rays(const qpsdm_model *m, const double *pts_dev, int n, double *out_dev, void *stream);
+
 /* Size in bytes of the 16-bit brick form of a table set (QPSDM_F16B and
  * QPSDM_U16B have the same size). */
 size_t qpsdm_tables_f16b_bytes(const qpsdm_tgeom *g);

@@ -453,3 +453,28 @@ extern "C" int qpsdm_tables_read_ref(const qpsdm_tgeom *g, const qpsdm_ref_layou
   (void)hipFree(dev);
   return rc;
 }
+
+// ---- ray integrals at arbitrary points --------------------------------------
+namespace {
+
+__global__ void rays_kernel(qpsdm_dev::ModelDev m, const double *__restrict__ pts, int n, double *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double *p = pts + 5 * (size_t)i;
+    double t, ts, sg;
+    qpsdm_dev::ray_integrals(m, p[0], p[1], p[2], p[3], p[4], &t, &ts, &sg);
+    out[3 * (size_t)i] = t;
+    out[3 * (size_t)i + 1] = ts;
+    out[3 * (size_t)i + 2] = sg;
+  }
+}
+
+}  // namespace
+
+extern "C" int qpsdm_rays(const qpsdm_model *m, const double *pts, int n, double *out, void *stream) {
+  QPSDM_REQUIRE(m && (pts || n == 0) && (out || n == 0) && n >= 0, "rays: bad args");
+  if (n == 0) return QPSDM_OK;
+  hipLaunchKernelGGL(rays_kernel, dim3(std::min((n + 255) / 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     qpsdm_dev::make_model(*m), pts, n, out);
+  QPSDM_HIP(hipGetLastError());
+  return QPSDM_OK;
+}

@@ -305,6 +305,15 @@ class Survey:
                                             _ptr(src) if src is not None else None, _ptr(work), nwg,
                                             _stream(stream)))
 
+    def src_grid(self):
+        """(nus, nvs, nw) of one shot's source table: the relative sub-grid
+        that reaches the image."""
+        c = self.cfg
+        g = c.tgeom
+        nus = min(g["nu"], math.ceil((c.nx - 1) * c.dx / g["du"]) + 2)
+        nvs = min(g["nv"], math.ceil((c.ny - 1) * c.dy / g["dv"]) + 2)
+        return nus, nvs, g["nw"]
+
     def src_bytes(self, nshot):
         return L.lib().qpsdm_surrogate_src_bytes(C.byref(self.ip), nshot)
 
@@ -409,6 +418,19 @@ def build_tables_into(model, tgeom, out, stream=None):
     n = g.nnx * g.nny * g.nv * g.nu * g.nw * 2
     assert out.dtype == torch.float32 and out.numel() >= n
     check(L.lib().qpsdm_tables_build(C.byref(model_struct(model)), C.byref(g), _ptr(out), _stream(stream)))
+    return out
+
+
+def rays(model, pts, stream=None):
+    """FP64 straight-ray integrals (t, t*, sigma) [n][3] of the model from the
+    surface points (sx, sy) to (x, y, z), pts [n][5] (numpy or a float64
+    device tensor); returned as a float64 device tensor."""
+    if not isinstance(pts, torch.Tensor):
+        pts = torch.from_numpy(np.ascontiguousarray(pts, np.float64))
+    p = pts.to("cuda", torch.float64).contiguous()
+    assert p.dim() == 2 and p.shape[1] == 5, "rays: pts is [n][5] (sx, sy, x, y, z)"
+    out = torch.empty((p.shape[0], 3), dtype=torch.float64, device="cuda")
+    check(L.lib().qpsdm_rays(C.byref(model_struct(model)), _ptr(p), p.shape[0], _ptr(out), _stream(stream)))
     return out
 
 
@@ -645,8 +667,4 @@ class Surrogate:
 
     def src_grid(self, survey):
         """(nus, nvs, nw) of one shot's source table."""
-        c = survey.cfg
-        g = c.tgeom
-        nus = min(g["nu"], math.ceil((c.nx - 1) * c.dx / g["du"]) + 2)
-        nvs = min(g["nv"], math.ceil((c.ny - 1) * c.dy / g["dv"]) + 2)
-        return nus, nvs, g["nw"]
+        return survey.src_grid()

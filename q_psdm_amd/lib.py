@@ -128,6 +128,7 @@ EXPORTS = [
     "qpsdm_surrogate_eval", "qpsdm_surrogate_grad", "qpsdm_surrogate_train", "qpsdm_surrogate_mask_bricks",
     "qpsdm_surrogate_mask", "qpsdm_surrogate_src_bytes", "qpsdm_surrogate_src_tables",
     "qpsdm_src_tables_nodes", "qpsdm_copy_f32", "qpsdm_tables_write_ref", "qpsdm_tables_read_ref",
+    "qpsdm_rays",
     "qpsdm_tables_bytes", "qpsdm_tables_lookup", "qpsdm_image_batch_grid",
     "qpsdm_segy_fgethd", "qpsdm_segy_initkeyindex", "qpsdm_segy_setkeyindex", "qpsdm_segy_fgettr",
     "qpsdm_segy_sgettr", "qpsdm_segy_decode", "qpsdm_segy_decode_dev", "qpsdm_ctx_push_segy",
@@ -264,6 +265,7 @@ def lib():
         L.qpsdm_surrogate_src_bytes.restype = C.c_size_t
         L.qpsdm_surrogate_src_bytes.argtypes = [C.POINTER(ImageParams), C.c_int]
         L.qpsdm_surrogate_src_tables.argtypes = [vp, C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp, vp]
+        L.qpsdm_rays.argtypes = [C.POINTER(Model), vp, C.c_int, vp, vp]
         L.qpsdm_tables_write_ref.argtypes = [C.POINTER(TGeom), vp, C.POINTER(RefLayout), C.c_char_p]
         L.qpsdm_tables_read_ref.argtypes = [C.POINTER(TGeom), C.POINTER(RefLayout), C.c_char_p, vp]
         L.qpsdm_src_tables_nodes.argtypes = [C.POINTER(ImageParams), vp, vp, C.c_int, vp, vp, vp]

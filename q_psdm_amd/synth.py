@@ -245,6 +245,43 @@ def diffraction_traces(cfg, geom, scat, fpk=25.0, amp=1e6, chunk=512):
     return out
 
 
+def diffraction_traces_rays(cfg, geom, scat, fpk=25.0, amp=1e6, chunk=2048):
+    """Point-scatterer data through any of the table builder's models (the
+    layered + anomaly one included): per scatterer and trace, the source and
+    receiver legs' t and t* from the library's FP64 straight-ray integrals
+    (engine.rays), a Ricker of peak fpk arriving at ts + tg, attenuated by
+    exp(-w t*/2) with Kolsky dispersion about fc, scaled by 1/(Ls Lr);
+    synthesised in the frequency domain in FP64 on the GPU.  Returns a
+    float32 device tensor [ntr, nt]."""
+    import torch
+    from . import engine
+    nfft = 1 << int(math.ceil(math.log2(2 * cfg.nt)))
+    f = np.fft.rfftfreq(nfft, cfg.dt)
+    w = torch.from_numpy(2 * math.pi * f).cuda()
+    W = torch.from_numpy(ricker_spectrum(f, fpk)).cuda()
+    lnw = torch.log(torch.where(w > 0, w, torch.ones_like(w)) / (2 * math.pi * cfg.fc))
+    g = torch.as_tensor(np.asarray(geom, np.float64)).cuda()
+    ntr = g.shape[0]
+    legs = []
+    for x, y, z in scat:
+        p = torch.tensor([x, y, z], dtype=torch.float64, device="cuda").expand(ntr, 3)
+        rs = engine.rays(cfg.model, torch.cat([g[:, 0:2], p], 1))
+        rg = engine.rays(cfg.model, torch.cat([g[:, 2:4], p], 1))
+        ls = torch.sqrt((g[:, 0] - x) ** 2 + (g[:, 1] - y) ** 2 + z * z)
+        lr = torch.sqrt((g[:, 2] - x) ** 2 + (g[:, 3] - y) ** 2 + z * z)
+        legs.append((rs[:, 0] + rg[:, 0], rs[:, 1] + rg[:, 1], amp / (ls * lr)))
+    out = torch.empty((ntr, cfg.nt), dtype=torch.float32, device="cuda")
+    for a0 in range(0, ntr, chunk):
+        sl = slice(a0, min(a0 + chunk, ntr))
+        U = torch.zeros((sl.stop - sl.start, f.size), dtype=torch.complex128, device="cuda")
+        for tau, tst, amp_ in legs:
+            tau, tst = tau[sl, None], tst[sl, None]
+            ph = -w[None, :] * tau + w[None, :] * tst * lnw[None, :] / math.pi
+            U += (amp_[sl, None] * W[None, :] * torch.exp(-w[None, :] * tst / 2)) * torch.polar(torch.ones_like(ph), ph)
+        out[sl] = (torch.fft.irfft(U, nfft, dim=1) / cfg.dt)[:, :cfg.nt].float()
+    return out
+
+
 def reflectivity_traces(cfg, ntr, seed, qeff=100.0, noise=0.05, nwin=8):
     """Band-limited 5-60 Hz reflectivity, attenuated per time window with
     t* = t_window/qeff and Kolsky dispersion, plus 5 % Gaussian noise."""
