@@ -201,6 +201,42 @@ def test_getscale_and_mute_match_reference_vectors():
         assert got == pytest.approx(ref, rel=1e-6), o
 
 
+def _write_vel_su(R, path, nyv, nxv, ns, scalco):
+    """A velocity SU file written by the reference's fputtr: nyv lines of
+    nxv traces, trace (l, c) at sx = 1000 l + 10 c, sy = 7 c - l."""
+    for l in range(nyv):
+        for c in range(nxv):
+            h = bytearray(240)
+            h[70:72] = int(scalco).to_bytes(2, "little", signed=True)
+            h[72:76] = (1000 * l + 10 * c).to_bytes(4, "little", signed=True)
+            h[76:80] = (7 * c - l).to_bytes(4, "little", signed=True)
+            h[114:116] = ns.to_bytes(2, "little")
+            d = (C.c_float * ns)(*np.full(ns, 1500.0 + l + c, np.float32))
+            assert R.ref_fputtr(str(path).encode(), int(l or c), bytes(h), d) == 0
+
+
+@needs_ref
+def test_get_vel_coor_matches_reference_build(tmp_path):
+    """get_vel_coor (src/psdmpkg.cc:979-1008) against the reference's own on
+    a velocity file from its fputtr: the scalar of the first trace, nxi
+    trace coordinates from column fci of a line, a line past the last one
+    clamped to it, and a velocity grid that starts at fcv, flv."""
+    R = O.ref()
+    path = tmp_path / "vel.su"
+    nyv, nxv = 5, 9
+    _write_vel_su(R, path, nyv, nxv, 11, -100)
+    for line, fci, nxi, fcv, flv in ((0, 0, 9, 0, 0), (3, 2, 5, 0, 0), (7, 1, 4, 0, 0), (12, 14, 3, 10, 10),
+                                     (20, 11, 6, 10, 10)):
+        got, ref = (L.CoorT * nxi)(), (L.CoorT * nxi)()
+        sg, sr = C.c_short(), C.c_short()
+        L.lib().get_vel_coor(str(path).encode(), line, got, C.byref(sg), fci, nxi, fcv, nxv, flv, nyv)
+        R.ref_get_vel_coor(str(path).encode(), line, ref, C.byref(sr), fci, nxi, fcv, nxv, flv, nyv)
+        pairs = [(c.sx, c.sy) for c in got]
+        assert pairs == [(c.sx, c.sy) for c in ref] and sg.value == sr.value == -100
+        l = min(line, flv + nyv - 1) - flv
+        assert pairs == [(1000 * l + 10 * (fci - fcv + i), 7 * (fci - fcv + i) - l) for i in range(nxi)]
+
+
 def test_getpar_argument_store():
     argv = [b"prog", b"ntl=1250", b"dt=0.004", b"path=/a/b", b"ntl2=5"]
     arr = (C.c_char_p * len(argv))(*argv)
