@@ -149,9 +149,23 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
     p = min(max(p, P.nf_ps), P.nf_pe);
     pa[m] = uniform_int(p);
   }
-  // slot (m, p) holds a point when pa[m] + p < pa[m + 1] (an SGPR compare
-  // where it is used: the slots' masks are not kept)
-#define QPSDM_ON(m, p) (pa[(m)] + (p) < pa[(m) + 1])
+  // slot (m, p) holds a point when bit m PM + p of the wave-uniform word
+  // `on` is set.  Inside the trace loop the word passes through an empty
+  // asm that claims to change it at the top of every phase: each slot's
+  // test is then a scalar bit test in its own block, instead of a per-slot
+  // lane mask that the compiler would hoist out of the loop, keep live in
+  // SGPR pairs and, short of SGPRs, spill to VGPR lanes and read back with
+  // vector instructions in every iteration.
+  unsigned on = 0;
+#pragma unroll
+  for (int m = 0; m < NI; ++m)
+#pragma unroll
+    for (int p = 0; p < PM; ++p)
+      if (pa[m] + p < pa[m + 1]) on |= 1u << (m * PM + p);
+  on = (unsigned)uniform_int((int)on);
+  unsigned onl = on;
+#define QPSDM_ON(m, p) ((onl >> ((m) * PM + (p))) & 1u)
+#define QPSDM_PHASE() asm volatile("" : "+s"(onl))
   float cf_pt[NI][PM];
 #pragma unroll
   for (int m = 0; m < NI; ++m)
@@ -217,8 +231,11 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
       // Three phases as in image_tile: the weights, fractions and sample
       // indices of every point (node values computed once per node), then
       // every gather in one branch-free block, then decode and accumulate.
+      // slots that hold no point are left unset: every later read of a
+      // slot is under the same test
       float cw[NI][PM], cf[NI][PM];
       int ci[NI][PM];
+      QPSDM_PHASE();
       NodeVal n0 = node_value<NI>(P, D, R, S, 0, tab.at(c, wn[0]), zn[0], dgx, dgy, hg2, unit, tmin);
 #pragma unroll
       for (int m = 0; m < NI; ++m) {
@@ -227,14 +244,12 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
         const NodeVal d = node_sub(n1, n0);
 #pragma unroll
         for (int p = 0; p < PM; ++p) {
-          cw[m][p] = 0.0f;
-          cf[m][p] = 0.0f;
-          ci[m][p] = 0;
           if (!QPSDM_ON(m, p)) continue;
           const float s = cf_pt[m][p];
           const float f = fmaf(s, d.F, n0.F);
-          const float wx = fminf(sat(fmaf(s, d.u1, n0.u1)), sat(fmaf(s, d.u2, n0.u2)));
-          const float wy = fminf(sat(fmaf(s, d.u3, n0.u3)), sat(fmaf(s, d.u4, n0.u4)));
+          // min(sat(a), sat(b)) = sat(min(a, b)): one clamp per axis
+          const float wx = sat(fminf(fmaf(s, d.u1, n0.u1), fmaf(s, d.u2, n0.u2)));
+          const float wy = sat(fminf(fmaf(s, d.u3, n0.u3), fmaf(s, d.u4, n0.u4)));
           const float w = (wx * wy) * fmaf(s, d.a, n0.a);
           const int i0 = (int)f;
           const int j = (int)__builtin_amdgcn_fmed3f(fmaf(s, d.L, n0.L), 0.0f, jmax);
@@ -248,11 +263,13 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
         n0 = n1;
       }
       typename BankRaw<BankT>::type cb[NI][PM];
+      QPSDM_PHASE();
 #pragma unroll
       for (int m = 0; m < NI; ++m)
 #pragma unroll
         for (int p = 0; p < PM; ++p)
           if (QPSDM_ON(m, p)) cb[m][p] = bank_raw(br, ci[m][p], bank);
+      QPSDM_PHASE();
 #pragma unroll
       for (int m = 0; m < NI; ++m)
 #pragma unroll
@@ -270,6 +287,7 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
       for (int p = 0; p < PM; ++p)
         if (QPSDM_ON(m, p)) img[col + pa[m] + p] = acc[m][p];
   }
+#undef QPSDM_PHASE
 #undef QPSDM_ON
 }
 
