@@ -510,13 +510,13 @@ def kernel_family(form):
 
 
 def measure_counters(args):
-    """Three counter passes of a probe step: SQ (VALU issue), FETCH_SIZE,
-    WRITE_SIZE."""
+    """Three counter passes of a probe step: SQ (VALU issue) with the
+    texture address / data units' busy cycles, FETCH_SIZE, WRITE_SIZE."""
     tmp = tempfile.mkdtemp(prefix="qpsdm_pmc_", dir="/tmp")
     try:
         res = {}
         for tag, ctrs in (("sq", ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_BUSY_CYCLES",
-                                  "SQ_INSTS_VALU_TRANS_F32"]),
+                                  "SQ_INSTS_VALU_TRANS_F32", "TA_TA_BUSY_sum", "TD_TD_BUSY_sum"]),
                           ("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE"])):
             rows, err = counter_run(ctrs, args, tmp, tag)
             if err:
@@ -549,6 +549,12 @@ def rooflines(res, results, head, upd_per_shot, ub_per_update, nshot):
                      "transcendental_wave_instr": c.get("SQ_INSTS_VALU_TRANS_F32"),
                      "salu_wave_instr": c.get("SQ_INSTS_SALU"), "vmem_rd_wave_instr": c.get("SQ_INSTS_VMEM_RD"),
                      "clock_GHz_from_busy_cycles": c["SQ_BUSY_CYCLES"] / 32 / t / 1e9 if t > 0 else None}
+        # busy cycles of the per-CU texture addresser (TA) and texture data
+        # unit (TD, which returns every vector load's data to the lanes),
+        # on the same per-CU-cycle scale as the VALU issue fraction
+        for unit, ctr in (("ta", "TA_TA_BUSY_sum"), ("td", "TD_TD_BUSY_sum")):
+            if c.get(ctr) is not None and t > 0:
+                out[form][f"{unit}_busy_frac"] = c[ctr] / (NCU * CLOCK_GHZ * 1e9 * t)
     # HBM traffic of the headline kernel (calibrated against the copy kernel)
     traffic = None
     kname = results[head]["imaging_kernel"]
@@ -926,10 +932,11 @@ def main():
                      "hbm": hbm if value_form == head else None, "per_form": roof,
                      "imaging_ms_per_shot": R["imaging_ms_per_shot"],
                      "note": "the imaging kernels issue vector-ALU instructions at the fraction `frac` of the "
-                             "chip's ceiling (one wave64 instruction per CU per cycle, 256 CUs at 2.4 GHz): that "
-                             "is the binding resource; `hbm` gives the unique bytes a tile must read once per "
-                             "trace (FP16 bank samples, U16Q corner words, brick headers, image) per update and "
-                             "the measured HBM traffic, both far from 8 TB/s"},
+                             "chip's ceiling (one wave64 instruction per CU per cycle, 256 CUs at 2.4 GHz) while "
+                             "the per-CU texture data unit, which hands every vector load's data to the lanes, is "
+                             "busy beside it (per_form td_busy_frac): the two bind together; `hbm` gives the "
+                             "unique bytes a tile must read once per trace (FP16 bank samples, U16Q corner words, "
+                             "brick headers, image) per update and the measured HBM traffic, both far from 8 TB/s"},
         "forms": results,
         "speedup_vs_fp32": (results[head]["value"] / results["fp32"]["value"]) if "fp32" in results else None,
         "image_rel_l2_vs_fp64": acc,
