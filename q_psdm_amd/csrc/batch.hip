@@ -95,7 +95,7 @@ __global__ void reset_kernel(unsigned *c, int n) {
 // over the shots in order and each shot's traces in order, exactly as
 // consecutive per-shot launches.  Held to four waves per SIMD (128 VGPRs):
 // uncapped, the tile loop around the body takes 134 and three waves fit.
-template <int ZP, typename BankT, typename Tab>
+template <int ZP, typename BankT, typename Tab, bool SRC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const float4 *__restrict__ geom,
                         const BankT *__restrict__ bank, ShotRuns runs, int shot, unsigned *counter, int ncol,
@@ -110,7 +110,7 @@ void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const fl
     if (tile >= total) break;
     // depth group slowest: consecutive tiles are neighbouring columns of one
     // depth slab, which share table and bank lines in L2
-    qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
+    qk::image_tile<ZP, SRC, false>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
   }
 }
 
@@ -119,25 +119,26 @@ void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const fl
 // the kernel: its registers are the per-shot kernel's, with no spills), so
 // per image point the sum still runs over the shots in order and each
 // shot's traces in order.  Scheduled by the hardware dispatcher.
-template <int ZP, typename BankT, typename Tab>
+// SRC: the shots carry source tables; RANGE: depth groups of [zb, ze).
+template <int ZP, typename BankT, typename Tab, bool SRC, bool RANGE>
 __global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                                const float4 *__restrict__ geom,
                                                                const BankT *__restrict__ bank, ShotRuns runs,
                                                                int shot, int ncol) {
   const int tile = (int)blockIdx.x;
   const OneShot one{runs(shot)};
-  qk::image_tile<ZP>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
+  qk::image_tile<ZP, SRC, RANGE>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
 }
 
 // Node form of one shot (weight_nodes = 1): one work-group per node tile
 // (blockIdx.x = column tile, blockIdx.y = node group).
-template <int NI, int PM, typename BankT, typename Tab>
+template <int NI, int PM, typename BankT, typename Tab, bool SRC>
 __global__ __launch_bounds__(64) void image_batch_nodes_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                                const float4 *__restrict__ geom,
                                                                const BankT *__restrict__ bank, ShotRuns runs,
                                                                int shot) {
   const OneShot one{runs(shot)};
-  qk::image_tile_nodes<NI, PM>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, one, 1);
+  qk::image_tile_nodes<NI, PM, SRC>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, one, 1);
 }
 
 template <int ZP>
@@ -164,17 +165,17 @@ struct Go {
   int *nwg_out;
 };
 
-template <int ZP, typename BankT, typename Tab>
+template <int ZP, bool SRC, typename BankT, typename Tab>
 int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
-  auto kern = image_batch_kernel<ZP, BankT, Tab>;
+  auto kern = image_batch_kernel<ZP, BankT, Tab, SRC>;
   const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
   const int nzg = (P.ze - P.zb + ZP - 1) / ZP;
   if (go.nwg == -1) {
     if (go.nwg_out) *go.nwg_out = -1;
     if (!go.run) return QPSDM_OK;
     for (int r = 0; r < go.nshot; ++r) {
-      hipLaunchKernelGGL((image_batch_tiles_kernel<ZP, BankT, Tab>), dim3(ncol * nzg), dim3(64), 0, go.s, P, tab,
-                         go.img, go.geom, bank, go.runs, r, ncol);
+      hipLaunchKernelGGL((image_batch_tiles_kernel<ZP, BankT, Tab, SRC, false>), dim3(ncol * nzg), dim3(64), 0, go.s,
+                         P, tab, go.img, go.geom, bank, go.runs, r, ncol);
       QPSDM_HIP(hipGetLastError());
     }
     return QPSDM_OK;
@@ -202,7 +203,7 @@ int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
 // Node form of every shot, in shot order: per shot the exact body (one
 // work-group per tile of 8 points) above the node range and beyond the
 // tables' depth range, then the node body (one work-group per node tile).
-template <int NI, int PM, typename BankT, typename Tab>
+template <int NI, int PM, bool SRC, typename BankT, typename Tab>
 int launch_batch_nodes_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   if (go.nwg_out) *go.nwg_out = -1;
   if (!go.run) return QPSDM_OK;
@@ -214,17 +215,17 @@ int launch_batch_nodes_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank,
     if (n1 > 0) {
       E.zb = 0;
       E.ze = P.nf_ps;
-      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab>), dim3(ncol * n1), dim3(64), 0, go.s, E, tab,
-                         go.img, go.geom, bank, go.runs, r, ncol);
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n1), dim3(64), 0, go.s,
+                         E, tab, go.img, go.geom, bank, go.runs, r, ncol);
     }
     if (n2 > 0) {
       E.zb = P.nf_pe;
       E.ze = P.nz;
-      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab>), dim3(ncol * n2), dim3(64), 0, go.s, E, tab,
-                         go.img, go.geom, bank, go.runs, r, ncol);
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n2), dim3(64), 0, go.s,
+                         E, tab, go.img, go.geom, bank, go.runs, r, ncol);
     }
-    hipLaunchKernelGGL((image_batch_nodes_kernel<NI, PM, BankT, Tab>), dim3(ncol, P.nf_ngroups), dim3(64), 0, go.s,
-                       P, tab, go.img, go.geom, bank, go.runs, r);
+    hipLaunchKernelGGL((image_batch_nodes_kernel<NI, PM, BankT, Tab, SRC>), dim3(ncol, P.nf_ngroups), dim3(64), 0,
+                       go.s, P, tab, go.img, go.geom, bank, go.runs, r);
     QPSDM_HIP(hipGetLastError());
   }
   return QPSDM_OK;
@@ -238,13 +239,19 @@ int launch_batch_nodes(const qk::ImgDev &P, const Tab &tab, const BankT *bank, c
     return qpsdm::set_error(QPSDM_EINVAL,
                             "image_batch: the node form takes FP32 or FP16 banks and FP32 or U16Q tables");
   } else {
+    const bool src = go.runs.src_hdr != nullptr;
     switch (P.tile_z * 4 + P.nf_pm) {
-      case 2 * 4 + 2: return launch_batch_nodes_t<2, 2>(P, tab, bank, go);
-      case 2 * 4 + 3: return launch_batch_nodes_t<2, 3>(P, tab, bank, go);
-      case 3 * 4 + 2: return launch_batch_nodes_t<3, 2>(P, tab, bank, go);
-      case 3 * 4 + 3: return launch_batch_nodes_t<3, 3>(P, tab, bank, go);
-      case 4 * 4 + 2: return launch_batch_nodes_t<4, 2>(P, tab, bank, go);
-      case 4 * 4 + 3: return launch_batch_nodes_t<4, 3>(P, tab, bank, go);
+#define QPSDM_NODES(ni, pm)                                                                  \
+  case ni * 4 + pm:                                                                          \
+    return src ? launch_batch_nodes_t<ni, pm, true>(P, tab, bank, go)                        \
+               : launch_batch_nodes_t<ni, pm, false>(P, tab, bank, go);
+      QPSDM_NODES(2, 2)
+      QPSDM_NODES(2, 3)
+      QPSDM_NODES(3, 2)
+      QPSDM_NODES(3, 3)
+      QPSDM_NODES(4, 2)
+      QPSDM_NODES(4, 3)
+#undef QPSDM_NODES
       default:
         return qpsdm::set_error(QPSDM_EINVAL, "image_batch: node form with %d intervals x %d points per tile",
                                 P.tile_z, P.nf_pm);
@@ -255,12 +262,16 @@ int launch_batch_nodes(const qk::ImgDev &P, const Tab &tab, const BankT *bank, c
 template <typename BankT, typename Tab>
 int launch_batch_tz(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   if (P.nf_ngroups > 0) return launch_batch_nodes(P, tab, bank, go);
+  const bool src = go.runs.src_hdr != nullptr;
   switch (P.tile_z) {
-    case 2: return launch_batch_t<2>(P, tab, bank, go);
-    case 3: return launch_batch_t<3>(P, tab, bank, go);
-    case 4: return launch_batch_t<4>(P, tab, bank, go);
-    case 6: return launch_batch_t<6>(P, tab, bank, go);
-    case 8: return launch_batch_t<8>(P, tab, bank, go);
+#define QPSDM_TZ(zp) \
+  case zp: return src ? launch_batch_t<zp, true>(P, tab, bank, go) : launch_batch_t<zp, false>(P, tab, bank, go);
+    QPSDM_TZ(2)
+    QPSDM_TZ(3)
+    QPSDM_TZ(4)
+    QPSDM_TZ(6)
+    QPSDM_TZ(8)
+#undef QPSDM_TZ
     default:
       return qpsdm::set_error(QPSDM_EINVAL, "image_batch: unsupported tile_z=%d (2, 3, 4, 6, 8)", P.tile_z);
   }

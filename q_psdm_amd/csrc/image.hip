@@ -80,11 +80,12 @@ inline OneRun one_run(const qk::ImgDev &P) {
 }
 
 // One wave per tile: blockIdx.x = column tile, blockIdx.y = depth group.
-template <int ZP, typename BankT, typename Tab>
+// RANGE: the depth groups of [zb, ze) (the node form's exact parts).
+template <int ZP, typename BankT, typename Tab, bool RANGE>
 __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                    const float4 *__restrict__ geom,
                                                    const BankT *__restrict__ bank, OneRun run) {
-  qk::image_tile<ZP>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
+  qk::image_tile<ZP, false, RANGE>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
 }
 
 // Node form: blockIdx.x = column tile, blockIdx.y = node group.
@@ -92,7 +93,7 @@ template <int NI, int PM, typename BankT, typename Tab>
 __global__ __launch_bounds__(64) void image_nodes_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                          const float4 *__restrict__ geom,
                                                          const BankT *__restrict__ bank, OneRun run) {
-  qk::image_tile_nodes<NI, PM>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
+  qk::image_tile_nodes<NI, PM, false>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
 }
 
 template <int ZP>
@@ -110,13 +111,13 @@ __global__ void scale_kernel(float *p, size_t n, float s) {
     p[i] *= s;
 }
 
-template <int ZP, typename BankT, typename Tab>
+template <int ZP, bool RANGE = false, typename BankT, typename Tab>
 int launch_t(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *geom, const BankT *bank,
              hipStream_t s) {
   const int nxt = (P.nx + P.tile_x - 1) / P.tile_x, nyt = (P.ny + P.tile_y - 1) / P.tile_y;
   if (P.ze <= P.zb) return QPSDM_OK;
   const dim3 grid(nxt * nyt, (P.ze - P.zb + ZP - 1) / ZP);
-  hipLaunchKernelGGL((image_kernel<ZP, BankT, Tab>), grid, dim3(64), 0, s, P, tab, img, geom, bank,
+  hipLaunchKernelGGL((image_kernel<ZP, BankT, Tab, RANGE>), grid, dim3(64), 0, s, P, tab, img, geom, bank,
                      one_run(P));
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
@@ -170,10 +171,10 @@ int launch_form(const qk::ImgDev &P, const Tab &tab, float *img, const float4 *g
     E.tile_z = 8;
     E.zb = 0;
     E.ze = P.nf_ps;
-    int rc = launch_t<8>(E, tab, img, geom, bank, s);
+    int rc = launch_t<8, true>(E, tab, img, geom, bank, s);
     E.zb = P.nf_pe;
     E.ze = P.nz;
-    if (!rc) rc = launch_t<8>(E, tab, img, geom, bank, s);
+    if (!rc) rc = launch_t<8, true>(E, tab, img, geom, bank, s);
     if (rc) return rc;
     switch (P.tile_z * 4 + P.nf_pm) {
       case 2 * 4 + 2: return launch_nodes_t<2, 2>(P, tab, img, geom, bank, s);

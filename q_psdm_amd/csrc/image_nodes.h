@@ -122,7 +122,8 @@ __device__ inline float uniform_float(float v) {
 
 // Node-form tile body: tile_col as in image_tile; group g owns intervals
 // [nf_ws + g NI, nf_ws + (g + 1) NI) of the node range.
-template <int NI, int PM, typename BankT, typename Tab, typename Runs>
+// SRC as in image_tile.
+template <int NI, int PM, bool SRC, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *__restrict__ img,
                                         const float4 *__restrict__ geom, const BankT *__restrict__ bank,
                                         int tile_col, int group, const Runs &runs, int nrun) {
@@ -204,7 +205,7 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
       cache_run = r;
       sx = gs.x;
       sy = gs.y;
-      if (run.src) {
+      if constexpr (SRC) {
         TabF32 st;
         st.p = run.src;
         st.nstride = (size_t)run.snv * run.snu * P.nw;
@@ -221,7 +222,9 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
     };
     float4 g;
     for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
-      if (foreign_source(run, g)) continue;
+      if constexpr (SRC) {
+        if (foreign_source(run, g)) continue;
+      }
       set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;

@@ -40,8 +40,7 @@ struct Run {
 // A trace of a source-table run whose source is not the table's own
 // (compared bitwise, on the scalar unit).
 __device__ inline bool foreign_source(const Run &run, const float4 &g) {
-  return run.src && ((__float_as_int(g.x) != __float_as_int(run.ssx)) |
-                     (__float_as_int(g.y) != __float_as_int(run.ssy)));
+  return (__float_as_int(g.x) != __float_as_int(run.ssx)) | (__float_as_int(g.y) != __float_as_int(run.ssy));
 }
 
 // Source-leg cache of the tile points, per source position: t in samples,
@@ -114,7 +113,13 @@ __device__ inline float dip_taper(const Dip &D, const Ramps &R, float X, float Y
   return wx * wy;
 }
 
-template <int ZP, typename BankT, typename Tab, typename Runs>
+// SRC: every run carries its own source table (Run::src), and traces of
+// another source are skipped; without it the source leg comes from the
+// node tables.  RANGE: the tile's depth groups start at ImgDev::zb and stop
+// at ze (the node form's exact parts); without it they cover [0, nz).  Both
+// are compile-time so that the common kernels carry neither the registers
+// nor the per-trace tests (they cost the exact body 4-14 % in time).
+template <int ZP, bool SRC, bool RANGE, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__restrict__ img,
                                   const float4 *__restrict__ geom, const BankT *__restrict__ bank,
                                   int tile_col, int zgroup, const Runs &runs, int nrun) {
@@ -122,8 +127,8 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
   const int ntx = (P.nx + P.tile_x - 1) / P.tile_x;
   const int tx0 = (tile_col % ntx) * P.tile_x;
   const int ty0 = (tile_col / ntx) * P.tile_y;
-  const int z0 = P.zb + zgroup * ZP;
-  const int nzv = min(ZP, P.ze - z0);
+  const int z0 = (RANGE ? P.zb : 0) + zgroup * ZP;
+  const int nzv = min(ZP, (RANGE ? P.ze : P.nz) - z0);
   const int cx = tx0 + lane % P.tile_x, cy = ty0 + lane / P.tile_x;
   const bool valid = cx < P.nx && cy < P.ny;
   const float x = (float)cx * P.dx, y = (float)cy * P.dy;
@@ -165,7 +170,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       cache_run = r;
       sx = gs.x;
       sy = gs.y;
-      if (run.src) {
+      if constexpr (SRC) {
         TabF32 st;
         st.p = run.src;
         st.nstride = (size_t)run.snv * run.snu * P.nw;
@@ -182,7 +187,9 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
     };
     float4 g;
     for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
-      if (foreign_source(run, g)) continue;
+      if constexpr (SRC) {
+        if (foreign_source(run, g)) continue;
+      }
       set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;
