@@ -832,7 +832,7 @@ def main():
     ap.add_argument("--oracle-traces", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-traces", type=int, default=256)
-    ap.add_argument("--cpu-config1-shots", type=int, default=2)
+    ap.add_argument("--cpu-config1-shots", type=int, default=16)
     ap.add_argument("--config4-shots", type=int, default=50)
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip config 1, config 4, SEG-Y and end-to-end")

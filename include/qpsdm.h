@@ -341,8 +341,8 @@ int qpsdm_count_shot(const qpsdm_image_params *p, const float *geom_dev, int tr0
 /* ---------------------------------------------------------------------- */
 /* Batched imaging (no counterpart in the reference, which launches once   */
 /* per trace): shot gathers of uneven sizes with per-shot dip apertures    */
-/* through one persistent kernel; tiles are owned by one work-group across */
-/* all shots (fixed summation order, no image atomics).                    */
+/* through one call, one dispatch per shot in shot order; a tile is owned  */
+/* by one work-group per shot (fixed summation order, no image atomics).   */
 /* ---------------------------------------------------------------------- */
 typedef struct {
   int tr0, ntr;                      /* geom/bank rows of the shot          */
