@@ -396,15 +396,27 @@ struct LayerBufs {
   __amdgpu_buffer_rsrc_t rZ, rD;
 };
 
-template <bool PACK>
-__device__ inline void bwd_epilogue(const f32x16 &c, int mb, const LayerBufs &lb, Acts &out, int n, int p) {
+// The pre-activations of one 32-row block for the lane's point p, loaded
+// before the block's MFMAs so that their latency hides behind them.
+__device__ inline void bwd_load_z(float (&z)[16], int mb, const LayerBufs &lb, int n, int p) {
   const int h = (threadIdx.x & 63) >> 5;
   const int vp = 4 * h * n + p;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int row = 32 * mb + (i >> 2) * 8 + (i & 3);
-    const float z = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(lb.rZ, vp * 4, row * n * 4, 0));
-    const float dz = c[i] * dsilu(z);
+    z[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(lb.rZ, vp * 4, row * n * 4, 0));
+  }
+}
+
+template <bool PACK>
+__device__ inline void bwd_epilogue(const f32x16 &c, const float (&z)[16], int mb, const LayerBufs &lb, Acts &out,
+                                    int n, int p) {
+  const int h = (threadIdx.x & 63) >> 5;
+  const int vp = 4 * h * n + p;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = 32 * mb + (i >> 2) * 8 + (i & 3);
+    const float dz = c[i] * dsilu(z[i]);
     const bf16 db = (bf16)dz;
     __builtin_amdgcn_raw_buffer_store_b16(bf16_bits(db), lb.rD, vp * 2, row * n * 2, 0);
     if (PACK) out.k[2 * mb + (i >> 3)][i & 7] = db;
@@ -417,6 +429,9 @@ __device__ inline void bwd_hidden(const bf16 *Wt, const Acts (&in)[2], Acts (&ou
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int mb = 0; mb < kMB; ++mb) {
+    float z0[16], z1[16];
+    bwd_load_z(z0, mb, lb, n, p0 + (lane & 31));
+    bwd_load_z(z1, mb, lb, n, p0 + 32 + (lane & 31));
     f32x16 c0 = {}, c1 = {};
 #pragma unroll
     for (int s = 0; s < kKS; ++s) {
@@ -424,8 +439,8 @@ __device__ inline void bwd_hidden(const bf16 *Wt, const Acts (&in)[2], Acts (&ou
       c0 = mfma(w, in[0].k[s], c0);
       c1 = mfma(w, in[1].k[s], c1);
     }
-    bwd_epilogue<PACK>(c0, mb, lb, out[0], n, p0 + (lane & 31));
-    bwd_epilogue<PACK>(c1, mb, lb, out[1], n, p0 + 32 + (lane & 31));
+    bwd_epilogue<PACK>(c0, z0, mb, lb, out[0], n, p0 + (lane & 31));
+    bwd_epilogue<PACK>(c1, z1, mb, lb, out[1], n, p0 + 32 + (lane & 31));
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -459,12 +474,15 @@ __global__ __launch_bounds__(kThreads) void train_bwd_kernel(const bf16 *wb, int
     Acts a[2], b[2];
 #pragma unroll
     for (int mb = 0; mb < kMB; ++mb) {
+      float z0[16], z1[16];
+      bwd_load_z(z0, mb, lb[3], n, p0 + (lane & 31));
+      bwd_load_z(z1, mb, lb[3], n, p0 + 32 + (lane & 31));
       const bf16x8 w = ld8(lw + bW5 + mb * kChunk + lane * 8);
       f32x16 c0 = {}, c1 = {};
       c0 = mfma(w, gb[0], c0);
       c1 = mfma(w, gb[1], c1);
-      bwd_epilogue<true>(c0, mb, lb[3], a[0], n, p0 + (lane & 31));
-      bwd_epilogue<true>(c1, mb, lb[3], a[1], n, p0 + 32 + (lane & 31));
+      bwd_epilogue<true>(c0, z0, mb, lb[3], a[0], n, p0 + (lane & 31));
+      bwd_epilogue<true>(c1, z1, mb, lb[3], a[1], n, p0 + 32 + (lane & 31));
       __builtin_amdgcn_sched_barrier(0);
     }
     bwd_hidden<true>(lw + bW4, a, b, lb[2], n, p0);
