@@ -910,16 +910,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fast": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables as "
-                          "corner quads; bf16-mfma surrogate source tables",
-                  "mixed": "f32 accumulate; f16 bank (per-trace power-of-two scale); u16 offset+step tables as "
-                           "corner quads",
-                  "fp32": "f32"}[value_form],
+        "dtype": "f32",
         "data": "synthetic: seeded layered + anomaly model (layer Q ~ U[30,200]), jittered geometry, band-limited "
                 "reflectivity traces with 5% noise",
         "config": {"workload": "401x401x301 image @10 m; 2000 receivers/shot over 4x4 km; nt=2001 @2 ms; K=32; "
                                "25 m tables on 5x5 nodes; layered+anomaly model with randomized Q",
                    "form": value_form, "sum_form": R["form"], "z_exact": R["z_exact"],
+                   "storage": {"fast": "f16 bank (per-trace power-of-two scale), u16 offset+step tables as "
+                                       "corner quads, f32 per-shot source tables from the bf16-mfma surrogate",
+                               "mixed": "f16 bank (per-trace power-of-two scale), u16 offset+step tables as "
+                                        "corner quads",
+                               "fp32": "f32 bank, f32 tables"}[value_form],
                    "shots_per_step": args.shots_per_step, "tile": R["launch"],
                    "updates_per_step": R["updates_per_shot"] * args.shots_per_step},
         "value_meets_1e-3_vs_fp64": acc.get(f"{value_form}_vs_fp64", 0.0) <= 1e-3,
