@@ -315,6 +315,15 @@ def accuracy_vs_fp64(W, args, forms, shapes, sur, mask, torch, engine):
         img = sv.new_image()
         sv.image_batch(img, g_d, sv.qbank(x), W.tab, shots, launch=shapes["mixed"][0], nwg=-1)
         out["node_form_fp32_storage_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_nodes)
+    if "mixed" in forms:
+        # config 3 as specified (BF16 bank, FP16 codes in 8^3 bricks under an
+        # FP32 offset and scale), exact form, per shot: why the mixed form
+        # stores the bank in FP16 and the tables as 16-bit fixed point
+        f16b = W.sv.quantize_tables(W.tab, code="f16")
+        img = W.sv.new_image()
+        W.sv.image_shot(img, g_d, 0, n, W.sv.qbank(x, dtype=torch.bfloat16), f16b, launch=shapes["fp32"][0])
+        out["config3_bf16_bank_f16b_tables_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_nodes)
+        del f16b
     out["subset"] = f"shot 0, first {n} traces, full 401x401x301 volume"
     log(f"accuracy vs FP64 oracle: {out}")
     return out
