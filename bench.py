@@ -931,7 +931,8 @@ def main():
                                         "corner quads",
                                "fp32": "f32 bank, f32 tables"}[value_form],
                    "shots_per_step": args.shots_per_step, "tile": R["launch"],
-                   "updates_per_step": R["updates_per_shot"] * args.shots_per_step},
+                   "updates_per_step": R["updates_per_shot"] * args.shots_per_step,
+                   "dense_updates_per_step": W.cfg.nx * W.cfg.ny * W.cfg.nz * int(W.steps[0]["traces"].shape[0])},
         "value_meets_1e-3_vs_fp64": acc.get(f"{value_form}_vs_fp64", 0.0) <= 1e-3,
         "roofline": {"bound": "valu", "kernel": R["imaging_kernel"],
                      "achieved": (hr.get("valu_wave_instr", 0) * 64 / (R["imaging_ms_per_shot"] * 1e-3) / 1e12)
