@@ -267,6 +267,14 @@ int ref_geo(const qpsdm_tgeom *g, const qpsdm_ref_layout *L, RefGeo *o) {
                     g->ndy == rintf(g->ndy),
                 "tables_ref: node spacing must be whole metres below 65536 (16-bit source-grid steps)");
   QPSDM_REQUIRE(g->nnx <= 65535 && g->nny <= 65535, "tables_ref: too many nodes for 16-bit source grids");
+  // the layout's lattice is the node tables' own: every payload value is a
+  // table entry, not an interpolation
+  QPSDM_REQUIRE(L->dx == g->du && L->dy == g->dv && L->dz == g->dw,
+                "tables_ref: layout spacing %g x %g x %g is not the tables' %g x %g x %g", L->dx, L->dy, L->dz, g->du,
+                g->dv, g->dw);
+  const double fu = ((double)L->x0 - g->n0x - g->u0) / g->du, fv = ((double)L->y0 - g->n0y - g->v0) / g->dv;
+  QPSDM_REQUIRE(fabs(fu - rint(fu)) < 1e-3 && fabs(fv - rint(fv)) < 1e-3,
+                "tables_ref: layout origin (%g, %g) is not on the tables' relative lattice", L->x0, L->y0);
   o->g = *g;
   o->L = *L;
   o->nxg = (L->ncol + L->nxi0 - 1) / L->nxi0;
@@ -366,6 +374,7 @@ extern "C" int qpsdm_tables_write_ref(const qpsdm_tgeom *g, const float *tab, co
       e.dys = (uint16_t)g->ndy;
       e.dxs = (uint16_t)g->ndx;
     }
+  QPSDM_HIP(hipDeviceSynchronize());  // the tables' producers (any stream) are done
   float *dev = nullptr;
   QPSDM_HIP(hipMalloc(&dev, sizeof(float) * R.per_line));
   std::vector<float> host(R.per_line);
@@ -376,7 +385,8 @@ extern "C" int qpsdm_tables_write_ref(const qpsdm_tgeom *g, const float *tab, co
     for (int l = 0; l < L->nlines && !rc; ++l) {
       const int grid = (int)std::min<long long>((R.per_line + 255) / 256, 65536);
       hipLaunchKernelGGL(ref_pack_kernel, dim3(grid), dim3(256), 0, 0, R, (const float2 *)tab, l, comp, dev);
-      if (hipMemcpy(host.data(), dev, sizeof(float) * R.per_line, hipMemcpyDeviceToHost) != hipSuccess) {
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpy(host.data(), dev, sizeof(float) * R.per_line, hipMemcpyDeviceToHost) != hipSuccess) {
         rc = qpsdm::set_error(QPSDM_EHIP, "tables_write_ref: copy of line %d failed", L->line1 + l);
         break;
       }
@@ -424,6 +434,7 @@ extern "C" int qpsdm_tables_read_ref(const qpsdm_tgeom *g, const qpsdm_ref_layou
                                 g->ndy, g->ndx);
   }
   const size_t nfl = (size_t)L->nlines * R.per_line;
+  QPSDM_HIP(hipDeviceSynchronize());  // no earlier work on any stream still uses tab
   float *dev = nullptr;
   QPSDM_HIP(hipMalloc(&dev, sizeof(float) * nfl));
   std::vector<float> host(R.per_line);

@@ -339,3 +339,24 @@ def test_image_writer_headers_and_bytes(tmp_path):
     # same header bytes as the reference's golden output file for its first trace
     gold = open(os.path.join(GOLD, "L350_first3.su"), "rb").read()
     assert gold[28:32] == raw[28:32] and gold[114:118][:2] != b""
+
+
+def test_ref_layout_must_be_the_table_lattice(tmp_path):
+    """The reference-layout writer and reader take only the node tables' own
+    lattice (spacing and origin), so every payload value is a table entry;
+    anything else is refused before any device work."""
+    from q_psdm_amd import synth
+    cfg = synth.tiny_layers()
+    tg = engine.tgeom_struct(cfg.tgeom)
+    sv_layout = engine.Survey(cfg, 0.01).ref_layout(nxi0=8, nzi0=4)
+    lib = L.lib()
+    d = str(tmp_path).encode()
+    bad_spacing = L.RefLayout(*[getattr(sv_layout, f) for f, _ in L.RefLayout._fields_])
+    bad_spacing.dx = cfg.tgeom["du"] / 2
+    bad_origin = L.RefLayout(*[getattr(sv_layout, f) for f, _ in L.RefLayout._fields_])
+    bad_origin.x0 += 3.0
+    for lay in (bad_spacing, bad_origin):
+        assert lib.qpsdm_tables_write_ref(C.byref(tg), None, C.byref(lay), d) < 0
+        assert "lattice" in lib.qpsdm_last_error().decode() or "spacing" in lib.qpsdm_last_error().decode()
+        assert lib.qpsdm_tables_read_ref(C.byref(tg), C.byref(lay), d, None) < 0
+    assert os.listdir(tmp_path) == []
