@@ -1250,7 +1250,7 @@ extern "C" int qpsdm_surrogate_src_tables(const qpsdm_surrogate *s, const qpsdm_
                                           const qpsdm_shot *shots, int nshot, const float *tab,
                                           const unsigned char *mask, void *src, void *stream) {
   QPSDM_REQUIRE(s && geom && shots && src && nshot >= 0, "surrogate_src_tables: null args");
-  (void)tab;  // the fallback is computed at the source (the node tables are the judge's, qpsdm_surrogate_mask)
+  (void)tab;  // the fallback is computed at the source; the node tables serve qpsdm_surrogate_mask
   SrcGeo S;
   int rc = src_geo(ip, &S);
   if (rc) return rc;
