@@ -253,10 +253,12 @@ def _lossless(model):
     return dict(model, q0=1e12, ql=[1e12] * len(model["ql"]))
 
 
-def test_config2_rays_match_oracle(gpu):
+@pytest.mark.parametrize("mk", [synth.config1, synth.config2])
+def test_config2_rays_match_oracle(gpu, mk):
     """The device ray integrals (qpsdm_rays) against the oracle's FP64 rays
-    through config 2's layers and anomaly."""
-    cfg = synth.config2()
+    through config 1's v(z) (closed form) and config 2's layers and
+    anomaly (quadrature)."""
+    cfg = mk()
     rng = np.random.default_rng(12)
     pts = np.stack([rng.uniform(0, 4000, 64), rng.uniform(0, 4000, 64), rng.uniform(0, 4000, 64),
                     rng.uniform(0, 4000, 64), rng.uniform(10, 3000, 64)], 1)
@@ -264,7 +266,9 @@ def test_config2_rays_match_oracle(gpu):
     m = o_model(cfg.model)
     for i, (sx, sy, x, y, z) in enumerate(pts):
         t, ts = O.ray_times(m, sx, sy, x, y, z)
+        sg = O.ray_sigma(m, sx, sy, x, y, z)
         assert abs(got[i, 0] - t) <= 1e-11 * t and abs(got[i, 1] - ts) <= 1e-11 * ts, i
+        assert abs(got[i, 2] - sg) <= 1e-11 * sg, i
 
 
 def _exact_src(sv, model, g_d, sl, tab):
