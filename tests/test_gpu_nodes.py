@@ -144,3 +144,31 @@ def test_node_form_with_source_tables_matches_oracle(gpu):
         ref += r
     got = img.cpu().numpy()
     assert rel_l2(got, ref) < 1e-5 and float(np.abs(got).sum()) > 0
+
+
+@pytest.mark.parametrize("dw, nw", [(10.0, 31), (15.0, 21), (30.0, 11)])
+def test_node_form_across_node_spacings(gpu, dw, nw):
+    """Table depth nodes at 1, 1.5 and 3 image depths (1, 1-2 and 3 points
+    per interval): the node form on the GPU against the oracle's, per shot
+    and batched, with the node range from 60 m down."""
+    cfg = synth.tiny()
+    cfg.tgeom = dict(cfg.tgeom, dw=dw, nw=nw)
+    probe = engine.Survey(cfg, 0.01)
+    tab = probe.build_tables()
+    sv = engine.Survey(cfg, engine.tstar_max_from_tables(tab)).set_form("nodes", 60.0)
+    geom, ranges = synth.geometry(synth.shots_tiny(2, 7))
+    rng = np.random.default_rng(25)
+    bank = sv.qbank(torch.from_numpy(rng.standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)).cuda())
+    g = torch.from_numpy(geom).cuda()
+    ref, _ = O.image_nodes(bank.cpu().numpy(), geom, tab.cpu().numpy(), o_iparams(sv), 60.0)
+    tans = cfg.dip_bounds()[:4]
+    for ni in (2, 3, 4):
+        img = sv.new_image()
+        for tr0, n in ranges:
+            sv.image_shot(img, g, tr0, n, bank[tr0:tr0 + n], tab, launch=(8, 8, ni))
+        img2 = sv.new_image()
+        sv.image_batch(img2, g, bank, tab, engine.ShotList([(tr0, n) + tuple(tans) for tr0, n in ranges]),
+                       launch=(8, 8, ni), nwg=-1)
+        torch.cuda.synchronize()
+        assert rel_l2(img.cpu().numpy(), ref) < 1e-5, (dw, ni)
+        assert torch.equal(img, img2), (dw, ni)
