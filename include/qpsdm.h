@@ -44,6 +44,13 @@ extern "C" {
                         the four bilinear corners' codes of a cell as one 16-byte word
                         (3.1x the U16X bytes; one vector load per table node instead of
                         four) */
+#define QPSDM_F32P 7 /* bank only: pair bank, sample i of a level row stored with its forward
+                        difference as float2 (b[i], b[i+1] - b[i]); entries 0 and nt-1 of
+                        every row are (0, 0) (outside the samples the sum may pick);
+                        [ntr][K][nt] float2 */
+#define QPSDM_F16P 8 /* bank only: the pair bank in FP16 under a power-of-two scale per
+                        trace: a 16-byte header {1/scale, scale, 0, 0} then [K][nt] half2,
+                        rows qpsdm_bank_row_bytes() apart */
 
 /* Two-window spectral set-up of one offset class; replaces the window block
  * of qpsdm() (src/q-psdm.cu:1593-1645) and the f3 -> nf3 conversion
@@ -233,10 +240,13 @@ int qpsdm_trace_spectra(const qpsdm_bank_params *p, const float *traces_dev, int
  * QPSDM_BF16 ([ntr][K][nt]) or QPSDM_F16 (per trace: a 16-byte header
  * {1/scale, scale, 0, 0} then [K][nt] samples times scale = 2^(15 - ex), ex
  * the binary exponent of a bound on the trace's largest |sample| over all
- * levels taken from its spectrum, so every stored sample is below 2^15). */
+ * levels taken from its spectrum, so every stored sample is below 2^15), or
+ * the pair banks QPSDM_F32P ([ntr][K][nt] float2) and QPSDM_F16P (header,
+ * then [K][nt] half2 scaled by 2^(14 - ex), so that the differences too stay
+ * below 2^15). */
 /* Bytes between consecutive traces of a bank (K * nt * sample size for
- * QPSDM_F32 and QPSDM_BF16; 16 + K * nt * 2 rounded up to 16 for QPSDM_F16);
- * 0 for another dtype. */
+ * QPSDM_F32, QPSDM_BF16 and QPSDM_F32P; 16 + K * nt * 2 (QPSDM_F16) or
+ * 16 + K * nt * 4 (QPSDM_F16P) rounded up to 16); 0 for another dtype. */
 size_t qpsdm_bank_row_bytes(int K, int nt, int dtype);
 int qpsdm_qbank(const qpsdm_bank_params *p, const float *traces_dev, int ntr, void *bank_dev,
                 int bank_dtype, void *stream);
@@ -282,12 +292,29 @@ typedef struct {
    * oracle where it is used. */
   int weight_nodes;
   float z_exact;                    /* node form: depth (m) imaged exactly above  */
+  /* 2: the span form -- the node form's times and levels (the level clamped
+   * to [0, K-1] at the table nodes), with the four ramps and the amplitude
+   * evaluated at the ends of spans of span_s[k] table intervals and
+   * interpolated between them linearly in 1/z (at the fraction
+   * ((z - z0) z1) / ((z1 - z0) z) of a point at depth z in the span
+   * [z0, z1]), the ramps clamped per point.  Band k of the node range starts
+   * at the first table interval at or below span_z[k] metres, rounded up to a
+   * multiple of 8 intervals past the node range's first interval, and ends
+   * where the next band starts (the last at the node range's end); its spans
+   * start at the band's start, the last cut at the band's end.  span_n bands
+   * (1..4), span_z ascending, span_s 1, 2, 4 or 8; intervals above the first
+   * band are spans of one interval.  Batched imaging only
+   * (qpsdm_image_batch*), with QPSDM_F32P or QPSDM_F16P banks and QPSDM_F32 or
+   * QPSDM_U16Q tables. */
+  int span_n;
+  float span_z[4];
+  int span_s[4];
 } qpsdm_image_params;
 
 /* Launch shape picked by the autotuner; a NULL shape or 0 fields take the
  * default 8 x 8 columns x 6 depth points.  In the node form tile_z counts
- * table depth intervals per tile (2, 3 or 4); the exact body's share of
- * the node form runs 8 depth points per tile. */
+ * table depth intervals per tile (2, 3 or 4), in the span form too (2, 4 or
+ * 8); the exact body's share of those forms runs 8 depth points per tile. */
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
 } qpsdm_launch;

@@ -123,6 +123,12 @@ int64_t or_image_src_f32(const float *bank, const float *geom, int ntr, const fl
 void or_node_range(const or_iparams *p, float z_exact, int *ps, int *pe);
 int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
                            const or_iparams *p, float z_exact, float *img);
+/* Span form (the library's weight_nodes = 2): the node form with the level
+ * clamped at the nodes and the ramps and amplitude interpolated, linearly
+ * in 1/z, between the weight nodes spans[2w] <= w < spans[2w + 1] of each
+ * interval w. */
+int64_t or_image_spans_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                           const or_iparams *p, float z_exact, const int *spans, float *img);
 /* Test-only: the FP64 sum's level rule (0 round, the product's; 1 floor;
  * 2 always level 0), for planting a wrong pick in known-answer tests. */
 void or_set_level_rule(int rule);

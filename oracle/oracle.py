@@ -6,6 +6,7 @@ by bench.py's cpu_baseline leg, always as the checker or the timed CPU
 baseline.  The product package (q_psdm_amd) never imports this module.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -95,6 +96,9 @@ def lib():
         L.or_image_nodes_f32.restype = C.c_int64
         L.or_image_nodes_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, fp]
         L.or_node_range.argtypes = [C.POINTER(IParams), C.c_float, ip, ip]
+        L.or_image_spans_f32.restype = C.c_int64
+        L.or_image_spans_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, ip,
+                                         fp]
         L.or_set_level_rule.argtypes = [C.c_int]
         L.or_count.restype = C.c_int64
         L.or_count.argtypes = [fp, C.c_int, C.POINTER(IParams)]
@@ -307,6 +311,68 @@ def image_nodes(bank, geom, tab, ip, z_exact, src=None):
         sp = Src(_f(hdr), _f(st), st.shape[1], st.shape[0])
     n = lib().or_image_nodes_f32(_f(bank), _f(geom), geom.shape[0], _f(tab), C.byref(sp) if sp else None,
                                  C.byref(ip), z_exact, _f(img))
+    return img, n
+
+
+def span_table(ip, z_exact, schedule):
+    """Weight nodes (start, end) of every table interval, int [nw - 1][2],
+    under the span form's schedule [(z, S), ...]: the node range's intervals
+    [ws, we) split into bands, band k from the first interval at or below z_k
+    rounded up to a multiple of 8 intervals past ws (intervals above the
+    first band: one-interval spans; a band that a later one starts at the
+    same interval is dropped), spans of S_k intervals from the band's start,
+    the last cut at the band's end.  Intervals outside the range get
+    (w, w + 1) (the oracle does not read them)."""
+    nw, dw = ip.tg.nw, ip.tg.dw
+    sp = np.array([[w, w + 1] for w in range(nw - 1)], np.int32)
+    ps, pe = node_range(ip, z_exact)
+    if ps >= pe:
+        return sp
+    ws = max(1, int(math.ceil(float(np.float32(z_exact)) / float(np.float32(dw)))))
+    inv_dw = np.float32(1.0) / np.float32(dw)
+    fw = np.float32(pe - 1) * np.float32(ip.dz) * inv_dw
+    fw = min(fw, np.float32(nw - 1))
+    we = min(int(fw), nw - 2) + 1
+    start, span = [], []
+    for z, S in schedule:
+        x = float(np.float32(z)) / float(np.float32(dw)) - ws
+        b = min(ws + 8 * int(math.ceil(max(0.0, x) / 8.0)), we)
+        if not start and b > ws:
+            start.append(ws)
+            span.append(1)
+        if start and start[-1] == b:
+            start.pop()
+            span.pop()
+        start.append(b)
+        span.append(int(S))
+    if start and start[-1] == we:
+        start.pop()
+        span.pop()
+    ends = start[1:] + [we]
+    for b, e, S in zip(start, ends, span):
+        for w in range(b, e):
+            w0 = b + (w - b) // S * S
+            sp[w] = (w0, min(w0 + S, e))
+    return sp
+
+
+def image_spans(bank, geom, tab, ip, z_exact, spans, src=None):
+    """Span form of the FP32 path (weight_nodes = 2): (image, count).
+    spans: int [nw - 1][2], the weight nodes (start, end) of each table
+    depth interval."""
+    geom = np.ascontiguousarray(geom, np.float32)
+    tab = np.ascontiguousarray(tab, np.float32)
+    bank = np.ascontiguousarray(bank, np.float32)
+    spans = np.ascontiguousarray(spans, np.int32)
+    assert spans.shape == (ip.tg.nw - 1, 2)
+    img = np.zeros((ip.ny, ip.nx, ip.nz), np.float32)
+    sp = None
+    if src is not None:
+        hdr = np.ascontiguousarray(src[0], np.float32).reshape(4)
+        st = np.ascontiguousarray(src[1], np.float32)
+        sp = Src(_f(hdr), _f(st), st.shape[1], st.shape[0])
+    n = lib().or_image_spans_f32(_f(bank), _f(geom), geom.shape[0], _f(tab), C.byref(sp) if sp else None,
+                                 C.byref(ip), z_exact, _i(spans), _f(img))
     return img, n
 
 

@@ -730,8 +730,6 @@ static int64_t image_f32_impl(const float *bank, const float *geom, int ntr, con
   return count;
 }
 
-/* Update count alone: pairs (trace, point) with z > 0 inside the widened
- * dip bounds, evaluated in FP32 with the FP32 path's order of operations. */
 /* ---------------------------------------------------------------------- */
 /* Node form of the FP32 path (the library's weight_nodes = 1): below      */
 /* z_exact and within the tables' depth range, the four dip-taper ramps,   */
@@ -823,8 +821,52 @@ typedef struct {
   float F, L, u[4], a;
 } nf_node;
 
-int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
-                           const or_iparams *p, float z_exact, float *img) {
+/* Ramps, amplitude, time and level of a column at depth node wn (one
+ * node of the node form). */
+typedef struct {
+  const or_tgeom *g;
+  const float *tab;
+  const or_src *src;
+  float sx, sy, gx, gy, x, y, dsx, dsy, dgx, dgy, hs2, hg2;
+  float tmin, inv_dt, inv_dts, rx1, rx2, ry1, ry2, cx1, cx2, cy1, cy2;
+} nf_col;
+
+static nf_node nf_value(const nf_col *C, int wn) {
+  const float zn = (float)wn * C->g->dw;
+  float ts, tss, tg, tsg;
+  node_table_f32(C->g, C->tab, C->src, 1, C->sx, C->sy, C->x, C->y, wn, &ts, &tss);
+  node_table_f32(C->g, C->tab, C->src, 0, C->gx, C->gy, C->x, C->y, wn, &tg, &tsg);
+  ts = fmaxf(ts, C->tmin);
+  const float d2 = fmaf(zn, zn, C->hs2);
+  const float ds = d2 * (1.0f / sqrtf(d2));
+  const float tsn = ts * C->inv_dt, tsl = fmaf(tss, C->inv_dts, 0.5f), its = 1.0f / ts;
+  const float rdg = 1.0f / sqrtf(fmaf(zn, zn, C->hg2));
+  const float q = ds * rdg;
+  const float X = fmaf(C->dgx, q, C->dsx), Y = fmaf(C->dgy, q, C->dsy), Z = fmaf(zn, q, zn);
+  const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
+  const float t = fmaxf(tg, C->tmin);
+  const float tr = (q * q) * q * (t * its);
+  nf_node v;
+  v.u[0] = fmaf(tanx, C->rx1, C->cx1);
+  v.u[1] = fmaf(tanx, -C->rx2, C->cx2);
+  v.u[2] = fmaf(tany, C->ry1, C->cy1);
+  v.u[3] = fmaf(tany, -C->ry2, C->cy2);
+  v.a = tr + 1.0f / tr;
+  v.F = fmaf(t, C->inv_dt, tsn);
+  v.L = fmaf(tsg, C->inv_dts, tsl);
+  return v;
+}
+
+/* The node form (spans == NULL) and the span form (this build's own
+ * method, like the node form).  Span form: the time and the level are
+ * interpolated between the interval's two depth nodes as in the node form,
+ * with the level clamped to [0, K-1] at the nodes; the four ramps and the
+ * amplitude are interpolated between the weight nodes w0 = spans[2w] and
+ * w1 = spans[2w + 1] of the interval's span (w0 <= w < w1) linearly in 1/z:
+ * at the fraction ((z - z0) z1) / ((z1 - z0) z) of a point at depth z,
+ * z0 = w0 dw, z1 = w1 dw; the ramps are clamped per point. */
+static int64_t nodes_impl(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                          const or_iparams *p, float z_exact, const int *spans, float *img) {
   int ps, pe;
   or_node_range(p, z_exact, &ps, &pe);
   int64_t count = image_f32_impl(bank, geom, ntr, tab, src, p, ps, pe, img);
@@ -834,67 +876,81 @@ int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const 
   const float tx1 = p->tanx1, tx2 = p->tanx2, ty1 = p->tany1, ty2 = p->tany2;
   const float tx1c = tx1 - tz * (1 + tx1 * tx1), tx2c = tx2 + tz * (1 + tx2 * tx2);
   const float ty1c = ty1 - tz * (1 + ty1 * ty1), ty2c = ty2 + tz * (1 + ty2 * ty2);
-  const float rx1 = 1.0f / (tx1 - tx1c), rx2 = 1.0f / (tx2c - tx2);
-  const float ry1 = 1.0f / (ty1 - ty1c), ry2 = 1.0f / (ty2c - ty2);
-  const float cx1 = -tx1c * rx1, cx2 = tx2c * rx2, cy1 = -ty1c * ry1, cy2 = ty2c * ry2;
-  const float inv_dt = 1.0f / p->dt, inv_dts = 1.0f / p->dtstar, inv_dw = 1.0f / g->dw;
+  const float inv_dw = 1.0f / g->dw;
   const float jmax = (float)(p->K - 1);
+  nf_col C0;
+  C0.g = g;
+  C0.tab = tab;
+  C0.src = src;
+  C0.tmin = p->tmin;
+  C0.inv_dt = 1.0f / p->dt;
+  C0.inv_dts = 1.0f / p->dtstar;
+  C0.rx1 = 1.0f / (tx1 - tx1c);
+  C0.rx2 = 1.0f / (tx2c - tx2);
+  C0.ry1 = 1.0f / (ty1 - ty1c);
+  C0.ry2 = 1.0f / (ty2c - ty2);
+  C0.cx1 = -tx1c * C0.rx1;
+  C0.cx2 = tx2c * C0.rx2;
+  C0.cy1 = -ty1c * C0.ry1;
+  C0.cy2 = ty2c * C0.ry2;
   int64_t ncount = 0;
 #pragma omp parallel for schedule(dynamic) reduction(+ : ncount)
   for (int iy = 0; iy < p->ny; ++iy) {
     for (int itr = 0; itr < ntr; ++itr) {
-      const float sx = geom[4 * itr], sy = geom[4 * itr + 1];
-      const float gx = geom[4 * itr + 2], gy = geom[4 * itr + 3];
-      if (sx != sx) continue;
+      nf_col C = C0;
+      C.sx = geom[4 * itr];
+      C.sy = geom[4 * itr + 1];
+      C.gx = geom[4 * itr + 2];
+      C.gy = geom[4 * itr + 3];
+      if (C.sx != C.sx) continue;
       if (src && !same_source(geom + 4 * itr, src)) continue;
       const float *bt = bank + (size_t)itr * p->K * p->nt;
-      const float y = (float)iy * p->dy;
+      C.y = (float)iy * p->dy;
       for (int ix = 0; ix < p->nx; ++ix) {
-        const float x = (float)ix * p->dx;
-        const float dsx = sx - x, dsy = sy - y, dgx = gx - x, dgy = gy - y;
-        const float hs2 = dsx * dsx + dsy * dsy, hg2 = dgx * dgx + dgy * dgy;
-        int wcur = -1;
-        nf_node nv[2];
+        C.x = (float)ix * p->dx;
+        C.dsx = C.sx - C.x;
+        C.dsy = C.sy - C.y;
+        C.dgx = C.gx - C.x;
+        C.dgy = C.gy - C.y;
+        C.hs2 = C.dsx * C.dsx + C.dsy * C.dsy;
+        C.hg2 = C.dgx * C.dgx + C.dgy * C.dgy;
+        int wcur = -1, scur = -1;
+        nf_node nv[2], sv[2];
         for (int iz = ps; iz < pe; ++iz) {
           float c;
           const int w = nf_interval(iz, p->dz, inv_dw, g->nw, &c);
           if (w != wcur) {
-            for (int k = 0; k < 2; ++k) {
-              const int wn = w + k;
-              const float zn = (float)wn * g->dw;
-              float ts, tss, tg, tsg;
-              node_table_f32(g, tab, src, 1, sx, sy, x, y, wn, &ts, &tss);
-              node_table_f32(g, tab, src, 0, gx, gy, x, y, wn, &tg, &tsg);
-              ts = fmaxf(ts, p->tmin);
-              const float d2 = fmaf(zn, zn, hs2);
-              const float ds = d2 * (1.0f / sqrtf(d2));
-              const float tsn = ts * inv_dt, tsl = fmaf(tss, inv_dts, 0.5f), its = 1.0f / ts;
-              const float rdg = 1.0f / sqrtf(fmaf(zn, zn, hg2));
-              const float q = ds * rdg;
-              const float X = fmaf(dgx, q, dsx), Y = fmaf(dgy, q, dsy), Z = fmaf(zn, q, zn);
-              const float rz = 1.0f / Z, tanx = X * rz, tany = Y * rz;
-              const float t = fmaxf(tg, p->tmin);
-              const float tr = (q * q) * q * (t * its);
-              nv[k].u[0] = fmaf(tanx, rx1, cx1);
-              nv[k].u[1] = fmaf(tanx, -rx2, cx2);
-              nv[k].u[2] = fmaf(tany, ry1, cy1);
-              nv[k].u[3] = fmaf(tany, -ry2, cy2);
-              nv[k].a = tr + 1.0f / tr;
-              nv[k].F = fmaf(t, inv_dt, tsn);
-              nv[k].L = fmaf(tsg, inv_dts, tsl);
+            nv[0] = nf_value(&C, w);
+            nv[1] = nf_value(&C, w + 1);
+            if (spans) {
+              nv[0].L = nv[0].L < 0.0f ? 0.0f : (nv[0].L > jmax ? jmax : nv[0].L);
+              nv[1].L = nv[1].L < 0.0f ? 0.0f : (nv[1].L > jmax ? jmax : nv[1].L);
             }
             wcur = w;
           }
-          const nf_node *n0 = &nv[0], *n1 = &nv[1];
+          const nf_node *n0 = &nv[0], *n1 = &nv[1], *m0 = n0, *m1 = n1;
+          float s = c;
+          if (spans) {
+            const int w0 = spans[2 * w], w1 = spans[2 * w + 1];
+            if (w0 != scur) {
+              sv[0] = nf_value(&C, w0);
+              sv[1] = nf_value(&C, w1);
+              scur = w0;
+            }
+            m0 = &sv[0];
+            m1 = &sv[1];
+            const float z = (float)iz * p->dz, z0 = (float)w0 * g->dw, z1 = (float)w1 * g->dw;
+            s = ((z - z0) * z1) / ((z1 - z0) * z);
+          }
           const float f = fmaf(c, n1->F - n0->F, n0->F);
-          const float wx = fminf(sat01(fmaf(c, n1->u[0] - n0->u[0], n0->u[0])),
-                                 sat01(fmaf(c, n1->u[1] - n0->u[1], n0->u[1])));
-          const float wy = fminf(sat01(fmaf(c, n1->u[2] - n0->u[2], n0->u[2])),
-                                 sat01(fmaf(c, n1->u[3] - n0->u[3], n0->u[3])));
-          const float wt = (wx * wy) * fmaf(c, n1->a - n0->a, n0->a);
+          const float wx = fminf(sat01(fmaf(s, m1->u[0] - m0->u[0], m0->u[0])),
+                                 sat01(fmaf(s, m1->u[1] - m0->u[1], m0->u[1])));
+          const float wy = fminf(sat01(fmaf(s, m1->u[2] - m0->u[2], m0->u[2])),
+                                 sat01(fmaf(s, m1->u[3] - m0->u[3], m0->u[3])));
+          const float wt = (wx * wy) * fmaf(s, m1->a - m0->a, m0->a);
           const int i0 = (int)f;
           float lv = fmaf(c, n1->L - n0->L, n0->L);
-          lv = lv < 0.0f ? 0.0f : (lv > jmax ? jmax : lv);
+          if (!spans) lv = lv < 0.0f ? 0.0f : (lv > jmax ? jmax : lv);
           const int j = (int)lv;
           if (!((unsigned)(i0 - 1) < (unsigned)(p->nt - 2))) continue;
           if (wt > 0.0f) ++ncount;
@@ -909,6 +965,18 @@ int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const 
   return count + ncount;
 }
 
+int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                           const or_iparams *p, float z_exact, float *img) {
+  return nodes_impl(bank, geom, ntr, tab, src, p, z_exact, NULL, img);
+}
+
+int64_t or_image_spans_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
+                           const or_iparams *p, float z_exact, const int *spans, float *img) {
+  return nodes_impl(bank, geom, ntr, tab, src, p, z_exact, spans, img);
+}
+
+/* Update count alone: pairs (trace, point) with z > 0 inside the widened
+ * dip bounds, evaluated in FP32 with the FP32 path's order of operations. */
 int64_t or_count(const float *geom, int ntr, const or_iparams *p) {
   int64_t count = 0;
   float tz = p->taperzone;

@@ -10,7 +10,8 @@ representative shot, timed with HIP events, and the fastest is kept.
 
 Candidates: column tiles 8x8, 16x4 and 4x16 (one wave of columns) times
 depth points per lane ZP in {2, 3, 4, 6, 8} (in the node form: table depth
-intervals per tile in {2, 3, 4}); for the batched kernel also the grid:
+intervals per tile in {2, 3, 4}; in the span form {2, 4, 8}); for the
+batched kernel also the grid:
 persistent at the resident capacity or twice it, or one work-group per tile
 (nwg = -1; the node form always runs one work-group per tile).  Results are cached per (geometry, storage
 types, kernel) in memory for the process.
@@ -22,14 +23,16 @@ import torch
 TILES = ((8, 8), (16, 4), (4, 16))
 DEPTHS = (2, 3, 4, 6, 8)
 NODE_INTERVALS = (2, 3, 4)  # node form: table depth intervals per tile
+SPAN_INTERVALS = (2, 4, 8)  # span form: table depth intervals per tile
 GRIDS = (0, -2, -1)  # batched kernel: resident, twice resident (-2), one group per tile (-1)
 _cache = {}
 
 
 def _key(sv, bank, tab, kind):
     c = sv.cfg
-    return (c.nx, c.ny, c.nz, c.K, c.nt, tuple(sorted(c.tgeom.items())), str(bank.dtype),
-            getattr(tab, "qpsdm_code", 0), kind, sv.form, float(sv.ip.z_exact))
+    return (c.nx, c.ny, c.nz, c.K, c.nt, tuple(sorted(c.tgeom.items())), sv.bank_code(bank),
+            getattr(tab, "qpsdm_code", 0), kind, sv.form, float(sv.ip.z_exact),
+            tuple(sv.spans()) if sv.form == "spans" else ())
 
 
 def _time(fn, reps=2):
@@ -68,6 +71,9 @@ def tune(sv, geom, bank, tab, shots=None, ntr=400, kind="batch", log=None, reps=
         grids = (0,)
     if sv.form == "nodes":  # node tiles: tile_z intervals, one work-group per tile
         depths, grids = NODE_INTERVALS, ((-1,) if kind == "batch" else (0,))
+    if sv.form == "spans":  # span tiles: tile_z intervals, batched, one work-group per tile
+        assert kind == "batch", "the span form runs through the batched kernel"
+        depths, grids = SPAN_INTERVALS, (-1,)
     for (tx, ty), zp, g in itertools.product(TILES, depths, grids):
         launch = (tx, ty, zp, 0)
         if kind == "batch":

@@ -19,7 +19,7 @@
 #include <type_traits>
 
 #include "common.h"
-#include "image_nodes.h"
+#include "batch_runs.h"
 
 namespace qk = qpsdm_dev;
 
@@ -28,61 +28,7 @@ int qpsdm_qbank_prepare(const qpsdm_bank_params *p);
 
 namespace {
 
-
-// Per-shot runs; with src_hdr set, each shot's source leg comes from its own
-// source table (header {sx, sy, su0, sv0} per shot, then the tables).
-struct ShotRuns {
-  const qpsdm_shot *shots;
-  float tz;
-  const float4 *src_hdr;
-  const float2 *src_tab;
-  long long src_stride;
-  int nus, nvs;
-  // The descriptor and header words are read through the constant address
-  // space (they do not change during a launch), so they are scalar loads
-  // and the run's fields stay in scalar registers.
-  __device__ qk::Run operator()(int r) const {
-    typedef const __attribute__((address_space(4))) int *cip;
-    typedef const __attribute__((address_space(4))) float *cfp;
-    const cip si = (cip)(shots + r);
-    const cfp sf = (cfp)(shots + r);
-    qk::Run run;
-    run.tr0 = si[0];
-    run.ntr = si[1];
-    run.brow0 = run.tr0;
-    run.dip = qk::uniform_dip(qk::make_dip(sf[2], sf[3], sf[4], sf[5], tz));
-    run.src = nullptr;
-    run.ssx = run.ssy = run.su0 = run.sv0 = 0.f;
-    run.snu = nus;
-    run.snv = nvs;
-    if (src_hdr) {
-      const cfp h = (cfp)(src_hdr + r);
-      run.src = src_tab + (size_t)r * src_stride;
-      run.ssx = h[0];
-      run.ssy = h[1];
-      run.su0 = h[2];
-      run.sv0 = h[3];
-    }
-    return run;
-  }
-};
-
-// One shot as a run list of length one.
-struct OneShot {
-  qk::Run run;
-  __device__ qk::Run operator()(int) const { return run; }
-};
-
-ShotRuns shot_runs(const qpsdm_image_params *p, const qpsdm_shot *shots, int nshot, const void *src) {
-  ShotRuns R{shots, p->taperzone, nullptr, nullptr, 0, 0, 0};
-  if (src) {
-    qk::src_subgrid(p, &R.nus, &R.nvs);
-    R.src_stride = (long long)R.nvs * R.nus * p->tg.nw;
-    R.src_hdr = (const float4 *)src;
-    R.src_tab = (const float2 *)((const char *)src + (size_t)nshot * 16);
-  }
-  return R;
-}
+using namespace qpsdm_batch;
 
 __global__ void reset_kernel(unsigned *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
@@ -114,22 +60,6 @@ void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const fl
   }
 }
 
-// One work-group per tile (nwg = -1): one launch per shot of the batch, in
-// shot order, each the tile body for that shot alone (no loop over shots in
-// the kernel: its registers are the per-shot kernel's, with no spills), so
-// per image point the sum still runs over the shots in order and each
-// shot's traces in order.  Scheduled by the hardware dispatcher.
-// SRC: the shots carry source tables; RANGE: depth groups of [zb, ze).
-template <int ZP, typename BankT, typename Tab, bool SRC, bool RANGE>
-__global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
-                                                               const float4 *__restrict__ geom,
-                                                               const BankT *__restrict__ bank, ShotRuns runs,
-                                                               int shot, int ncol) {
-  const int tile = (int)blockIdx.x;
-  const OneShot one{runs(shot)};
-  qk::image_tile<ZP, SRC, RANGE>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
-}
-
 // Node form of one shot (weight_nodes = 1): one work-group per node tile
 // (blockIdx.x = column tile, blockIdx.y = node group).
 template <int NI, int PM, typename BankT, typename Tab, bool SRC>
@@ -150,20 +80,6 @@ __global__ __launch_bounds__(64) void count_batch_kernel(qk::ImgDev P, const flo
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if (threadIdx.x == 0 && v) atomicAdd(count, v);
 }
-
-// Launch arguments of one batch; with run == false only the resident grid
-// size is resolved (a query made before graph capture).
-struct Go {
-  float *img;
-  const float4 *geom;
-  ShotRuns runs;
-  int nshot;
-  unsigned *work;
-  int nwg;
-  hipStream_t s;
-  bool run;
-  int *nwg_out;
-};
 
 template <int ZP, bool SRC, typename BankT, typename Tab>
 int launch_batch_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
@@ -290,6 +206,7 @@ int batch_dispatch(const qpsdm_image_params *p, const qpsdm_launch *shape, const
   qk::ImgDev P;
   int rc = qk::make_dev(p, shape, 0, 0, &P);
   if (rc) return rc;
+  if (p->weight_nodes == 2) return launch_batch_spans(p, P, tab, table_dtype, bank, bank_dtype, go);
   switch (table_dtype) {
     case QPSDM_F32: return launch_batch_bank(P, qk::make_tab_f32(p->tg, tab), bank, bank_dtype, go);
     case QPSDM_F16B: return launch_batch_bank(P, qk::make_tab_b16<false>(p->tg, tab), bank, bank_dtype, go);

@@ -17,6 +17,8 @@ namespace qk = qpsdm_dev;
 
 namespace qpsdm_dev {
 
+int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o);
+
 // Node range of the node form (ImgDev nf_*), with the device's float rule
 // for a point's interval: intervals from the first node at or below z_exact
 // (node 1 at least: the surface node has no bisector) to the last interval
@@ -26,6 +28,56 @@ int node_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   const float dz = p->dz, inv_dw = o->inv_dw;
   QPSDM_REQUIRE(p->z_exact >= 0.0f, "image: z_exact must be >= 0");
   QPSDM_REQUIRE(ni >= 2 && ni <= 4, "image: the node form takes tile_z = 2, 3 or 4 depth intervals per tile");
+  return node_range_ni(p, ni, o);
+}
+
+// The span form's node range (the node form's) and its bands: band k starts
+// at the first interval at or below span_z[k], rounded up to a multiple of 8
+// intervals past the range's first interval ws (so that tiles of 2, 4 or 8
+// intervals from a band's start never straddle two bands and every span
+// length divides the band's offsets); intervals above the first band form a
+// band of one-interval spans; a band that a later one starts at the same
+// interval is dropped.
+int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
+  QPSDM_REQUIRE(p->z_exact >= 0.0f, "image: z_exact must be >= 0");
+  QPSDM_REQUIRE(ni == 2 || ni == 4 || ni == 8, "image: the span form takes tile_z = 2, 4 or 8 intervals per tile");
+  QPSDM_REQUIRE(p->span_n >= 1 && p->span_n <= 4, "image: the span form takes 1 to 4 bands (span_n)");
+  for (int k = 0; k < p->span_n; ++k) {
+    const int S = p->span_s[k];
+    QPSDM_REQUIRE(S == 1 || S == 2 || S == 4 || S == 8, "image: span_s[%d] = %d (1, 2, 4 or 8)", k, S);
+    QPSDM_REQUIRE(p->span_z[k] >= 0.0f && (k == 0 || p->span_z[k] >= p->span_z[k - 1]),
+                  "image: span_z must be >= 0 and ascending");
+  }
+  int rc = node_range_ni(p, ni, o);
+  if (rc) return rc;
+  o->nf_ngroups = 0;
+  if (o->nf_ps >= o->nf_pe) return QPSDM_OK;  // no node range: the exact body images every point
+  const int ws = o->nf_ws, we = o->nf_we;
+  int start[5], span[5], n = 0;
+  for (int k = 0; k < p->span_n; ++k) {
+    const double x = (double)p->span_z[k] / (double)p->tg.dw - (double)ws;
+    const int b = std::min(ws + 8 * (int)std::ceil(std::max(0.0, x) / 8.0), we);
+    if (n == 0 && b > ws) {
+      start[n] = ws;
+      span[n++] = 1;
+    }
+    if (n > 0 && start[n - 1] == b) --n;
+    start[n] = b;
+    span[n++] = p->span_s[k];
+  }
+  if (n > 0 && start[n - 1] == we) --n;
+  o->sp_n = n;
+  for (int k = 0; k < n; ++k) {
+    o->sp_w[k] = start[k];
+    o->sp_s[k] = span[k];
+  }
+  o->sp_w[n] = we;
+  return QPSDM_OK;
+}
+
+int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o) {
+  const int nz = p->nz, nw = p->tg.nw;
+  const float dz = p->dz, inv_dw = o->inv_dw;
   const int ws = std::max(1, (int)std::ceil((double)p->z_exact / (double)p->tg.dw));
   int pe = nz;
   for (int iz = 0; iz < nz; ++iz) {
@@ -207,6 +259,7 @@ extern "C" int qpsdm_image_shot(const qpsdm_image_params *p, const qpsdm_launch 
                                 const float *geom, int tr0, int ntr, const void *bank,
                                 int bank_dtype, const void *tab, int table_dtype, void *stream) {
   qk::ImgDev P;
+  QPSDM_REQUIRE(p && p->weight_nodes != 2, "image: the span form (weight_nodes = 2) runs through qpsdm_image_batch");
   int rc = qk::make_dev(p, shape, tr0, ntr, &P);
   if (rc) return rc;
   QPSDM_REQUIRE(img && geom && bank && tab, "image: null buffer");

@@ -10,8 +10,10 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
+#include "pair_bank.h"
 
 namespace qpsdm_dev {
 
@@ -93,9 +95,15 @@ struct ImgDev {
   // points [nf_ps, nf_pe), nf_ngroups tiles of tile_z intervals per column
   // group, at most nf_pm points per interval; nf_ngroups = 0 without it
   int nf_ws, nf_we, nf_ps, nf_pe, nf_ngroups, nf_pm;
+  // span form (weight_nodes = 2): the node range's bands (sp_n of them;
+  // band k covers intervals [sp_w[k], sp_w[k + 1]) in spans of sp_s[k]),
+  // and the band [sp_b, sp_e) of one launch
+  int sp_n, sp_w[6], sp_s[5];
+  int sp_b, sp_e;
 };
 
 int node_range(const qpsdm_image_params *p, int ni, ImgDev *o);
+int span_range(const qpsdm_image_params *p, int ni, ImgDev *o);
 
 // The parameters with the exact form selected: for the kernels that evaluate
 // single points (update count, table lookup, dip-angle gathers), which have
@@ -167,9 +175,12 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->ze = p->nz;
   o->nf_ws = o->nf_we = o->nf_ngroups = o->nf_pm = 0;
   o->nf_ps = o->nf_pe = p->nz;
-  QPSDM_REQUIRE(p->weight_nodes == 0 || p->weight_nodes == 1,
-                "image: weight_nodes must be 0 (exact form) or 1 (node form)");
-  if (p->weight_nodes) return node_range(p, tzp, o);
+  o->sp_n = 0;
+  o->sp_b = o->sp_e = 0;
+  QPSDM_REQUIRE(p->weight_nodes >= 0 && p->weight_nodes <= 2,
+                "image: weight_nodes must be 0 (exact form), 1 (node form) or 2 (span form)");
+  if (p->weight_nodes == 1) return node_range(p, tzp, o);
+  if (p->weight_nodes == 2) return span_range(p, tzp, o);
   return QPSDM_OK;
 }
 
@@ -681,6 +692,72 @@ __device__ inline __amdgpu_buffer_rsrc_t bank_row(const __half *bank, size_t row
 template <typename BankT>
 __device__ inline float2 bank_pair(__amdgpu_buffer_rsrc_t r, int idx, const BankT *b) {
   return bank_decode(bank_raw(r, idx, b), b);
+}
+
+// Pair banks (QPSDM_F32P, QPSDM_F16P): sample i of a level row is stored
+// with its forward difference, (b[i], b[i + 1] - b[i]), so the linear
+// interpolation at fraction f is one fused multiply-add, b[i] + f d[i]
+// (for FP16 one v_fma_mix_f32 reading both halves of the 32-bit word).
+// Entries i = 0 and i = nt - 1 of every row hold (0, 0): the reference
+// images a sample pair only for 0 < i and i + 1 < nt (src/q-psdm.cu:
+// 2354-2357), and those two entries are exactly the indices in [0, nt - 1]
+// outside that range, so a kernel that clamps i to nt - 1 needs no range
+// test per point.  F32P rows are K * nt float2; F16P rows carry the FP16
+// bank's 16-byte {1/scale, scale, 0, 0} header, then K * nt half2.
+template <>
+struct BankRaw<PairF32> {
+  typedef uint2 type;
+};
+template <>
+struct BankRaw<PairF16> {
+  typedef unsigned type;
+};
+__device__ inline uint2 bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const PairF32 *) {
+  return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+}
+__device__ inline unsigned bank_raw(__amdgpu_buffer_rsrc_t r, int idx, const PairF16 *) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0);
+}
+
+// Linear interpolation at fraction f of the raw pair (b[i], b[i + 1]) or
+// (b[i], d[i]) as loaded.
+__device__ inline float bank_interp(uint2 v, float f, const float *) {
+  const float b0 = __uint_as_float(v.x), b1 = __uint_as_float(v.y);
+  return fmaf(f, b1 - b0, b0);
+}
+__device__ inline float bank_interp(unsigned v, float f, const __hip_bfloat16 *b) {
+  const float2 p = bank_decode(v, b);
+  return fmaf(f, p.y - p.x, p.x);
+}
+__device__ inline float bank_interp(unsigned v, float f, const __half *b) {
+  const float2 p = bank_decode(v, b);
+  return fmaf(f, p.y - p.x, p.x);
+}
+__device__ inline float bank_interp(uint2 v, float f, const PairF32 *) {
+  return fmaf(f, __uint_as_float(v.y), __uint_as_float(v.x));
+}
+__device__ inline float bank_interp(unsigned v, float f, const PairF16 *) {
+  const __half2 h = __builtin_bit_cast(__half2, v);
+  return fmaf(f, __high2float(h), __low2float(h));
+}
+
+// Banks stored under a per-trace scale (the row header's 1/scale folds into
+// the weight).
+template <typename BankT>
+__host__ __device__ constexpr bool bank_scaled() {
+  return std::is_same<BankT, __half>::value || std::is_same<BankT, PairF16>::value;
+}
+
+__device__ inline __amdgpu_buffer_rsrc_t bank_row(const PairF32 *bank, size_t row, int K, int nt, float *unit) {
+  *unit = 1.0f;
+  return make_rsrc(bank + row * K * nt, (unsigned)(K * nt * 8));
+}
+__device__ inline __amdgpu_buffer_rsrc_t bank_row(const PairF16 *bank, size_t row, int K, int nt, float *unit) {
+  typedef const __attribute__((address_space(4))) float *cfp;
+  const size_t words = (16 + ((size_t)K * nt * 4 + 15) / 16 * 16) / 4;  // qpsdm_bank_row_bytes / 4
+  const PairF16 *r = bank + row * words;
+  *unit = *(cfp)r;
+  return make_rsrc(r + 4, (unsigned)(K * nt * 4));
 }
 
 }  // namespace qpsdm_dev

@@ -235,7 +235,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
           const int j = (int)__builtin_amdgcn_fmed3f(fmaf(tsg, P.inv_dtstar, S.tsl[i]), 0.0f, jmax);
           ci[i] += (int)__umul24((unsigned)j, nt);
           const float tr = S.A[i] * (rdg * rdg * rdg) * tg;
-          if constexpr (std::is_same<BankT, __half>::value)
+          if constexpr (bank_scaled<BankT>())
             cw[i] = (w * unit) * (tr + __builtin_amdgcn_rcpf(tr));
           else
             cw[i] = w * (tr + __builtin_amdgcn_rcpf(tr));
@@ -245,10 +245,7 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
 #pragma unroll
       for (int i = 0; i < ZP; ++i) cb[i] = bank_raw(br, ci[i], bank);
 #pragma unroll
-      for (int i = 0; i < ZP; ++i) {
-        const float2 b = bank_decode(cb[i], bank);
-        acc[i] = fmaf(cw[i], fmaf(cf[i], b.y - b.x, b.x), acc[i]);
-      }
+      for (int i = 0; i < ZP; ++i) acc[i] = fmaf(cw[i], bank_interp(cb[i], cf[i], bank), acc[i]);
     }
   }
   if (valid) {

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pair_bank.h"
 
 namespace {
 
@@ -215,6 +216,27 @@ __device__ inline void store_out(float *p, float v) { *p = v; }
 __device__ inline void store_out(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
 __device__ inline void store_out(__half *p, float v) { *p = __float2half(v); }
 
+using qpsdm_dev::PairF16;
+using qpsdm_dev::PairF32;
+
+// Pair banks: each entry holds the sample and its forward difference.
+template <typename OutT>
+constexpr bool pair_out() {
+  return std::is_same<OutT, PairF32>::value || std::is_same<OutT, PairF16>::value;
+}
+// Banks stored under a per-trace power-of-two scale with a row header.
+template <typename OutT>
+constexpr bool scaled_out() {
+  return std::is_same<OutT, __half>::value || std::is_same<OutT, PairF16>::value;
+}
+__device__ inline void store_pair(PairF32 *p, float b0, float b1) { *p = PairF32{b0, b1 - b0}; }
+__device__ inline void store_pair(PairF16 *p, float b0, float b1) {
+  const __half2 h = __halves2half2(__float2half(b0), __float2half(b1 - b0));
+  *p = PairF16{__builtin_bit_cast(unsigned, h)};
+}
+__device__ inline void store_zero(__half *p) { *p = __float2half(0.0f); }
+__device__ inline void store_zero(PairF16 *p) { *p = PairF16{0u}; }
+
 // Largest value over the work-group (every thread gets it).
 __device__ inline float block_max(float m) {
   __shared__ float red[kThreads / 64];
@@ -317,7 +339,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     return (n >= 0 && n < N) ? (s2 ? Y[n].y : Y[n].x) : 0.0f;
   };
   float scale = 1.0f;
-  if constexpr (std::is_same<OutT, __half>::value) {
+  if constexpr (scaled_out<OutT>()) {
     // eight lanes per level, each over every eighth bin, then summed across
     // the eight (kThreads is a multiple of 64, so every pass is wave-uniform)
     float m = 0.0f;
@@ -341,10 +363,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
       (void)frexpf(m, &ex);  // m in [2^(ex-1), 2^ex)
       ex = max(-100, min(ex, 100));
     }
-    scale = ldexpf(1.0f, 15 - ex);
+    // pair banks store differences of two samples too: one bit more headroom
+    const int top = pair_out<OutT>() ? 14 : 15;
+    scale = ldexpf(1.0f, top - ex);
     if (threadIdx.x == 0) {
       float *h = reinterpret_cast<float *>(bank + (size_t)tr * P.row);
-      h[0] = ldexpf(1.0f, ex - 15);
+      h[0] = ldexpf(1.0f, ex - top);
       h[1] = scale;
       h[2] = 0.0f;
       h[3] = 0.0f;
@@ -352,13 +376,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     // the row's tail padding (to a 16-byte multiple) is written too, so the
     // output is fully determined by the input
     const long long used = P.lead + (long long)P.K * P.nt;
-    for (long long i = used + threadIdx.x; i < P.row; i += blockDim.x)
-      bank[(size_t)tr * P.row + i] = __float2half(0.0f);
+    for (long long i = used + threadIdx.x; i < P.row; i += blockDim.x) store_zero(bank + (size_t)tr * P.row + i);
   }
   for (int j = 0; j < P.K; ++j) {
     const float2 *Y = level(j);
     OutT *o = bank + (size_t)tr * P.row + P.lead + (size_t)j * P.nt;
-    for (int it = threadIdx.x; it < P.nt; it += blockDim.x) store_out(o + it, level_sample(Y, it) * scale);
+    if constexpr (pair_out<OutT>()) {
+      // (b[i], b[i+1] - b[i]); entries 0 and nt-1 zero
+      for (int it = threadIdx.x; it < P.nt; it += blockDim.x) {
+        const bool edge = it == 0 || it == P.nt - 1;
+        const float b0 = edge ? 0.0f : level_sample(Y, it) * scale;
+        const float b1 = edge ? 0.0f : level_sample(Y, it + 1) * scale;
+        store_pair(o + it, b0, b1);
+      }
+    } else {
+      for (int it = threadIdx.x; it < P.nt; it += blockDim.x) store_out(o + it, level_sample(Y, it) * scale);
+    }
     __syncthreads();
   }
 }
@@ -529,7 +562,9 @@ int fit_lds(size_t lds, int nsb, bool spectra, int dtype) {
     case QPSDM_F32: return fit_kernel_lds(qbank_kernel<1, float>, lds, nsb);
     case QPSDM_BF16: return fit_kernel_lds(qbank_kernel<1, __hip_bfloat16>, lds, nsb);
     case QPSDM_F16: return fit_kernel_lds(qbank_kernel<1, __half>, lds, nsb);
-    default: return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16", dtype);
+    case QPSDM_F32P: return fit_kernel_lds(qbank_kernel<1, PairF32>, lds, nsb);
+    case QPSDM_F16P: return fit_kernel_lds(qbank_kernel<1, PairF16>, lds, nsb);
+    default: return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16/F32P/F16P", dtype);
   }
 }
 
@@ -567,6 +602,10 @@ int launch_coefs(const qpsdm_bank_params *p, const Coefs *c, const float *traces
     P.row = (long long)(qpsdm_bank_row_bytes(p->K, p->nt, QPSDM_F16) / 2);
     P.lead = 8;  // the 16-byte header
   }
+  if (dtype == QPSDM_F16P) {
+    P.row = (long long)(qpsdm_bank_row_bytes(p->K, p->nt, QPSDM_F16P) / 4);
+    P.lead = 4;
+  }
   P.plan = c->plan;
   P.tw = c->tw;
   P.H = c->H;
@@ -588,8 +627,14 @@ int launch_coefs(const qpsdm_bank_params *p, const Coefs *c, const float *traces
   } else if (dtype == QPSDM_F16) {
     hipLaunchKernelGGL((qbank_kernel<1, __half>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
                        (__half *)bank, (float *)nullptr, (float *)nullptr);
+  } else if (dtype == QPSDM_F32P) {
+    hipLaunchKernelGGL((qbank_kernel<1, PairF32>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
+                       (PairF32 *)bank, (float *)nullptr, (float *)nullptr);
+  } else if (dtype == QPSDM_F16P) {
+    hipLaunchKernelGGL((qbank_kernel<1, PairF16>), dim3(ntr), dim3(kThreads), lds, s, P, traces,
+                       (PairF16 *)bank, (float *)nullptr, (float *)nullptr);
   } else {
-    return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16", dtype);
+    return qpsdm::set_error(QPSDM_EINVAL, "bank: dtype %d not F32/BF16/F16/F32P/F16P", dtype);
   }
   QPSDM_HIP(hipGetLastError());
   return QPSDM_OK;
@@ -619,6 +664,8 @@ extern "C" size_t qpsdm_bank_row_bytes(int K, int nt, int dtype) {
     case QPSDM_F32: return n * 4;
     case QPSDM_BF16: return n * 2;
     case QPSDM_F16: return 16 + (n * 2 + 15) / 16 * 16;
+    case QPSDM_F32P: return n * 8;
+    case QPSDM_F16P: return 16 + (n * 4 + 15) / 16 * 16;
     default: return 0;
   }
 }

@@ -95,19 +95,35 @@ class Survey:
         self.ip = ip
         self.tg = ip.tg
 
-    def set_form(self, form="exact", z_exact=400.0):
+    DEFAULT_SPANS = ((0.0, 2), (800.0, 4), (1600.0, 8))
+
+    def set_form(self, form="exact", z_exact=400.0, spans=None):
         """Form of the imaging sum: "exact" (every point evaluates its taper
-        and amplitude) or "nodes" (below z_exact metres, the taper ramps and
+        and amplitude), "nodes" (below z_exact metres, the taper ramps and
         the amplitude are evaluated at the tables' depth nodes and
-        interpolated; times and levels stay exact).  Returns self."""
-        assert form in ("exact", "nodes")
-        self.ip.weight_nodes = 1 if form == "nodes" else 0
+        interpolated; times and levels stay exact) or "spans" (the same with
+        the ramps and amplitude at the ends of spans of table intervals,
+        interpolated in 1/z; spans = [(z, S), ...]: from depth z down, spans
+        of S intervals; batched imaging with pair banks only).  Returns self."""
+        assert form in ("exact", "nodes", "spans")
+        self.ip.weight_nodes = {"exact": 0, "nodes": 1, "spans": 2}[form]
         self.ip.z_exact = float(z_exact)
+        if form == "spans":
+            sp = tuple(spans if spans is not None else self.DEFAULT_SPANS)
+            assert 1 <= len(sp) <= 4
+            self.ip.span_n = len(sp)
+            for k, (z, S) in enumerate(sp):
+                self.ip.span_z[k] = float(z)
+                self.ip.span_s[k] = int(S)
         return self
 
     @property
     def form(self):
-        return "nodes" if self.ip.weight_nodes else "exact"
+        return ("exact", "nodes", "spans")[self.ip.weight_nodes]
+
+    def spans(self):
+        """The span schedule [(z, S), ...] of the span form."""
+        return [(float(self.ip.span_z[k]), int(self.ip.span_s[k])) for k in range(self.ip.span_n)]
 
     # -- tables ----------------------------------------------------------
     def table_shape(self):
@@ -183,34 +199,59 @@ class Survey:
 
     _BANK_CODES = {torch.float32: L.QPSDM_F32, torch.bfloat16: L.QPSDM_BF16, torch.float16: L.QPSDM_F16}
 
-    def bank_shape(self, n, dtype=torch.float32):
+    def bank_shape(self, n, dtype=torch.float32, pair=False):
         """Tensor shape of an n-trace bank: (n, K, nt) for FP32 and BF16; for
         FP16 (n, row) halves per trace, a 16-byte header {1/scale, scale}
-        then the K x nt scaled samples."""
+        then the K x nt scaled samples.  Pair banks (pair=True): FP32
+        (n, K, nt, 2) of (sample, difference to the next); FP16 (n, row)
+        halves, the header then K x nt half pairs."""
         c = self.cfg
+        if pair:
+            if dtype == torch.float32:
+                return (n, c.K, c.nt, 2)
+            assert dtype == torch.float16, "pair banks are FP32 or FP16"
+            return (n, L.lib().qpsdm_bank_row_bytes(c.K, c.nt, L.QPSDM_F16P) // 2)
         if dtype == torch.float16:
             return (n, L.lib().qpsdm_bank_row_bytes(c.K, c.nt, L.QPSDM_F16) // 2)
         return (n, c.K, c.nt)
 
+    def bank_code(self, bank):
+        """Storage code of a bank tensor (pair banks are told by their shape)."""
+        if bank.dtype == torch.float32 and bank.dim() == 4:
+            return L.QPSDM_F32P
+        if bank.dtype == torch.float16 and bank.dim() == 2 and \
+                bank.shape[1] == self.bank_shape(1, torch.float16, pair=True)[1]:
+            return L.QPSDM_F16P
+        return self._BANK_CODES[bank.dtype]
+
     def _check_bank(self, bank, rows):
         assert bank.dtype in self._BANK_CODES and bank.is_contiguous() and bank.shape[0] >= rows
-        assert tuple(bank.shape[1:]) == self.bank_shape(1, bank.dtype)[1:]
+        pair = self.bank_code(bank) in (L.QPSDM_F32P, L.QPSDM_F16P)
+        assert tuple(bank.shape[1:]) == self.bank_shape(1, bank.dtype, pair)[1:]
 
     def decode_bank(self, bank):
-        """Bank in filtered-trace units as FP32 (n, K, nt) (host-side check)."""
+        """Bank in filtered-trace units as FP32 (n, K, nt) (host-side check);
+        of a pair bank, the samples."""
+        c = self.cfg
+        code = self.bank_code(bank)
+        if code == L.QPSDM_F32P:
+            return bank[..., 0].contiguous()
+        if code == L.QPSDM_F16P:
+            hdr = bank[:, :8].contiguous().view(torch.float32)
+            v = bank[:, 8:8 + 2 * c.K * c.nt].float().reshape(-1, c.K, c.nt, 2)[..., 0]
+            return v * hdr[:, 0].reshape(-1, 1, 1)
         if bank.dtype != torch.float16:
             return bank.float()
-        c = self.cfg
         hdr = bank[:, :8].contiguous().view(torch.float32)  # [n][4]: 1/scale, scale, 0, 0
         return bank[:, 8:8 + c.K * c.nt].float().reshape(-1, c.K, c.nt) * hdr[:, 0].reshape(-1, 1, 1)
 
-    def qbank(self, traces, dtype=torch.float32, out=None, stream=None):
+    def qbank(self, traces, dtype=torch.float32, out=None, stream=None, pair=False):
         n = traces.shape[0]
         assert traces.dtype == torch.float32 and traces.dim() == 2 and traces.shape[1] == self.cfg.nt
         if out is None:
-            out = torch.empty(self.bank_shape(n, dtype), dtype=dtype, device="cuda")
+            out = torch.empty(self.bank_shape(n, dtype, pair), dtype=dtype, device="cuda")
         self._check_bank(out, n)
-        code = self._BANK_CODES[out.dtype]
+        code = self.bank_code(out)
         check(L.lib().qpsdm_qbank(C.byref(self.bp), _ptr(traces), n, _ptr(out), code, _stream(stream)))
         return out
 
@@ -284,7 +325,7 @@ class Survey:
         else:
             tcode = getattr(tab, "qpsdm_code", L.QPSDM_F16B)
             assert tab.numel() == L.lib().qpsdm_tables_bytes(C.byref(self.tg), tcode)
-        return self._BANK_CODES[bank.dtype], tcode
+        return self.bank_code(bank), tcode
 
     def image_batch(self, img, geom, bank, tab, shots, launch=None, nwg=0, work=None, src=None, stream=None):
         """Accumulate every shot of a batch into img with the persistent

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqpsdm.so")
 
 QPSDM_F32, QPSDM_BF16, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X, QPSDM_F16, QPSDM_U16Q = 0, 1, 2, 3, 4, 5, 6
+QPSDM_F32P, QPSDM_F16P = 7, 8
 
 
 class Window(C.Structure):
@@ -69,7 +70,8 @@ class ImageParams(C.Structure):
                 ("K", C.c_int), ("dtstar", C.c_float), ("tanx1", C.c_float),
                 ("tanx2", C.c_float), ("tany1", C.c_float), ("tany2", C.c_float),
                 ("taperzone", C.c_float), ("tmin", C.c_float), ("tg", TGeom),
-                ("weight_nodes", C.c_int), ("z_exact", C.c_float)]
+                ("weight_nodes", C.c_int), ("z_exact", C.c_float), ("span_n", C.c_int),
+                ("span_z", C.c_float * 4), ("span_s", C.c_int * 4)]
 
 
 class Launch(C.Structure):

@@ -449,6 +449,77 @@ def test_node_form_is_exact_at_the_nodes_and_converges_with_depth():
     assert errs[0] > 1e-2 and errs[3] < 1.5 * floor
 
 
+# ------------------------------------------------------------------ span form
+def test_span_table_follows_the_band_rule():
+    """Bands start at the first interval at or below each schedule depth,
+    rounded up to a multiple of 8 intervals past the node range's first
+    interval; spans tile each band from its start, the last cut at the band's
+    end (worked by hand for 20 m tables, z_exact 300 m, points to 990 m)."""
+    cfg, tab, ip = _deep_case()
+    sp = O.span_table(ip, 300.0, ((0.0, 2), (500.0, 4), (700.0, 8)))
+    # ws = 15, we = 50: band 2 at [15, 31), band 4 at [31, 39), band 8 at [39, 50)
+    assert tuple(sp[15]) == (15, 17) and tuple(sp[16]) == (15, 17) and tuple(sp[30]) == (29, 31)
+    assert tuple(sp[31]) == (31, 35) and tuple(sp[38]) == (35, 39)
+    assert tuple(sp[39]) == (39, 47) and tuple(sp[46]) == (39, 47) and tuple(sp[47]) == (47, 50)
+    assert tuple(sp[49]) == (47, 50) and tuple(sp[14]) == (14, 15)
+    # a schedule that starts below ws: one-interval spans above it
+    sp = O.span_table(ip, 300.0, ((450.0, 4),))
+    assert tuple(sp[15]) == (15, 16) and tuple(sp[22]) == (22, 23) and tuple(sp[23]) == (23, 27)
+    # bands that start at the same interval: the later one holds
+    sp = O.span_table(ip, 300.0, ((0.0, 2), (300.0, 8)))
+    assert tuple(sp[15]) == (15, 23)
+    # half an interval below ws rounds up to the next multiple of 8
+    sp = O.span_table(ip, 300.0, ((0.0, 2), (310.0, 8)))
+    assert tuple(sp[15]) == (15, 17) and tuple(sp[23]) == (23, 31)
+
+
+def test_span_form_is_exact_for_zero_offset_traces():
+    """Zero-offset traces (source = receiver): the bisector's tangent is
+    (sx - x) / z exactly, so the ramps are linear in 1/z, and tr = 1, so the
+    amplitude is constant: the span form's 1/z interpolation reproduces the
+    exact FP32 path to rounding for spans of any length, where the node
+    form's interpolation linear in z over single intervals does not (a
+    fraction taken linear in z, or with the span's ends swapped, fails)."""
+    cfg, tab, ip = _deep_case()
+    rng = np.random.default_rng(9)
+    xy = rng.uniform(-40.0, 110.0, (10, 2)).astype(np.float32)
+    geom = np.concatenate([xy, xy], 1)
+    bank = rng.standard_normal((geom.shape[0], cfg.K, cfg.nt)).astype(np.float32)
+    e32, _ = O.image(bank, geom, tab, ip, fp32=True)
+    e64, _ = O.image(bank.astype(np.float64), geom, tab, ip)
+    ze = 60.0
+    ps, pe = O.node_range(ip, ze)
+    floor = rel_l2(e32[..., ps:pe], e64[..., ps:pe])  # the FP32 path's own rounding, ~3e-5
+    for sched in (((0.0, 8),), ((0.0, 1), (300.0, 4), (600.0, 8))):
+        img, _ = O.image_spans(bank, geom, tab, ip, ze, O.span_table(ip, ze, sched))
+        assert rel_l2(img[..., ps:pe], e64[..., ps:pe]) < 1.2 * floor
+    nodes, _ = O.image_nodes(bank, geom, tab, ip, ze)
+    assert rel_l2(nodes[..., ps:pe], e64[..., ps:pe]) > 50 * floor
+
+
+def test_span_form_converges_as_spans_shrink():
+    """Offset traces: the span form's error against the FP64 sum falls as
+    the spans shrink (the interpolation error of smooth functions), and with
+    the weight nodes at the points themselves (z_exact below the image) it
+    is the FP32 path."""
+    cfg, tab, ip = _deep_case()
+    rng = np.random.default_rng(6)
+    geom, _ = synth.geometry(synth.shots_tiny(2, 6, extent=(300.0, 300.0)))
+    bank = rng.standard_normal((geom.shape[0], cfg.K, cfg.nt)).astype(np.float32)
+    e32, _ = O.image(bank, geom, tab, ip, fp32=True)
+    e64, _ = O.image(bank.astype(np.float64), geom, tab, ip)
+    ze = 200.0
+    ps, pe = O.node_range(ip, ze)
+    errs = []
+    for S in (8, 4, 2, 1):
+        img, _ = O.image_spans(bank, geom, tab, ip, ze, O.span_table(ip, ze, ((0.0, S),)))
+        errs.append(rel_l2(img[..., ps:pe], e64[..., ps:pe]))
+    assert errs[0] > errs[1] > errs[2] > errs[3]
+    assert errs[0] > 4 * errs[2]
+    below, _ = O.image_spans(bank, geom, tab, ip, 2000.0, O.span_table(ip, 2000.0, ((0.0, 8),)))
+    assert np.array_equal(below, e32)
+
+
 # ------------------------------------------------------ per-shot source tables
 def _cut_source_table(tab, g, sx, sy, nus, nvs):
     """A per-shot source table holding the nearest node's own values on the
