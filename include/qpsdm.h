@@ -43,7 +43,7 @@ extern "C" {
 #define QPSDM_U16Q 6 /* tables: the U16X bricks' codes stored per cell: at each depth node
                         the four bilinear corners' codes of a cell as one 16-byte word
                         (3.1x the U16X bytes; one vector load per table node instead of
-                        four) */
+                        four); the brick header's steps are stored divided by 65535 */
 #define QPSDM_F32P 7 /* bank only: pair bank, sample i of a level row stored with its forward
                         difference as float2 (b[i], b[i+1] - b[i]); entries 0 and nt-1 of
                         every row are (0, 0) (outside the samples the sum may pick);
@@ -143,7 +143,8 @@ size_t qpsdm_tables_bytes(const qpsdm_tgeom *g, int dtype);
  * an offset and a scale per quantity, codes h = (v - off)/scale in [0, 1]
  * stored as FP16 (QPSDM_F16B) or as round(h * 65535) (QPSDM_U16B); or per
  * 9x9x8 edge-sharing brick an offset and a step, codes round((v - off)/step)
- * (QPSDM_U16X), or the same codes as corner quads (QPSDM_U16Q). */
+ * (QPSDM_U16X), or the same codes as corner quads (QPSDM_U16Q, header
+ * {off, step / 65535, off*, step* / 65535}). */
 int qpsdm_tables_quantize(const qpsdm_tgeom *g, const float *tab_dev, void *out_dev, int code,
                           void *stream);
 
@@ -313,8 +314,8 @@ typedef struct {
 
 /* Launch shape picked by the autotuner; a NULL shape or 0 fields take the
  * default 8 x 8 columns x 6 depth points.  In the node form tile_z counts
- * table depth intervals per tile (2, 3 or 4), in the span form too (2, 4 or
- * 8); the exact body's share of those forms runs 8 depth points per tile. */
+ * table depth intervals per tile (2, 3 or 4), in the span form too (4 or 8);
+ * the exact body's share of those forms runs 8 depth points per tile. */
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
 } qpsdm_launch;

@@ -10,7 +10,7 @@ representative shot, timed with HIP events, and the fastest is kept.
 
 Candidates: column tiles 8x8, 16x4 and 4x16 (one wave of columns) times
 depth points per lane ZP in {2, 3, 4, 6, 8} (in the node form: table depth
-intervals per tile in {2, 3, 4}; in the span form {2, 4, 8}); for the
+intervals per tile in {2, 3, 4}; in the span form {4, 8}); for the
 batched kernel also the grid:
 persistent at the resident capacity or twice it, or one work-group per tile
 (nwg = -1; the node form always runs one work-group per tile).  Results are cached per (geometry, storage
@@ -23,7 +23,7 @@ import torch
 TILES = ((8, 8), (16, 4), (4, 16))
 DEPTHS = (2, 3, 4, 6, 8)
 NODE_INTERVALS = (2, 3, 4)  # node form: table depth intervals per tile
-SPAN_INTERVALS = (2, 4, 8)  # span form: table depth intervals per tile
+SPAN_INTERVALS = (4, 8)  # span form: table depth intervals per tile
 GRIDS = (0, -2, -1)  # batched kernel: resident, twice resident (-2), one group per tile (-1)
 _cache = {}
 

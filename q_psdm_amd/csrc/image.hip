@@ -40,7 +40,7 @@ int node_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
 // interval is dropped.
 int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   QPSDM_REQUIRE(p->z_exact >= 0.0f, "image: z_exact must be >= 0");
-  QPSDM_REQUIRE(ni == 2 || ni == 4 || ni == 8, "image: the span form takes tile_z = 2, 4 or 8 intervals per tile");
+  QPSDM_REQUIRE(ni == 4 || ni == 8, "image: the span form takes tile_z = 4 or 8 intervals per tile");
   QPSDM_REQUIRE(p->span_n >= 1 && p->span_n <= 4, "image: the span form takes 1 to 4 bands (span_n)");
   for (int k = 0; k < p->span_n; ++k) {
     const int S = p->span_s[k];
@@ -72,6 +72,15 @@ int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
     o->sp_s[k] = span[k];
   }
   o->sp_w[n] = we;
+  // the 3, 2 slot pattern: every interval of the range (band starts are 8
+  // intervals apart, so the parity from ws is the parity within a tile)
+  int pat = 1;
+  for (int w = ws; w < we && pat; ++w) {
+    const int a = std::max(first_point(w, p->nz, p->dz, o->inv_dw, p->tg.nw), o->nf_ps);
+    const int b = std::min(first_point(w + 1, p->nz, p->dz, o->inv_dw, p->tg.nw), o->nf_pe);
+    if (b - a > (((w - ws) & 1) ? 2 : 3)) pat = 0;
+  }
+  o->sp_pat = pat;
   return QPSDM_OK;
 }
 

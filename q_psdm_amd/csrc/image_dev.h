@@ -100,6 +100,7 @@ struct ImgDev {
   // and the band [sp_b, sp_e) of one launch
   int sp_n, sp_w[6], sp_s[5];
   int sp_b, sp_e;
+  int sp_pat;  // the range's intervals hold at most 3, 2, 3, 2, ... points from its start
 };
 
 int node_range(const qpsdm_image_params *p, int ni, ImgDev *o);
@@ -177,6 +178,7 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->nf_ps = o->nf_pe = p->nz;
   o->sp_n = 0;
   o->sp_b = o->sp_e = 0;
+  o->sp_pat = 0;
   QPSDM_REQUIRE(p->weight_nodes >= 0 && p->weight_nodes <= 2,
                 "image: weight_nodes must be 0 (exact form), 1 (node form) or 2 (span form)");
   if (p->weight_nodes == 1) return node_range(p, tzp, o);
@@ -410,7 +412,7 @@ struct TabU16Q {
     int o;     // the cell's corner word at w = 0 of the brick
     unsigned row0, row1;  // the cell's fixed-point bilinear weights
     int hb;    // depth brick of the cached header
-    float4 h;  // its header, steps divided by 65535 (the weights' sum)
+    float4 h;  // its header (steps divided by 65535, the weights' sum)
   };
   __device__ inline Col column(int node, int iv, int iu, float a, float b) const {
     Col c;
@@ -423,12 +425,10 @@ struct TabU16Q {
     return c;
   }
   typedef TabU16X::Raw Raw;
-  // header of the column's brick sb bytes below depth brick 0, as reduce takes it
+  // header of the column's brick sb bytes below depth brick 0 (its steps
+  // stored divided by 65535, as reduce takes them)
   __device__ static inline float4 header(const Col &c, int sb) {
-    float4 h = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.base, sb, 0));
-    h.y *= 1.0f / 65535.0f;
-    h.w *= 1.0f / 65535.0f;
-    return h;
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.base, sb, 0));
   }
   __device__ inline Raw fetch(Col &c, int w) const {
     const int bw = w >> 3;

@@ -44,6 +44,8 @@
 
 namespace qpsdm_dev {
 
+typedef const __attribute__((address_space(3))) unsigned long long *lds_u64;
+
 // The ramps and amplitude at a weight node.
 struct WVal {
   float u1, u2, u3, u4, a;
@@ -107,7 +109,11 @@ __device__ inline void span_source(const ImgDev &P, const Tab &tab, typename Tab
 // The span tile body: tile_col as in image_tile; group g owns intervals
 // [sp_b + g NI, sp_b + (g + 1) NI) of the band [sp_b, sp_e) (ImgDev),
 // cut at the node range's points [nf_ps, nf_pe).  SRC as in image_tile.
-template <int NI, int S, int PM, bool SRC, typename BankT, typename Tab, typename Runs>
+// PAT: the node range alternates 3 and 2 points per interval from the
+// band's start (a 10 m image under 25 m tables, or any geometry with at
+// most that many), so the slots are 3, 2, 3, 2, ... instead of PM per
+// interval.
+template <int NI, int S, int PM, int PAT, bool SRC, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *__restrict__ img,
                                         const float4 *__restrict__ geom, const BankT *__restrict__ bank,
                                         int tile_col, int group, const Runs &runs, int nrun) {
@@ -118,6 +124,9 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
   constexpr int SI = INSIDE ? S : NI;        // intervals per span inside the tile
   constexpr int NW = NSP + 1;                // weight nodes
   constexpr int CI = SI < 2 ? SI : 2;        // intervals per gather chunk
+  static_assert(!PAT || (PM == 3 && NI % 2 == 0), "the 3, 2 slot pattern takes PM = 3 and an even NI");
+  // point slots of interval m
+  auto nslot = [](int m) { return PAT ? ((m & 1) ? 2 : 3) : PM; };
   __shared__ float2 cst[NI * PM];            // per point slot: (fraction in its interval, in its span)
 
   const int lane = threadIdx.x;
@@ -154,25 +163,27 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
   for (int m = 0; m < NI; ++m)
 #pragma unroll
     for (int p = 0; p < PM; ++p)
-      if (pa[m] + p < pa[m + 1]) on |= 1u << (m * PM + p);
+      if (p < nslot(m) && pa[m] + p < pa[m + 1]) on |= 1u << (m * PM + p);
   on = (unsigned)uniform_int((int)on);
   if (lane < NI * PM) {
+    // a slot that holds no point gets (0, 0): it reads its interval's first
+    // node, inside the tables and the bank
     const int m = lane / PM, p = lane % PM;
     const int k = INSIDE ? m / S : 0;
-    const int iz = min(pa[m] + p, P.nf_pe - 1);
-    float c = 0.0f;
-    point_interval(iz, P.dz, P.inv_dw, P.nw, &c);
-    const float z = (float)iz * P.dz;
-    const float z0 = (float)wk[k] * P.dw, z1 = (float)wk[k + 1] * P.dw;
-    const float sf = (z1 > z0) ? ((z - z0) * z1) / ((z1 - z0) * z) : 0.0f;
-    cst[lane] = make_float2(c, sf);
+    float2 v = make_float2(0.0f, 0.0f);
+    if (pa[m] + p < pa[m + 1]) {
+      const int iz = pa[m] + p;
+      point_interval(iz, P.dz, P.inv_dw, P.nw, &v.x);
+      const float z = (float)iz * P.dz;
+      const float z0 = (float)wk[k] * P.dw, z1 = (float)wk[k + 1] * P.dw;
+      v.y = ((z - z0) * z1) / ((z1 - z0) * z);
+    }
+    cst[lane] = v;
   }
   __syncthreads();
   const int pfirst = pa[0], plast = pa[NI];
   if (pfirst >= plast) return;
-  unsigned onl = on;
-#define QPSDM_ON(m, p) ((onl >> ((m) * PM + (p))) & 1u)
-#define QPSDM_PHASE() asm volatile("" : "+s"(onl))
+#define QPSDM_ON(m, p) ((on >> ((m) * PM + (p))) & 1u)
 
   float acc[NI][PM];
 #pragma unroll
@@ -230,13 +241,18 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
       // time (samples) and level at the table nodes; the receiver time kept
       // at the weight nodes that are table nodes
       float Fn[NI + 1], Ln[NI + 1], tw[NW];
-      {
-        typename Tab::Raw raw[NI + 1];
+      // (in two halves of the nodes: the raw words of all nine nodes of an
+      // eight-interval tile would hold 36 registers at once)
+      constexpr int NH = NI / 2 + 1;
 #pragma unroll
-        for (int m = 0; m <= NI; ++m) raw[m] = tab.fetch(c, wn[m]);
+      for (int h = 0; h < 2; ++h) {
+        const int m0 = h ? NH : 0, m1 = h ? NI + 1 : NH;
+        typename Tab::Raw raw[NH];
 #pragma unroll
-        for (int m = 0; m <= NI; ++m) {
-          const float2 v = tab.reduce(c, raw[m]);
+        for (int m = m0; m < m1; ++m) raw[m - m0] = tab.fetch(c, wn[m]);
+#pragma unroll
+        for (int m = m0; m < m1; ++m) {
+          const float2 v = tab.reduce(c, raw[m - m0]);
           Fn[m] = fmaf(fmaxf(v.x, tmin), P.inv_dt, SC.tsn[m]);
           Ln[m] = __builtin_amdgcn_fmed3f(fmaf(v.y, P.inv_dtstar, SC.tsl[m]), 0.0f, jmax);
           if constexpr (INSIDE) {
@@ -254,9 +270,9 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
         for (int k = 1; k < NW; ++k)
           if (wk[k] != wa + k * S) tw[k] = tab.at(c, wk[k]).x;
       }
-      // the per-slot constants are re-read from LDS in every trace (the asm
-      // keeps the compiler from holding all of them in registers)
-      const float2 *cs = cst;
+      // the per-slot constants are read from LDS in every trace (ds_read;
+      // the asm keeps the compiler from hoisting them into registers)
+      lds_u64 cs = (lds_u64)cst;
       asm volatile("" : "+v"(cs));
       WVal W0 = weight_value(D, R, SC.ds[0], SC.its[0], tw[0], (float)wk[0] * P.dw, SC.dsx, SC.dsy, dgx, dgy, hg2,
                              unit, tmin);
@@ -273,64 +289,61 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
         const bool all_in = __all(!valid || mlo >= 1.0f);
         const WVal dW = wval_sub(W1, W0);
         if (!all_out) {
+          // every slot of the span's intervals, branch-free (a slot that
+          // holds no point reads its interval's first node and accumulates
+          // into a register that is never stored), in chunks of CI
+          // intervals: time, level and weight, the gathers, the sums
 #pragma unroll
           for (int c0 = k * SI; c0 < (k + 1) * SI; c0 += CI) {
-            float cw[CI][PM], cf[CI][PM];
+            float cw[CI][PM], cf[CI][PM], sf[CI][PM];
             int ci[CI][PM];
             typename BankRaw<BankT>::type cb[CI][PM];
-            QPSDM_PHASE();
-            if (all_in) {
 #pragma unroll
-              for (int i = 0; i < CI; ++i) {
-                const int m = c0 + i;
-                const float dF = Fn[m + 1] - Fn[m], dL = Ln[m + 1] - Ln[m];
+            for (int i = 0; i < CI; ++i) {
+              const int m = c0 + i;
+              const float dF = Fn[m + 1] - Fn[m], dL = Ln[m + 1] - Ln[m];
 #pragma unroll
-                for (int p = 0; p < PM; ++p) {
-                  if (!QPSDM_ON(m, p)) continue;
-                  const float2 q = cs[m * PM + p];
-                  const float f = fmaf(q.x, dF, Fn[m]);
-                  const unsigned j = (unsigned)fmaf(q.x, dL, Ln[m]);
-                  const unsigned i0 = min((unsigned)f, ntm1);
-                  cf[i][p] = __builtin_amdgcn_fractf(f);
-                  ci[i][p] = (int)(__umul24(j, nt) + i0);
-                  cw[i][p] = fmaf(q.y, dW.a, W0.a);
-                }
-              }
-            } else {
-#pragma unroll
-              for (int i = 0; i < CI; ++i) {
-                const int m = c0 + i;
-                const float dF = Fn[m + 1] - Fn[m], dL = Ln[m + 1] - Ln[m];
-#pragma unroll
-                for (int p = 0; p < PM; ++p) {
-                  if (!QPSDM_ON(m, p)) continue;
-                  const float2 q = cs[m * PM + p];
-                  const float f = fmaf(q.x, dF, Fn[m]);
-                  const unsigned j = (unsigned)fmaf(q.x, dL, Ln[m]);
-                  const unsigned i0 = min((unsigned)f, ntm1);
-                  const float s = q.y;
-                  const float wx = sat(fminf(fmaf(s, dW.u1, W0.u1), fmaf(s, dW.u2, W0.u2)));
-                  const float wy = sat(fminf(fmaf(s, dW.u3, W0.u3), fmaf(s, dW.u4, W0.u4)));
-                  cf[i][p] = __builtin_amdgcn_fractf(f);
-                  ci[i][p] = (int)(__umul24(j, nt) + i0);
-                  cw[i][p] = (wx * wy) * fmaf(s, dW.a, W0.a);
-                }
+              for (int p = 0; p < PM; ++p) {
+                if (p >= nslot(m)) continue;
+                const float2 q = __builtin_bit_cast(float2, cs[m * PM + p]);
+                const float f = fmaf(q.x, dF, Fn[m]);
+                const unsigned j = (unsigned)fmaf(q.x, dL, Ln[m]);
+                const unsigned i0 = min((unsigned)f, ntm1);
+                cf[i][p] = __builtin_amdgcn_fractf(f);
+                ci[i][p] = (int)(__umul24(j, nt) + i0);
+                sf[i][p] = q.y;
               }
             }
-            QPSDM_PHASE();
+            // the weights: one branch per chunk on the span's class
+            if (all_in) {
+#pragma unroll
+              for (int i = 0; i < CI; ++i)
+#pragma unroll
+                for (int p = 0; p < PM; ++p)
+                  if (p < nslot(c0 + i)) cw[i][p] = fmaf(sf[i][p], dW.a, W0.a);
+            } else {
+#pragma unroll
+              for (int i = 0; i < CI; ++i)
+#pragma unroll
+                for (int p = 0; p < PM; ++p) {
+                  if (p >= nslot(c0 + i)) continue;
+                  const float s = sf[i][p];
+                  const float wx = sat(fminf(fmaf(s, dW.u1, W0.u1), fmaf(s, dW.u2, W0.u2)));
+                  const float wy = sat(fminf(fmaf(s, dW.u3, W0.u3), fmaf(s, dW.u4, W0.u4)));
+                  cw[i][p] = (wx * wy) * fmaf(s, dW.a, W0.a);
+                }
+            }
 #pragma unroll
             for (int i = 0; i < CI; ++i)
 #pragma unroll
               for (int p = 0; p < PM; ++p)
-                if (QPSDM_ON(c0 + i, p)) cb[i][p] = bank_raw(br, ci[i][p], bank);
-            QPSDM_PHASE();
+                if (p < nslot(c0 + i)) cb[i][p] = bank_raw(br, ci[i][p], bank);
 #pragma unroll
             for (int i = 0; i < CI; ++i)
 #pragma unroll
-              for (int p = 0; p < PM; ++p) {
-                if (!QPSDM_ON(c0 + i, p)) continue;
-                acc[c0 + i][p] = fmaf(cw[i][p], bank_interp(cb[i][p], cf[i][p], bank), acc[c0 + i][p]);
-              }
+              for (int p = 0; p < PM; ++p)
+                if (p < nslot(c0 + i))
+                  acc[c0 + i][p] = fmaf(cw[i][p], bank_interp(cb[i][p], cf[i][p], bank), acc[c0 + i][p]);
           }
         }
         W0 = W1;
@@ -344,7 +357,6 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
       for (int p = 0; p < PM; ++p)
         if (QPSDM_ON(m, p)) img[col + pa[m] + p] = acc[m][p];
   }
-#undef QPSDM_PHASE
 #undef QPSDM_ON
 }
 

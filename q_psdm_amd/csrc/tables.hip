@@ -140,8 +140,10 @@ __global__ __launch_bounds__(256) void quant_x_kernel(qpsdm_tgeom g, int nbv, in
     if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
     for (int k = threadIdx.x; k < N; k += blockDim.x) *(unsigned *)(o + 16 + k * 4) = code(k);
   } else {
+    // the steps divided by 65535, the sum of a cell's fixed-point bilinear
+    // weights (a reader decodes a weighted code sum with one fma)
     unsigned char *o = out + b * qpsdm_dev::kQBrickBytes;
-    if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst, s0, sst);
+    if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst * (1.0f / 65535.0f), s0, sst * (1.0f / 65535.0f));
     // cell (lv, lu) at depth lw: corners k = ((lv [+1]) * 9 + lu [+1]) * 8 + lw of the 9 x 9 x 8 nodes
     for (int c = threadIdx.x; c < 8 * 8 * 8; c += blockDim.x) {
       const int lw = c & 7, lu = (c >> 3) & 7, lv = c >> 6;

@@ -278,9 +278,10 @@ def test_u16x_image_matches_f32_tables(gpu, mk):  # tables' reads are pinned by 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])
 def test_u16q_holds_the_u16x_codes(gpu, mk):
-    """The corner-quad layout stores each U16X brick's header and codes: every
-    cell's word at every depth node holds the codes of its four corners, read
-    back from the U16X brick that owns the cell."""
+    """The corner-quad layout stores each U16X brick's header (its steps
+    divided by 65535) and codes: every cell's word at every depth node holds
+    the codes of its four corners, read back from the U16X brick that owns the
+    cell."""
     cfg = mk()
     sv, tab = _survey(cfg)
     g = cfg.tgeom
@@ -289,7 +290,10 @@ def test_u16q_holds_the_u16x_codes(gpu, mk):
     nn = g["nnx"] * g["nny"]
     x = sv.quantize_tables(tab, code="u16x").cpu().numpy().reshape(nn, nbv, nbu, nbw, 16 + 648 * 4)
     q = sv.quantize_tables(tab, code="u16q").cpu().numpy().reshape(nn, nbv, nbu, nbw, 16 + 512 * 16)
-    assert np.array_equal(x[..., :16], q[..., :16])
+    hx = x[..., :16].copy().view(np.float32)
+    hq = q[..., :16].copy().view(np.float32)
+    assert np.array_equal(hq[..., 0::2], hx[..., 0::2])  # offsets
+    assert np.array_equal(hq[..., 1::2], hx[..., 1::2] * np.float32(1.0 / 65535.0))  # steps / 65535
     xc = x[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 9, 9, 8)
     qc = q[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 8, 8, 8, 4)
     for k, (dv, du) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):

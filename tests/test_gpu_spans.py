@@ -15,13 +15,14 @@ from tests.helpers import o_bparams, o_iparams, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _deep_survey(nw=52):
+def _deep_survey(nw=52, dw=20.0):
     """A 1 km deep v(z) survey (8 x 6 columns x 100 depths at 10 m, 20 m
-    tables): depth enough for every band and span length to matter."""
+    tables): depth enough for every band and span length to matter.  With
+    30 m tables every interval holds 3 points (not the 3, 2 slot pattern)."""
     cfg = synth.tiny()
     cfg.nx, cfg.ny, cfg.nz = 8, 6, 100
     cfg.nt = 1201
-    cfg.tgeom = dict(cfg.tgeom, nw=nw)
+    cfg.tgeom = dict(cfg.tgeom, nw=nw, dw=dw)
     probe = engine.Survey(cfg, 0.01)
     tab = probe.build_tables()
     return cfg, engine.Survey(cfg, engine.tstar_max_from_tables(tab)), tab
@@ -74,11 +75,12 @@ def _span_image(sv, cfg, geom, ranges, bank, tab, ni, src=None):
     return img, shots, g
 
 
-@pytest.mark.parametrize("ni", [2, 4, 8])
+@pytest.mark.parametrize("ni", [4, 8])
 @pytest.mark.parametrize("sched", [((0.0, 1),), ((0.0, 2), (500.0, 4), (700.0, 8)), ((0.0, 8),),
                                    ((450.0, 4),)])
-def test_span_form_matches_oracle(gpu, ni, sched):
-    cfg, sv, tab = _deep_survey()
+@pytest.mark.parametrize("dw", [20.0, 30.0])
+def test_span_form_matches_oracle(gpu, ni, sched, dw):
+    cfg, sv, tab = _deep_survey(nw=52 if dw == 20.0 else 36, dw=dw)
     geom, ranges, tr = _data(cfg)
     x = torch.from_numpy(tr).cuda()
     b32 = sv.qbank(x)
@@ -92,7 +94,7 @@ def test_span_form_matches_oracle(gpu, ni, sched):
     assert rel_l2(img.cpu().numpy(), ref) < 1e-5
 
 
-@pytest.mark.parametrize("ni", [2, 8])
+@pytest.mark.parametrize("ni", [4, 8])
 def test_span_form_with_source_tables_matches_oracle(gpu, ni):
     """Each shot's source leg from its own source table (the four-node
     blend), a trace of a foreign source in the gather skipped."""
@@ -172,7 +174,8 @@ def test_span_form_rejects_other_storage_and_per_shot_calls(gpu):
     from q_psdm_amd.lib import QpsdmError
     with pytest.raises(QpsdmError):  # a plain bank
         sv.image_batch(sv.new_image(), g, sv.qbank(x), tab, shots, launch=(8, 8, 4), nwg=-1)
-    with pytest.raises(QpsdmError):  # tile_z outside 2, 4, 8
-        sv.image_batch(sv.new_image(), g, sv.qbank(x, pair=True), tab, shots, launch=(8, 8, 3), nwg=-1)
+    for tz in (2, 3):  # tile_z outside 4, 8
+        with pytest.raises(QpsdmError):
+            sv.image_batch(sv.new_image(), g, sv.qbank(x, pair=True), tab, shots, launch=(8, 8, tz), nwg=-1)
     with pytest.raises(QpsdmError):  # the per-shot entry point
         sv.image_shot(sv.new_image(), g, 0, ranges[0][1], sv.qbank(x, pair=True), tab)
