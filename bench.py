@@ -9,11 +9,13 @@ U[30, 200].  A step images `--shots-per-step` shots (Q bank of their traces
 forms image the same shots:
 
   fp32   FP32 bank, FP32 tables, exact form, one launch per shot
-  mixed  FP16 bank (a power-of-two scale per trace), 16-bit fixed-point
-         tables stored as corner quads (U16Q), the batched kernel under
-         captured hipGraphs; the form of the sum (exact, or the node form:
-         taper ramps and amplitude at table depth nodes below z_exact) and
-         the launch shape are picked by the autotuner
+  mixed  FP16 pair bank (sample and difference per entry, a power-of-two
+         scale per trace), 16-bit fixed-point tables stored as corner quads
+         (U16Q), the batched kernel under captured hipGraphs, the span form
+         of the sum (below z_exact the taper ramps and amplitude at the ends
+         of spans of table intervals, interpolated in 1/z; --mixed-form picks
+         another form, auto the autotuner's fastest); the launch shape is
+         picked by the autotuner
   fast   mixed + the learned source leg: per-shot source tables from the
          BF16-MFMA surrogate (trained on the device first; table fallback
          where the network misses by more than 0.5 ms), in the same graphs
@@ -64,11 +66,13 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def kernel_name(form, nwg, node_form):
+def kernel_name(form, nwg, sum_form):
     """Main imaging kernel a form launches."""
     if form == "fp32":
-        return "image_nodes_kernel" if node_form else "image_kernel"
-    if node_form:
+        return "image_nodes_kernel" if sum_form == "nodes" else "image_kernel"
+    if sum_form == "spans":
+        return "image_batch_spans_kernel"
+    if sum_form == "nodes":
         return "image_batch_nodes_kernel"
     return "image_batch_tiles_kernel" if nwg == -1 else "image_batch_kernel"
 
@@ -88,7 +92,7 @@ class Workload:
         tsm = engine.tstar_max_from_tables(self.tab)
         self.sv = engine.Survey(cfg, tsm)                                   # fp32: exact form
         self.svm = engine.Survey(cfg, tsm).set_form(args.mixed_form if args.mixed_form != "auto" else "exact",
-                                                    args.z_exact)           # mixed / fast
+                                                    args.z_exact, parse_spans(args.spans))  # mixed / fast
         log(f"tables {tuple(self.tab.shape)} {self.tab.numel() * 4 / 1e9:.2f} GB in {time.time() - t0:.1f}s; "
             f"dtstar={self.sv.dtstar * 1e3:.3f} ms nsb={self.sv.win.nsb} nww={self.sv.win.nww}")
         self.tab_u16 = self.sv.quantize_tables(self.tab, code="u16q")
@@ -117,6 +121,11 @@ class Workload:
                                    shots=engine.ShotList(desc), updates=sum(self.counts[i] for i in idx)))
 
 
+def parse_spans(text):
+    """--spans "z:S,z:S,...": from depth z (m) down, spans of S table intervals."""
+    return [(float(a), int(b)) for a, b in (t.split(":") for t in text.split(",") if t)]
+
+
 def tune_form(form, W, args, torch, engine):
     """Launch shape (and, for the mixed forms with --mixed-form auto, the form
     of the sum) from the autotuner on a 400-trace slice of the first shot,
@@ -131,17 +140,17 @@ def tune_form(form, W, args, torch, engine):
         launch, nwg, res = autotune.tune(W.sv, st["geom"], bank, W.tab, ntr=n, kind="shot")
         best = min(r[2] for r in res)
     else:
-        bank = W.svm.qbank(st["traces"][:n], dtype=torch.float16)
         cands = []
-        forms = ("exact", "nodes") if args.mixed_form == "auto" else (args.mixed_form,)
+        forms = ("exact", "nodes", "spans") if args.mixed_form == "auto" else (args.mixed_form,)
         for f in forms:
-            W.svm.set_form(f, args.z_exact)
+            W.svm.set_form(f, args.z_exact, parse_spans(args.spans))
+            bank = W.svm.qbank(st["traces"][:n], dtype=torch.float16, pair=W.svm.pair_bank())
             launch, nwg, res = autotune.tune(W.svm, st["geom"], bank, W.tab_u16, shots=st["shots"], ntr=n,
                                              kind="batch")
             cands.append((min(r[2] for r in res), f, launch, nwg))
             log(f"[{form}] autotune {f} form: {launch[:3]} nwg {nwg} {cands[-1][0]:.1f} ms on the slice")
         best, f, launch, nwg = min(cands)
-        W.svm.set_form(f, args.z_exact)
+        W.svm.set_form(f, args.z_exact, parse_spans(args.spans))
     log(f"[{form}] launch {launch[:3]} nwg {nwg or 'resident'} form {W.svm.form if form != 'fp32' else 'exact'} "
         f"({best:.1f} ms on the slice)")
     return launch, nwg
@@ -175,7 +184,8 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
         graphs = []
         for k in range(W.nstep):
             st = W.steps[k]
-            bank = torch.empty(sv.bank_shape(st["traces"].shape[0], torch.float16), dtype=torch.float16, device="cuda")
+            bank = torch.empty(sv.bank_shape(st["traces"].shape[0], torch.float16, pair=sv.pair_bank()),
+                               dtype=torch.float16, device="cuda")
             su = sur if form == "fast" else None
             g_pre = engine.BatchGraph(sv, st["traces"], bank, img, st["geom"], W.tab_u16, st["shots"], launch=launch,
                                       nwg=nwg, surrogate=su, tab_f32=W.tab, mask=mask, stages=3)
@@ -214,10 +224,11 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
     # per shot (one imaging dispatch per shot in every form, plus the exact
     # body's shallow dispatch in the node form), as rocprofv3 averages it
     avg_shot_ms = float(np.mean(img_ms)) / per
-    node_form = sv.form == "nodes"
+    node_form = sv.form in ("nodes", "spans")
     stats = {"value": upd / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "launch": list(launch[:3]),
              "nwg": nwg, "form": sv.form, "z_exact": sv.ip.z_exact if node_form else None,
-             "imaging_kernel": kernel_name(form, nwg, node_form), "imaging_ms_per_shot": avg_shot_ms,
+             "spans": sv.spans() if sv.form == "spans" else None,
+             "imaging_kernel": kernel_name(form, nwg, sv.form), "imaging_ms_per_shot": avg_shot_ms,
              "updates_per_shot": upd / (len(img_ms) * per),
              "imaging_share_of_step": sum(img_ms) / (elapsed * 1e3)}
     if form != "fp32":
@@ -296,7 +307,7 @@ def accuracy_vs_fp64(W, args, forms, shapes, sur, mask, torch, engine):
         imgs["fp32"] = img.cpu().numpy()
         out["fp32_vs_fp64"] = rel_l2(imgs["fp32"], ref_nodes)
     sv = W.svm
-    b16 = sv.qbank(x, dtype=torch.float16)
+    b16 = sv.qbank(x, dtype=torch.float16, pair=sv.pair_bank())
     if "mixed" in forms:
         img = sv.new_image()
         sv.image_batch(img, g_d, b16, W.tab_u16, shots, launch=shapes["mixed"][0], nwg=shapes["mixed"][1])
@@ -310,11 +321,12 @@ def accuracy_vs_fp64(W, args, forms, shapes, sur, mask, torch, engine):
         img = sv.new_image()
         sv.image_batch(img, g_d, b16, W.tab_u16, shots, launch=shapes["fast"][0], nwg=shapes["fast"][1], src=src)
         out["fast_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_src)
-    if "mixed" in forms and sv.form == "nodes":
-        # the node form alone: FP32 bank and FP32 tables in the node form
+    if "mixed" in forms and sv.form in ("nodes", "spans"):
+        # the form of the sum alone: FP32 bank (pairs for the span form) and
+        # FP32 tables
         img = sv.new_image()
-        sv.image_batch(img, g_d, sv.qbank(x), W.tab, shots, launch=shapes["mixed"][0], nwg=-1)
-        out["node_form_fp32_storage_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_nodes)
+        sv.image_batch(img, g_d, sv.qbank(x, pair=sv.pair_bank()), W.tab, shots, launch=shapes["mixed"][0], nwg=-1)
+        out[f"{sv.form}_form_fp32_storage_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_nodes)
     if "mixed" in forms:
         # config 3 as specified (BF16 bank, FP16 codes in 8^3 bricks under an
         # FP32 offset and scale), exact form, per shot: why the mixed form
@@ -387,7 +399,8 @@ def source_leg_model_error(W, sur, mask, forms, shapes, torch, engine):
 def unique_bytes_per_update(W, form_sv, torch, nsample=24, zp=8):
     """HBM bytes a tile must read at least once per trace, per update: for a
     sample of 8x8-column x zp-depth tiles and every trace of shot 0 that
-    updates them, the distinct FP16 bank samples read (2 B each), the
+    updates them, the distinct FP16 bank samples read (2 B each; a pair
+    bank's sample-and-difference entries, 4 B each), the
     distinct U16Q corner words of the receiver table (16 B each per depth
     node, plus a 16 B brick header per cell brick), divided by the updates
     (points inside the dip aperture); plus the image read and write once per
@@ -432,7 +445,12 @@ def unique_bytes_per_update(W, form_sv, torch, nsample=24, zp=8):
             lev = np.clip(np.floor((tsrc[:, 1] + trec[:, 1]) / sv.dtstar + 0.5), 0, cfg.K - 1)
             i0 = np.floor(t / cfg.dt)
             ok = inside & (i0 > 0) & (i0 + 1 < cfg.nt)
-            samp = np.unique(np.concatenate([lev[ok] * cfg.nt + i0[ok], lev[ok] * cfg.nt + i0[ok] + 1]))
+            if form_sv.pair_bank():
+                samp = np.unique(lev[ok] * cfg.nt + i0[ok])
+                sbytes = 4
+            else:
+                samp = np.unique(np.concatenate([lev[ok] * cfg.nt + i0[ok], lev[ok] * cfg.nt + i0[ok] + 1]))
+                sbytes = 2
             # receiver table: corner cells under the 64 columns, depth nodes spanned
             iu = np.floor((xs - gx - g["u0"]) / g["du"]).astype(int)
             iv = np.floor((ys - gy - g["v0"]) / g["dv"]).astype(int)
@@ -440,7 +458,7 @@ def unique_bytes_per_update(W, form_sv, torch, nsample=24, zp=8):
             w0, w1 = int(zs[0] // g["dw"]), int(zs[-1] // g["dw"]) + 1
             nodes = w1 - w0 + 1
             bricks = len(np.unique(iu // 8)) * len(np.unique(iv // 8)) * len(np.unique(np.arange(w0, w1 + 1) // 8))
-            tile_bytes += 2 * samp.size + 16 * cells * nodes + 16 * bricks
+            tile_bytes += sbytes * samp.size + 16 * cells * nodes + 16 * bricks
             tot_upd += m
         tot_bytes += tile_bytes
     return tot_bytes / max(tot_upd, 1)
@@ -481,6 +499,7 @@ def counter_run(counters, args, tmp, tag):
     cmd = [rp, "--pmc"] + counters + ["--output-format", "csv", "-d", d, "-o", "probe", "--", sys.executable,
                                       os.path.join(ROOT, "bench.py"), "--probe", args.head,
                                       "--probe-form", args.head_form, "--z-exact", str(args.z_exact),
+                                      "--spans", args.spans,
                                       "--shots-per-step", str(args.shots_per_step),
                                       "--tile-x", str(args.head_shape[0][0]), "--tile-y", str(args.head_shape[0][1]),
                                       "--ztile", str(args.head_shape[0][2]), "--nwg", str(args.head_shape[1]),
@@ -687,7 +706,8 @@ def time_config4(args, torch, engine, synth, W):
         xt = x[r0:r1]
         gg = g[r0:r1]
         sl = engine.ShotList([(ranges[i][0] - r0, ranges[i][1]) + desc[i][2:] for i in bidx])
-        bank = torch.empty(svm.bank_shape(r1 - r0, torch.float16), dtype=torch.float16, device="cuda")
+        bank = torch.empty(svm.bank_shape(r1 - r0, torch.float16, pair=svm.pair_bank()), dtype=torch.float16,
+                           device="cuda")
         graphs.append(engine.BatchGraph(svm, xt, bank, img, gg, W.tab_u16, sl, launch=launch, nwg=nwg, stages=5))
     for gr in graphs:
         gr.launch()
@@ -827,9 +847,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots-per-step", type=int, default=2)
     ap.add_argument("--forms", default="fp32,mixed,fast")
-    ap.add_argument("--mixed-form", choices=["auto", "exact", "nodes"], default="auto",
-                    help="form of the sum for the mixed and fast forms (auto: the autotuner's faster)")
-    ap.add_argument("--z-exact", type=float, default=400.0, help="node form: depth imaged exactly (m)")
+    ap.add_argument("--mixed-form", choices=["auto", "exact", "nodes", "spans"], default="spans",
+                    help="form of the sum for the mixed and fast forms (auto: the autotuner's fastest)")
+    ap.add_argument("--z-exact", type=float, default=400.0, help="node and span forms: depth imaged exactly (m)")
+    ap.add_argument("--spans", default="0:1,800:2,1600:4",
+                    help="span form: z:S,... -- from depth z (m) down, weights at the ends of spans of S table "
+                         "intervals")
     ap.add_argument("--ztile", type=int, default=0, help="pin depth points (node form: intervals) per tile")
     ap.add_argument("--tile-x", type=int, default=8)
     ap.add_argument("--tile-y", type=int, default=8)
@@ -908,6 +931,7 @@ def main():
         if not args.no_extras:
             cpu["config1"] = cpu_config1(c1data, args)
     hr = roof.get(value_form, {})
+    bank_kind = "pair bank (sample, difference)" if W.svm.pair_bank() else "bank"
     out = {
         "metric": "image-point x trace updates/s @1 GPU (FP32 vs mixed+surrogate+batched, <=1e-3 rel-L2 vs FP64)",
         "value": R["value"],
@@ -925,10 +949,11 @@ def main():
         "config": {"workload": "401x401x301 image @10 m; 2000 receivers/shot over 4x4 km; nt=2001 @2 ms; K=32; "
                                "25 m tables on 5x5 nodes; layered+anomaly model with randomized Q",
                    "form": value_form, "sum_form": R["form"], "z_exact": R["z_exact"],
-                   "storage": {"fast": "f16 bank (per-trace power-of-two scale), u16 offset+step tables as "
-                                       "corner quads, f32 per-shot source tables from the bf16-mfma surrogate",
-                               "mixed": "f16 bank (per-trace power-of-two scale), u16 offset+step tables as "
-                                        "corner quads",
+                   "spans": R.get("spans"),
+                   "storage": {"fast": f"f16 {bank_kind} (per-trace power-of-two scale), u16 offset+step tables "
+                                       "as corner quads, f32 per-shot source tables from the bf16-mfma surrogate",
+                               "mixed": f"f16 {bank_kind} (per-trace power-of-two scale), u16 offset+step tables "
+                                        "as corner quads",
                                "fp32": "f32 bank, f32 tables"}[value_form],
                    "shots_per_step": args.shots_per_step, "tile": R["launch"],
                    "updates_per_step": R["updates_per_shot"] * args.shots_per_step,

@@ -41,9 +41,10 @@ extern "C" {
                         trace's row is a 16-byte header {float 1/scale, float scale, 0, 0}
                         then [K][nt] f16, rows qpsdm_bank_row_bytes() apart */
 #define QPSDM_U16Q 6 /* tables: the U16X bricks' codes stored per cell: at each depth node
-                        the four bilinear corners' codes of a cell as one 16-byte word
-                        (3.1x the U16X bytes; one vector load per table node instead of
-                        four); the brick header's steps are stored divided by 65535 */
+                        the four bilinear corners' codes of a cell as one 16-byte word,
+                        u16 halves t00 t01 t10 t11 t*00 t*01 t*10 t*11 (3.1x the U16X
+                        bytes; one vector load per table node instead of four); the brick
+                        header's steps are stored divided by 65535 */
 #define QPSDM_F32P 7 /* bank only: pair bank, sample i of a level row stored with its forward
                         difference as float2 (b[i], b[i+1] - b[i]); entries 0 and nt-1 of
                         every row are (0, 0) (outside the samples the sum may pick);

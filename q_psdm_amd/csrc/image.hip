@@ -74,13 +74,17 @@ int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   o->sp_w[n] = we;
   // the 3, 2 slot pattern: every interval of the range (band starts are 8
   // intervals apart, so the parity from ws is the parity within a tile)
-  int pat = 1;
-  for (int w = ws; w < we && pat; ++w) {
+  auto fits = [&](int w) {
     const int a = std::max(first_point(w, p->nz, p->dz, o->inv_dw, p->tg.nw), o->nf_ps);
     const int b = std::min(first_point(w + 1, p->nz, p->dz, o->inv_dw, p->tg.nw), o->nf_pe);
-    if (b - a > (((w - ws) & 1) ? 2 : 3)) pat = 0;
-  }
+    return b - a <= (((w - ws) & 1) ? 2 : 3);
+  };
+  int pat = 1;
+  for (int w = ws; w < we - 1 && pat; ++w) pat = fits(w);
   o->sp_pat = pat;
+  // (the last interval can hold the points down to the tables' last node:
+  // one more than the pattern)
+  o->sp_pat_last = pat && fits(we - 1);
   return QPSDM_OK;
 }
 

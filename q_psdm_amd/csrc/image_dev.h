@@ -99,8 +99,10 @@ struct ImgDev {
   // band k covers intervals [sp_w[k], sp_w[k + 1]) in spans of sp_s[k]),
   // and the band [sp_b, sp_e) of one launch
   int sp_n, sp_w[6], sp_s[5];
-  int sp_b, sp_e;
-  int sp_pat;  // the range's intervals hold at most 3, 2, 3, 2, ... points from its start
+  int sp_b, sp_e, sp_g0;  // a launch's band and its first tile group
+  // the range's intervals (sp_pat: all but the last; sp_pat_last: the last
+  // too) hold at most 3, 2, 3, 2, ... points from its start
+  int sp_pat, sp_pat_last;
 };
 
 int node_range(const qpsdm_image_params *p, int ni, ImgDev *o);
@@ -177,8 +179,8 @@ inline int make_dev(const qpsdm_image_params *p, const qpsdm_launch *sh, int tr0
   o->nf_ws = o->nf_we = o->nf_ngroups = o->nf_pm = 0;
   o->nf_ps = o->nf_pe = p->nz;
   o->sp_n = 0;
-  o->sp_b = o->sp_e = 0;
-  o->sp_pat = 0;
+  o->sp_b = o->sp_e = o->sp_g0 = 0;
+  o->sp_pat = o->sp_pat_last = 0;
   QPSDM_REQUIRE(p->weight_nodes >= 0 && p->weight_nodes <= 2,
                 "image: weight_nodes must be 0 (exact form), 1 (node form) or 2 (span form)");
   if (p->weight_nodes == 1) return node_range(p, tzp, o);
@@ -369,12 +371,13 @@ struct TabU16X {
   __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
 };
 
-// Corner-quad bricks (QPSDM_U16Q): the U16X bricks, same headers and
-// codes, with the codes stored per cell: brick (bv, bu, bw) is the float4
-// header, then for cells v in [8 bv, 8 bv + 8), u in [8 bu, 8 bu + 8) and
-// depth nodes w in [8 bw, 8 bw + 8), in [v][u][w] order, the codes of the
-// cell's corners (v, u), (v, u + 1), (v + 1, u), (v + 1, u + 1) as one
-// 16-byte word.  A column reads one word per depth node (the corners share
+// Corner-quad bricks (QPSDM_U16Q): the U16X bricks, same codes (header
+// steps stored divided by 65535), with the codes stored per cell: brick
+// (bv, bu, bw) is the float4 header, then for cells v in [8 bv, 8 bv + 8),
+// u in [8 bu, 8 bu + 8) and depth nodes w in [8 bw, 8 bw + 8), in [v][u][w]
+// order, the codes of the cell's corners 00 = (v, u), 01 = (v, u + 1),
+// 10 = (v + 1, u), 11 = (v + 1, u + 1) as one 16-byte word
+// {t00 | t01 << 16, t10 | t11 << 16, t*00 | t*01 << 16, t*10 | t*11 << 16}.  A column reads one word per depth node (the corners share
 // a cache line with the next seven nodes) instead of four: the imaging
 // kernel's vector-memory pipeline, not its arithmetic, sets its pace.
 constexpr int kQBrickBytes = 16 + 8 * 8 * 8 * 16;
@@ -440,16 +443,13 @@ struct TabU16Q {
     const uint4 q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.o, sb + (w & 7) * 16, 0));
     return Raw{q.x, q.y, q.z, q.w, c.h};
   }
-  // Each row's two t codes (low halves) and two t* codes (high halves) are
-  // paired by byte permutes and weighted by one 16-bit dot product each;
-  // the weighted code sum decodes as off + (step / 65535) * sum.
+  // The word holds each row's two t codes and two t* codes already paired
+  // ({t00 | t01, t10 | t11, s00 | s01, s10 | s11}), each pair weighted by
+  // one 16-bit dot product; the weighted code sum decodes as
+  // off + (step / 65535) * sum.
   __device__ inline float2 reduce(const Col &c, const Raw &q) const {
-    const unsigned t0 = __builtin_amdgcn_perm(q.q01, q.q00, 0x05040100u);
-    const unsigned t1 = __builtin_amdgcn_perm(q.q11, q.q10, 0x05040100u);
-    const unsigned s0 = __builtin_amdgcn_perm(q.q01, q.q00, 0x07060302u);
-    const unsigned s1 = __builtin_amdgcn_perm(q.q11, q.q10, 0x07060302u);
-    const float t = (float)dot2_u16(t1, c.row1, dot2_u16(t0, c.row0, 0u));
-    const float s = (float)dot2_u16(s1, c.row1, dot2_u16(s0, c.row0, 0u));
+    const float t = (float)dot2_u16(q.q01, c.row1, dot2_u16(q.q00, c.row0, 0u));
+    const float s = (float)dot2_u16(q.q11, c.row1, dot2_u16(q.q10, c.row0, 0u));
     return make_float2(fmaf(q.h.y, t, q.h.x), fmaf(q.h.w, s, q.h.z));
   }
   __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }

@@ -107,8 +107,9 @@ __device__ inline void span_source(const ImgDev &P, const Tab &tab, typename Tab
 }
 
 // The span tile body: tile_col as in image_tile; group g owns intervals
-// [sp_b + g NI, sp_b + (g + 1) NI) of the band [sp_b, sp_e) (ImgDev),
-// cut at the node range's points [nf_ps, nf_pe).  SRC as in image_tile.
+// [sp_b + (sp_g0 + g) NI, sp_b + (sp_g0 + g + 1) NI) of the band
+// [sp_b, sp_e) (ImgDev), cut at the node range's points [nf_ps, nf_pe).
+// SRC as in image_tile.
 // PAT: the node range alternates 3 and 2 points per interval from the
 // band's start (a 10 m image under 25 m tables, or any geometry with at
 // most that many), so the slots are 3, 2, 3, 2, ... instead of PM per
@@ -140,7 +141,7 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
 
   // wave-uniform tile geometry: table nodes, weight nodes, each interval's
   // first point, and per point slot its two fractions (in LDS)
-  const int wa = P.sp_b + group * NI;
+  const int wa = P.sp_b + (P.sp_g0 + group) * NI;
   int wn[NI + 1], pa[NI + 1], wk[NW];
 #pragma unroll
   for (int m = 0; m <= NI; ++m) {

@@ -22,7 +22,32 @@ __global__ __launch_bounds__(64) void image_batch_spans_kernel(qk::ImgDev P, Tab
   qk::image_tile_spans<NI, S, 3, PAT, SRC>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, one, 1);
 }
 
+// One band's tile groups [g0, g1) (ImgDev sp_b, sp_e set), span S.
 template <int NI, int PAT, bool SRC, typename BankT, typename Tab>
+int launch_band(qk::ImgDev B, int S, int g0, int g1, int ncol, const Tab &tab, const BankT *bank, const Go &go,
+                int r) {
+  if (g1 <= g0) return QPSDM_OK;
+  B.sp_g0 = g0;
+  const dim3 grid(ncol, g1 - g0);
+  switch (S) {
+#define QPSDM_SPAN(s_)                                                                                          \
+  case s_:                                                                                                      \
+    hipLaunchKernelGGL((image_batch_spans_kernel<NI, s_, PAT, BankT, Tab, SRC>), grid, dim3(64), 0, go.s, B, tab, \
+                       go.img, go.geom, bank, go.runs, r);                                                      \
+    return QPSDM_OK;
+    QPSDM_SPAN(1)
+    QPSDM_SPAN(2)
+    QPSDM_SPAN(4)
+    QPSDM_SPAN(8)
+#undef QPSDM_SPAN
+    default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span of %d intervals", S);
+  }
+}
+
+// The bands of one shot: tile groups in the 3, 2 slot pattern where the
+// range has it (the last group of the range without it when its last
+// interval holds one point more).
+template <int NI, bool SRC, typename BankT, typename Tab>
 int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   if (go.nwg_out) *go.nwg_out = -1;
   if (!go.run) return QPSDM_OK;
@@ -47,38 +72,33 @@ int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
       qk::ImgDev B = P;
       B.sp_b = P.sp_w[k];
       B.sp_e = P.sp_w[k + 1];
-      const dim3 grid(ncol, (B.sp_e - B.sp_b + NI - 1) / NI);
-      switch (P.sp_s[k]) {
-#define QPSDM_SPAN(s_)                                                                                          \
-  case s_:                                                                                                      \
-    hipLaunchKernelGGL((image_batch_spans_kernel<NI, s_, PAT, BankT, Tab, SRC>), grid, dim3(64), 0, go.s, B, tab, \
-                       go.img, go.geom, bank, go.runs, r);                                                      \
-    break;
-        QPSDM_SPAN(1)
-        QPSDM_SPAN(2)
-        QPSDM_SPAN(4)
-        QPSDM_SPAN(8)
-#undef QPSDM_SPAN
-        default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span of %d intervals", P.sp_s[k]);
+      const int S = P.sp_s[k], ng = (B.sp_e - B.sp_b + NI - 1) / NI;
+      const bool last = k == P.sp_n - 1;
+      int rc;
+      if (!P.sp_pat) {
+        rc = launch_band<NI, 0, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
+      } else if (!last || P.sp_pat_last) {
+        rc = launch_band<NI, 1, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
+      } else {
+        rc = launch_band<NI, 1, SRC>(B, S, 0, ng - 1, ncol, tab, bank, go, r);
+        if (!rc) rc = launch_band<NI, 0, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r);
       }
+      if (rc) return rc;
     }
     QPSDM_HIP(hipGetLastError());
   }
   return QPSDM_OK;
 }
 
-// Tiles of 4 or 8 intervals; the 3, 2 slot pattern where the node range has it.
+// Tiles of 4 or 8 intervals.
 template <typename BankT, typename Tab>
 int launch_spans(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   const bool src = go.runs.src_hdr != nullptr;
-  switch (P.tile_z * 2 + (P.sp_pat ? 1 : 0)) {
-#define QPSDM_NI(ni, pat)                                                                      \
-  case ni * 2 + pat:                                                                           \
-    return src ? launch_spans_t<ni, pat, true>(P, tab, bank, go) : launch_spans_t<ni, pat, false>(P, tab, bank, go);
-    QPSDM_NI(4, 0)
-    QPSDM_NI(4, 1)
-    QPSDM_NI(8, 0)
-    QPSDM_NI(8, 1)
+  switch (P.tile_z) {
+#define QPSDM_NI(ni) \
+  case ni: return src ? launch_spans_t<ni, true>(P, tab, bank, go) : launch_spans_t<ni, false>(P, tab, bank, go);
+    QPSDM_NI(4)
+    QPSDM_NI(8)
 #undef QPSDM_NI
     default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span form with %d intervals per tile", P.tile_z);
   }

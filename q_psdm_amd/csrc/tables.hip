@@ -144,11 +144,16 @@ __global__ __launch_bounds__(256) void quant_x_kernel(qpsdm_tgeom g, int nbv, in
     // weights (a reader decodes a weighted code sum with one fma)
     unsigned char *o = out + b * qpsdm_dev::kQBrickBytes;
     if (threadIdx.x == 0) *(float4 *)o = make_float4(t0, tst * (1.0f / 65535.0f), s0, sst * (1.0f / 65535.0f));
-    // cell (lv, lu) at depth lw: corners k = ((lv [+1]) * 9 + lu [+1]) * 8 + lw of the 9 x 9 x 8 nodes
+    // cell (lv, lu) at depth lw: corners k = ((lv [+1]) * 9 + lu [+1]) * 8 + lw of the 9 x 9 x 8 nodes;
+    // the word pairs each row's two t codes and two t* codes:
+    // {t00 | t01 << 16, t10 | t11 << 16, s00 | s01 << 16, s10 | s11 << 16}
     for (int c = threadIdx.x; c < 8 * 8 * 8; c += blockDim.x) {
       const int lw = c & 7, lu = (c >> 3) & 7, lv = c >> 6;
       const int k00 = (lv * 9 + lu) * 8 + lw;
-      *(uint4 *)(o + 16 + c * 16) = make_uint4(code(k00), code(k00 + 8), code(k00 + 72), code(k00 + 80));
+      const unsigned c00 = code(k00), c01 = code(k00 + 8), c10 = code(k00 + 72), c11 = code(k00 + 80);
+      *(uint4 *)(o + 16 + c * 16) =
+          make_uint4((c00 & 0xffffu) | (c01 << 16), (c10 & 0xffffu) | (c11 << 16), (c00 >> 16) | (c01 & 0xffff0000u),
+                     (c10 >> 16) | (c11 & 0xffff0000u));
     }
   }
 }

@@ -121,6 +121,11 @@ class Survey:
     def form(self):
         return ("exact", "nodes", "spans")[self.ip.weight_nodes]
 
+    def pair_bank(self):
+        """The form's bank layout: pair banks (sample and difference) for the
+        span form, plain banks otherwise."""
+        return self.form == "spans"
+
     def spans(self):
         """The span schedule [(z, S), ...] of the span form."""
         return [(float(self.ip.span_z[k]), int(self.ip.span_s[k])) for k in range(self.ip.span_n)]

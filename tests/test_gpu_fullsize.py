@@ -170,19 +170,22 @@ def test_fullsize_fast_form_against_fp64_and_surrogate_gain(full):
     nus, nvs, nw = sur.src_grid(sv)
     hdr = src[:16].view(torch.float32).cpu().numpy()
     body = src[16:].view(torch.float32).view(nvs, nus, nw, 2)
-    # fast form vs the FP64 oracle fed the same source table
+    # fast form (in the span form, the bench's, and in the node form) vs the
+    # FP64 oracle fed the same source table
     m = 32
     sub = np.ascontiguousarray(g1[:m])
-    svm = engine.Survey(cfg, sv.tstar_max).set_form("nodes", 400.0)
-    q = svm.quantize_tables(tab, code="u16q")
-    img = svm.new_image()
-    svm.image_batch(img, torch.from_numpy(sub).cuda(), svm.qbank(x[:m].contiguous(), dtype=torch.float16), q,
-                    _shot_list([(0, m)], tans), launch=(8, 8, 4), nwg=-1, src=src)
     ref, _ = O.image(O.qbank(x[:m].cpu().numpy(), o_bparams(sv)), sub, tab.cpu().numpy(), o_iparams(sv),
                      src=(hdr, body.cpu().numpy()))
-    e = rel_l2(img.cpu().numpy(), ref)
-    print(f"fast form vs FP64 oracle with its own source table: {e:.2e}")
-    assert e <= 1e-3
+    for form, launch in (("spans", (8, 8, 4)), ("nodes", (8, 8, 4))):
+        svm = engine.Survey(cfg, sv.tstar_max).set_form(form, 400.0, ((0.0, 1), (800.0, 2), (1600.0, 4)))
+        q = svm.quantize_tables(tab, code="u16q")
+        img = svm.new_image()
+        bank = svm.qbank(x[:m].contiguous(), dtype=torch.float16, pair=svm.pair_bank())
+        svm.image_batch(img, torch.from_numpy(sub).cuda(), bank, q, _shot_list([(0, m)], tans), launch=launch,
+                        nwg=-1, src=src)
+        e = rel_l2(img.cpu().numpy(), ref)
+        print(f"fast form ({form}) vs FP64 oracle with its own source table: {e:.2e}")
+        assert e <= 1e-3
     # gain over the table methods
     exact = src.clone()
     engine.build_tables_into(cfg.model, dict(cfg.tgeom, nnx=1, nny=1, n0x=float(hdr[0]), n0y=float(hdr[1]), nu=nus,
@@ -213,15 +216,15 @@ def test_fullsize_fast_form_against_fp64_and_surrogate_gain(full):
     assert err["surrogate"] < 0.75 * min(err["blend"], err["nearest"])
 
 
-@pytest.mark.parametrize("form", ["exact", "nodes"])
+@pytest.mark.parametrize("form", ["exact", "nodes", "spans"])
 def test_fullsize_config4_graph_replays_equal_eager(full, form):
     """Four shot gathers of the batching variant (uneven trace counts,
     per-shot apertures) over the full image through a captured graph of
     [Q bank -> batched imaging]: each of three replays, with eager work and
     synchronisations between them, equals the eager batched call bit for
-    bit; in the node form too."""
+    bit; in the node and span forms too."""
     cfg, sv0, tab, _ = full
-    sv = engine.Survey(cfg, sv0.tstar_max).set_form(form, 400.0)
+    sv = engine.Survey(cfg, sv0.tstar_max).set_form(form, 400.0, ((0.0, 1), (800.0, 2), (1600.0, 4)))
     shots = synth.shots_config4(nshots=4, seed=0x5EED0044)
     geom, ranges = synth.geometry(shots)
     desc = [(tr0, n) + synth.dip_tans(a) for (tr0, n), (*_, a) in zip(ranges, shots)]
@@ -229,11 +232,12 @@ def test_fullsize_config4_graph_replays_equal_eager(full, form):
     g_d = torch.from_numpy(geom).cuda()
     q = sv.quantize_tables(tab, code="u16q")
     sl = engine.ShotList(desc)
-    launch = (8, 8, 4) if form == "nodes" else (8, 8, 8)
+    launch = (8, 8, 4) if form != "exact" else (8, 8, 8)
     eager = sv.new_image()
-    sv.image_batch(eager, g_d, sv.qbank(x, dtype=torch.float16), q, sl, launch=launch, nwg=-1)
+    sv.image_batch(eager, g_d, sv.qbank(x, dtype=torch.float16, pair=sv.pair_bank()), q, sl, launch=launch, nwg=-1)
     img = sv.new_image()
-    bank = torch.empty(sv.bank_shape(x.shape[0], torch.float16), dtype=torch.float16, device="cuda")
+    bank = torch.empty(sv.bank_shape(x.shape[0], torch.float16, pair=sv.pair_bank()), dtype=torch.float16,
+                       device="cuda")
     gr = engine.BatchGraph(sv, x, bank, img, g_d, q, sl, launch=launch, nwg=-1, stages=5)
     try:
         for _ in range(3):

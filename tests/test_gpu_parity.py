@@ -280,8 +280,8 @@ def test_u16x_image_matches_f32_tables(gpu, mk):  # tables' reads are pinned by 
 def test_u16q_holds_the_u16x_codes(gpu, mk):
     """The corner-quad layout stores each U16X brick's header (its steps
     divided by 65535) and codes: every cell's word at every depth node holds
-    the codes of its four corners, read back from the U16X brick that owns the
-    cell."""
+    the t codes of its four corners, then their t* codes, read back from the
+    U16X brick that owns the cell."""
     cfg = mk()
     sv, tab = _survey(cfg)
     g = cfg.tgeom
@@ -295,9 +295,12 @@ def test_u16q_holds_the_u16x_codes(gpu, mk):
     assert np.array_equal(hq[..., 0::2], hx[..., 0::2])  # offsets
     assert np.array_equal(hq[..., 1::2], hx[..., 1::2] * np.float32(1.0 / 65535.0))  # steps / 65535
     xc = x[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 9, 9, 8)
-    qc = q[..., 16:].copy().view(np.uint32).reshape(nn, nbv, nbu, nbw, 8, 8, 8, 4)
+    qc = q[..., 16:].copy().view(np.uint16).reshape(nn, nbv, nbu, nbw, 8, 8, 8, 8)
+    # word halves: t00 t01 t10 t11 t*00 t*01 t*10 t*11
     for k, (dv, du) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
-        assert np.array_equal(qc[..., k], xc[..., dv:dv + 8, du:du + 8, :]), k
+        corner = xc[..., dv:dv + 8, du:du + 8, :]
+        assert np.array_equal(qc[..., k], (corner & 0xffff).astype(np.uint16)), k
+        assert np.array_equal(qc[..., 4 + k], (corner >> 16).astype(np.uint16)), k
 
 
 @pytest.mark.parametrize("mk", [synth.tiny, synth.tiny_layers])

@@ -15,12 +15,14 @@ from tests.helpers import o_bparams, o_iparams, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _deep_survey(nw=52, dw=20.0):
+def _deep_survey(nw=52, dw=20.0, nz=100):
     """A 1 km deep v(z) survey (8 x 6 columns x 100 depths at 10 m, 20 m
     tables): depth enough for every band and span length to matter.  With
-    30 m tables every interval holds 3 points (not the 3, 2 slot pattern)."""
+    30 m tables every interval holds 3 points (not the 3, 2 slot pattern);
+    with 25 m tables and 101 depths the pattern holds but for the last
+    interval, which ends at the tables' last node and holds 3."""
     cfg = synth.tiny()
-    cfg.nx, cfg.ny, cfg.nz = 8, 6, 100
+    cfg.nx, cfg.ny, cfg.nz = 8, 6, nz
     cfg.nt = 1201
     cfg.tgeom = dict(cfg.tgeom, nw=nw, dw=dw)
     probe = engine.Survey(cfg, 0.01)
@@ -78,9 +80,10 @@ def _span_image(sv, cfg, geom, ranges, bank, tab, ni, src=None):
 @pytest.mark.parametrize("ni", [4, 8])
 @pytest.mark.parametrize("sched", [((0.0, 1),), ((0.0, 2), (500.0, 4), (700.0, 8)), ((0.0, 8),),
                                    ((450.0, 4),)])
-@pytest.mark.parametrize("dw", [20.0, 30.0])
-def test_span_form_matches_oracle(gpu, ni, sched, dw):
-    cfg, sv, tab = _deep_survey(nw=52 if dw == 20.0 else 36, dw=dw)
+@pytest.mark.parametrize("geo", [(20.0, 100, 52), (30.0, 100, 36), (25.0, 101, 41)])
+def test_span_form_matches_oracle(gpu, ni, sched, geo):
+    dw, nz, nw = geo
+    cfg, sv, tab = _deep_survey(nw=nw, dw=dw, nz=nz)
     geom, ranges, tr = _data(cfg)
     x = torch.from_numpy(tr).cuda()
     b32 = sv.qbank(x)
