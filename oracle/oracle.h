@@ -121,12 +121,15 @@ int64_t or_image_src_f32(const float *bank, const float *geom, int ntr, const fl
 /* Node form (the library's weight_nodes = 1) of the FP32 path; the points
  * [ps, pe) it interpolates from depth nodes come from or_node_range. */
 void or_node_range(const or_iparams *p, float z_exact, int *ps, int *pe);
+/* The same with the first interval rounded up to a multiple of align (the
+ * span form's range: align 8). */
+void or_node_range_aligned(const or_iparams *p, float z_exact, int align, int *ps, int *pe);
 int64_t or_image_nodes_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
                            const or_iparams *p, float z_exact, float *img);
-/* Span form (the library's weight_nodes = 2): the node form with the level
- * clamped at the nodes and the ramps and amplitude interpolated, linearly
- * in 1/z, between the weight nodes spans[2w] <= w < spans[2w + 1] of each
- * interval w. */
+/* Span form (the library's weight_nodes = 2): the node form over the node
+ * range aligned to 8 intervals, with the level clamped at the nodes and the
+ * ramps and amplitude interpolated, linearly in 1/z, between the weight
+ * nodes spans[2w] <= w < spans[2w + 1] of each interval w. */
 int64_t or_image_spans_f32(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
                            const or_iparams *p, float z_exact, const int *spans, float *img);
 /* Test-only: the FP64 sum's level rule (0 round, the product's; 1 floor;

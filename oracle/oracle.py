@@ -96,6 +96,7 @@ def lib():
         L.or_image_nodes_f32.restype = C.c_int64
         L.or_image_nodes_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, fp]
         L.or_node_range.argtypes = [C.POINTER(IParams), C.c_float, ip, ip]
+        L.or_node_range_aligned.argtypes = [C.POINTER(IParams), C.c_float, C.c_int, ip, ip]
         L.or_image_spans_f32.restype = C.c_int64
         L.or_image_spans_f32.argtypes = [fp, fp, C.c_int, fp, C.POINTER(Src), C.POINTER(IParams), C.c_float, ip,
                                          fp]
@@ -291,10 +292,11 @@ def image(bank, geom, tab, ip, fp32=False, src=None):
     return img, n
 
 
-def node_range(ip, z_exact):
-    """Points [ps, pe) the node form interpolates from depth nodes."""
+def node_range(ip, z_exact, align=1):
+    """Points [ps, pe) the node form (align 1) or the span form (align 8)
+    interpolates from depth nodes."""
     a, b = C.c_int(), C.c_int()
-    lib().or_node_range(C.byref(ip), z_exact, C.byref(a), C.byref(b))
+    lib().or_node_range_aligned(C.byref(ip), z_exact, align, C.byref(a), C.byref(b))
     return a.value, b.value
 
 
@@ -317,7 +319,7 @@ def image_nodes(bank, geom, tab, ip, z_exact, src=None):
 def span_table(ip, z_exact, schedule):
     """Weight nodes (start, end) of every table interval, int [nw - 1][2],
     under the span form's schedule [(z, S), ...]: the node range's intervals
-    [ws, we) split into bands, band k from the first interval at or below z_k
+    [ws, we) (ws rounded up to a multiple of 8) split into bands, band k from the first interval at or below z_k
     rounded up to a multiple of 8 intervals past ws (intervals above the
     first band: one-interval spans; a band that a later one starts at the
     same interval is dropped), spans of S_k intervals from the band's start,
@@ -325,10 +327,11 @@ def span_table(ip, z_exact, schedule):
     (w, w + 1) (the oracle does not read them)."""
     nw, dw = ip.tg.nw, ip.tg.dw
     sp = np.array([[w, w + 1] for w in range(nw - 1)], np.int32)
-    ps, pe = node_range(ip, z_exact)
+    ps, pe = node_range(ip, z_exact, align=8)
     if ps >= pe:
         return sp
     ws = max(1, int(math.ceil(float(np.float32(z_exact)) / float(np.float32(dw)))))
+    ws = (ws + 7) // 8 * 8
     inv_dw = np.float32(1.0) / np.float32(dw)
     fw = np.float32(pe - 1) * np.float32(ip.dz) * inv_dw
     fw = min(fw, np.float32(nw - 1))

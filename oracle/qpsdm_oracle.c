@@ -760,10 +760,17 @@ static int nf_first_point(int W, int nz, float dz, float inv_dw, int nw) {
 }
 
 void or_node_range(const or_iparams *p, float z_exact, int *ps_out, int *pe_out) {
+  or_node_range_aligned(p, z_exact, 1, ps_out, pe_out);
+}
+
+/* The node range with its first interval rounded up to a multiple of align
+ * (the span form's: 8). */
+void or_node_range_aligned(const or_iparams *p, float z_exact, int align, int *ps_out, int *pe_out) {
   const int nz = p->nz, nw = p->tg.nw;
   const float dz = p->dz, inv_dw = 1.0f / p->tg.dw;
   int ws = (int)ceil((double)z_exact / (double)p->tg.dw);
   if (ws < 1) ws = 1;
+  ws = (ws + align - 1) / align * align;
   int pe = nz;
   for (int iz = 0; iz < nz; ++iz) {
     const float z = (float)iz * dz;
@@ -868,7 +875,7 @@ static nf_node nf_value(const nf_col *C, int wn) {
 static int64_t nodes_impl(const float *bank, const float *geom, int ntr, const float *tab, const or_src *src,
                           const or_iparams *p, float z_exact, const int *spans, float *img) {
   int ps, pe;
-  or_node_range(p, z_exact, &ps, &pe);
+  or_node_range_aligned(p, z_exact, spans ? 8 : 1, &ps, &pe);
   int64_t count = image_f32_impl(bank, geom, ntr, tab, src, p, ps, pe, img);
   if (ps >= pe) return count;
   const or_tgeom *g = &p->tg;

@@ -17,7 +17,7 @@ namespace qk = qpsdm_dev;
 
 namespace qpsdm_dev {
 
-int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o);
+int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o, int align);
 
 // Node range of the node form (ImgDev nf_*), with the device's float rule
 // for a point's interval: intervals from the first node at or below z_exact
@@ -28,10 +28,11 @@ int node_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   const float dz = p->dz, inv_dw = o->inv_dw;
   QPSDM_REQUIRE(p->z_exact >= 0.0f, "image: z_exact must be >= 0");
   QPSDM_REQUIRE(ni >= 2 && ni <= 4, "image: the node form takes tile_z = 2, 3 or 4 depth intervals per tile");
-  return node_range_ni(p, ni, o);
+  return node_range_ni(p, ni, o, 1);
 }
 
-// The span form's node range (the node form's) and its bands: band k starts
+// The span form's node range (the node form's, its first interval rounded
+// up to a multiple of 8) and its bands: band k starts
 // at the first interval at or below span_z[k], rounded up to a multiple of 8
 // intervals past the range's first interval ws (so that tiles of 2, 4 or 8
 // intervals from a band's start never straddle two bands and every span
@@ -48,7 +49,7 @@ int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
     QPSDM_REQUIRE(p->span_z[k] >= 0.0f && (k == 0 || p->span_z[k] >= p->span_z[k - 1]),
                   "image: span_z must be >= 0 and ascending");
   }
-  int rc = node_range_ni(p, ni, o);
+  int rc = node_range_ni(p, ni, o, 8);
   if (rc) return rc;
   o->nf_ngroups = 0;
   if (o->nf_ps >= o->nf_pe) return QPSDM_OK;  // no node range: the exact body images every point
@@ -88,10 +89,14 @@ int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   return QPSDM_OK;
 }
 
-int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o) {
+// (align: the first interval rounded up to a multiple of it; the span
+// form's is 8, so that a tile of 4 or 8 intervals from a band's start keeps
+// all but its last table node in one 8-node depth brick)
+int node_range_ni(const qpsdm_image_params *p, int ni, ImgDev *o, int align) {
   const int nz = p->nz, nw = p->tg.nw;
   const float dz = p->dz, inv_dw = o->inv_dw;
-  const int ws = std::max(1, (int)std::ceil((double)p->z_exact / (double)p->tg.dw));
+  int ws = std::max(1, (int)std::ceil((double)p->z_exact / (double)p->tg.dw));
+  ws = (ws + align - 1) / align * align;
   int pe = nz;
   for (int iz = 0; iz < nz; ++iz) {
     const float z = (float)iz * dz;

@@ -453,6 +453,13 @@ struct TabU16Q {
     return make_float2(fmaf(q.h.y, t, q.h.x), fmaf(q.h.w, s, q.h.z));
   }
   __device__ inline float2 at(Col &c, int w) const { return reduce(c, fetch(c, w)); }
+  // The corner word of depth node w with a given header (a caller that knows
+  // which depth brick the node lies in loads the header once).
+  __device__ inline Raw word(const Col &c, int w, const float4 &h) const {
+    const uint4 q = __builtin_bit_cast(
+        uint4, __builtin_amdgcn_raw_buffer_load_b128(c.r, c.o, (w >> 3) * kQBrickBytes + (w & 7) * 16, 0));
+    return Raw{q.x, q.y, q.z, q.w, h};
+  }
 };
 
 inline void u16x_bricks(const qpsdm_tgeom &g, int *nbv, int *nbu, int *nbw) {

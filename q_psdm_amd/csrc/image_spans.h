@@ -106,6 +106,28 @@ __device__ inline void span_source(const ImgDev &P, const Tab &tab, typename Tab
   }
 }
 
+// The table words of a tile's nodes wn[m0..m1).  U16Q: nodes 0..NI-1 lie in
+// the depth brick of wn[0] (the span form's tiles start at multiples of 4 or
+// 8 intervals from a multiple of 8), node NI in its own; their two headers
+// are loaded once per trace instead of checked per node.  Nodes past the
+// tables' last node are clamped to it and own no points.
+template <int NI>
+__device__ inline void tile_words(const TabF32 &tab, TabF32::Col &c, const int (&wn)[NI + 1], int m0, int m1,
+                                  TabF32::Raw *raw, const float4 &, const float4 &) {
+#pragma unroll
+  for (int m = m0; m < m1; ++m) raw[m - m0] = tab.fetch(c, wn[m]);
+}
+template <int NI>
+__device__ inline void tile_words(const TabU16Q &tab, TabU16Q::Col &c, const int (&wn)[NI + 1], int m0, int m1,
+                                  TabU16Q::Raw *raw, const float4 &h0, const float4 &h1) {
+#pragma unroll
+  for (int m = m0; m < m1; ++m) raw[m - m0] = tab.word(c, wn[m], m == NI ? h1 : h0);
+}
+__device__ inline float4 tile_header(const TabF32 &, const TabF32::Col &, int) { return float4{}; }
+__device__ inline float4 tile_header(const TabU16Q &, const TabU16Q::Col &c, int w) {
+  return TabU16Q::header(c, (w >> 3) * kQBrickBytes);
+}
+
 // The span tile body: tile_col as in image_tile; group g owns intervals
 // [sp_b + (sp_g0 + g) NI, sp_b + (sp_g0 + g + 1) NI) of the band
 // [sp_b, sp_e) (ImgDev), cut at the node range's points [nf_ps, nf_pe).
@@ -245,12 +267,12 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
       // (in two halves of the nodes: the raw words of all nine nodes of an
       // eight-interval tile would hold 36 registers at once)
       constexpr int NH = NI / 2 + 1;
+      const float4 h0 = tile_header(tab, c, wn[0]), h1 = tile_header(tab, c, wn[NI]);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int m0 = h ? NH : 0, m1 = h ? NI + 1 : NH;
         typename Tab::Raw raw[NH];
-#pragma unroll
-        for (int m = m0; m < m1; ++m) raw[m - m0] = tab.fetch(c, wn[m]);
+        tile_words<NI>(tab, c, wn, m0, m1, raw, h0, h1);
 #pragma unroll
         for (int m = m0; m < m1; ++m) {
           const float2 v = tab.reduce(c, raw[m - m0]);

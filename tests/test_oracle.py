@@ -451,26 +451,28 @@ def test_node_form_is_exact_at_the_nodes_and_converges_with_depth():
 
 # ------------------------------------------------------------------ span form
 def test_span_table_follows_the_band_rule():
-    """Bands start at the first interval at or below each schedule depth,
-    rounded up to a multiple of 8 intervals past the node range's first
-    interval; spans tile each band from its start, the last cut at the band's
-    end (worked by hand for 20 m tables, z_exact 300 m, points to 990 m)."""
+    """The span form's node range starts at a multiple of 8 intervals; bands
+    start at the first interval at or below each schedule depth, rounded up
+    to a multiple of 8 intervals past the range's start; spans tile each band
+    from its start, the last cut at the band's end (worked by hand for 20 m
+    tables, z_exact 300 m, points to 990 m)."""
     cfg, tab, ip = _deep_case()
+    assert O.node_range(ip, 300.0, align=8)[0] == 32 and O.node_range(ip, 300.0)[0] == 30
     sp = O.span_table(ip, 300.0, ((0.0, 2), (500.0, 4), (700.0, 8)))
-    # ws = 15, we = 50: band 2 at [15, 31), band 4 at [31, 39), band 8 at [39, 50)
-    assert tuple(sp[15]) == (15, 17) and tuple(sp[16]) == (15, 17) and tuple(sp[30]) == (29, 31)
-    assert tuple(sp[31]) == (31, 35) and tuple(sp[38]) == (35, 39)
-    assert tuple(sp[39]) == (39, 47) and tuple(sp[46]) == (39, 47) and tuple(sp[47]) == (47, 50)
-    assert tuple(sp[49]) == (47, 50) and tuple(sp[14]) == (14, 15)
+    # ws = 16, we = 50: band 2 at [16, 32), band 4 at [32, 40), band 8 at [40, 50)
+    assert tuple(sp[16]) == (16, 18) and tuple(sp[17]) == (16, 18) and tuple(sp[31]) == (30, 32)
+    assert tuple(sp[32]) == (32, 36) and tuple(sp[39]) == (36, 40)
+    assert tuple(sp[40]) == (40, 48) and tuple(sp[47]) == (40, 48) and tuple(sp[48]) == (48, 50)
+    assert tuple(sp[49]) == (48, 50) and tuple(sp[15]) == (15, 16)
     # a schedule that starts below ws: one-interval spans above it
     sp = O.span_table(ip, 300.0, ((450.0, 4),))
-    assert tuple(sp[15]) == (15, 16) and tuple(sp[22]) == (22, 23) and tuple(sp[23]) == (23, 27)
+    assert tuple(sp[16]) == (16, 17) and tuple(sp[23]) == (23, 24) and tuple(sp[24]) == (24, 28)
     # bands that start at the same interval: the later one holds
-    sp = O.span_table(ip, 300.0, ((0.0, 2), (300.0, 8)))
-    assert tuple(sp[15]) == (15, 23)
+    sp = O.span_table(ip, 300.0, ((0.0, 2), (320.0, 8)))
+    assert tuple(sp[16]) == (16, 24)
     # half an interval below ws rounds up to the next multiple of 8
-    sp = O.span_table(ip, 300.0, ((0.0, 2), (310.0, 8)))
-    assert tuple(sp[15]) == (15, 17) and tuple(sp[23]) == (23, 31)
+    sp = O.span_table(ip, 300.0, ((0.0, 2), (330.0, 8)))
+    assert tuple(sp[16]) == (16, 18) and tuple(sp[24]) == (24, 32)
 
 
 def test_span_form_is_exact_for_zero_offset_traces():
@@ -488,7 +490,7 @@ def test_span_form_is_exact_for_zero_offset_traces():
     e32, _ = O.image(bank, geom, tab, ip, fp32=True)
     e64, _ = O.image(bank.astype(np.float64), geom, tab, ip)
     ze = 60.0
-    ps, pe = O.node_range(ip, ze)
+    ps, pe = O.node_range(ip, ze)  # the node form's range (the span form's starts at 160 m: exact above)
     floor = rel_l2(e32[..., ps:pe], e64[..., ps:pe])  # the FP32 path's own rounding, ~3e-5
     for sched in (((0.0, 8),), ((0.0, 1), (300.0, 4), (600.0, 8))):
         img, _ = O.image_spans(bank, geom, tab, ip, ze, O.span_table(ip, ze, sched))
@@ -509,7 +511,7 @@ def test_span_form_converges_as_spans_shrink():
     e32, _ = O.image(bank, geom, tab, ip, fp32=True)
     e64, _ = O.image(bank.astype(np.float64), geom, tab, ip)
     ze = 200.0
-    ps, pe = O.node_range(ip, ze)
+    ps, pe = O.node_range(ip, ze, align=8)
     errs = []
     for S in (8, 4, 2, 1):
         img, _ = O.image_spans(bank, geom, tab, ip, ze, O.span_table(ip, ze, ((0.0, S),)))
