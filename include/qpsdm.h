@@ -315,8 +315,9 @@ typedef struct {
 
 /* Launch shape picked by the autotuner; a NULL shape or 0 fields take the
  * default 8 x 8 columns x 6 depth points.  In the node form tile_z counts
- * table depth intervals per tile (2, 3 or 4), in the span form too (4 or 8);
- * the exact body's share of those forms runs 8 depth points per tile. */
+ * table depth intervals per tile (2, 3 or 4), in the span form too (4 or 8;
+ * a band of spans of 8 intervals runs tiles of 8); the exact body's share of
+ * those forms runs 8 depth points per tile. */
 typedef struct {
   int tile_x, tile_y, tile_z; /* image tile per work-group (columns x, y; depth points) */
 } qpsdm_launch;

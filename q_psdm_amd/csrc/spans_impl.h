@@ -5,6 +5,7 @@
 #define QPSDM_SPANS_IMPL_H_
 
 #include <hip/hip_fp16.h>
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include "batch_runs.h"
@@ -44,10 +45,23 @@ int launch_band(qk::ImgDev B, int S, int g0, int g1, int ncol, const Tab &tab, c
   }
 }
 
-// The bands of one shot: tile groups in the 3, 2 slot pattern where the
-// range has it (the last group of the range without it when its last
-// interval holds one point more).
+// One band in tiles of NI intervals: tile groups in the 3, 2 slot pattern
+// where the range has it (the last group of the range without it when its
+// last interval holds one point more).
 template <int NI, bool SRC, typename BankT, typename Tab>
+int launch_band_ni(const qk::ImgDev &P, const qk::ImgDev &B, int S, bool last, int ncol, const Tab &tab,
+                   const BankT *bank, const Go &go, int r) {
+  const int ng = (B.sp_e - B.sp_b + NI - 1) / NI;
+  if (!P.sp_pat) return launch_band<NI, 0, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
+  if (!last || P.sp_pat_last) return launch_band<NI, 1, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
+  int rc = launch_band<NI, 1, SRC>(B, S, 0, ng - 1, ncol, tab, bank, go, r);
+  if (!rc) rc = launch_band<NI, 0, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r);
+  return rc;
+}
+
+// The bands of one shot.  A band's tiles hold tile_z intervals, or one span
+// where its spans are longer (8 intervals: two weight nodes per tile).
+template <bool SRC, typename BankT, typename Tab>
 int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   if (go.nwg_out) *go.nwg_out = -1;
   if (!go.run) return QPSDM_OK;
@@ -72,17 +86,10 @@ int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
       qk::ImgDev B = P;
       B.sp_b = P.sp_w[k];
       B.sp_e = P.sp_w[k + 1];
-      const int S = P.sp_s[k], ng = (B.sp_e - B.sp_b + NI - 1) / NI;
+      const int S = P.sp_s[k];
       const bool last = k == P.sp_n - 1;
-      int rc;
-      if (!P.sp_pat) {
-        rc = launch_band<NI, 0, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
-      } else if (!last || P.sp_pat_last) {
-        rc = launch_band<NI, 1, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
-      } else {
-        rc = launch_band<NI, 1, SRC>(B, S, 0, ng - 1, ncol, tab, bank, go, r);
-        if (!rc) rc = launch_band<NI, 0, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r);
-      }
+      const int rc = std::max(P.tile_z, S) == 8 ? launch_band_ni<8, SRC>(P, B, S, last, ncol, tab, bank, go, r)
+                                                : launch_band_ni<4, SRC>(P, B, S, last, ncol, tab, bank, go, r);
       if (rc) return rc;
     }
     QPSDM_HIP(hipGetLastError());
@@ -93,15 +100,9 @@ int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const
 // Tiles of 4 or 8 intervals.
 template <typename BankT, typename Tab>
 int launch_spans(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
-  const bool src = go.runs.src_hdr != nullptr;
-  switch (P.tile_z) {
-#define QPSDM_NI(ni) \
-  case ni: return src ? launch_spans_t<ni, true>(P, tab, bank, go) : launch_spans_t<ni, false>(P, tab, bank, go);
-    QPSDM_NI(4)
-    QPSDM_NI(8)
-#undef QPSDM_NI
-    default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span form with %d intervals per tile", P.tile_z);
-  }
+  if (P.tile_z != 4 && P.tile_z != 8)
+    return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span form with %d intervals per tile", P.tile_z);
+  return go.runs.src_hdr ? launch_spans_t<true>(P, tab, bank, go) : launch_spans_t<false>(P, tab, bank, go);
 }
 
 }  // namespace spans_detail
