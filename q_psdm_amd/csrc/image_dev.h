@@ -516,7 +516,8 @@ inline TabB16<UNORM> make_tab_b16(const qpsdm_tgeom &g, const void *p) {
 }
 
 __device__ inline void frac_index(float q, int n, int *i0, float *a) {
-  q = fminf(fmaxf(q, 0.0f), (float)(n - 1));
+  // clamp to [0, n - 1] as one v_med3_f32 (q is finite)
+  q = __builtin_amdgcn_fmed3f(q, 0.0f, (float)(n - 1));
   int i = min((int)q, n - 2);
   *i0 = i;
   *a = q - (float)i;

@@ -40,16 +40,31 @@
 #ifndef QPSDM_IMAGE_SPANS_H_
 #define QPSDM_IMAGE_SPANS_H_
 
+#include <float.h>
+
 #include "image_nodes.h"
 
 namespace qpsdm_dev {
 
 typedef const __attribute__((address_space(3))) unsigned long long *lds_u64;
 
+// max(x, y) and min(x, y) of finite values as one v_med3_f32 against
+// +-FLT_MAX.  fmaxf / fminf first quiet each operand the compiler cannot
+// prove canonical (any value from another basic block: loop-carried ramps,
+// times after a conditional reload), one more instruction per operand.
+__device__ inline float fmax_fin(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, FLT_MAX); }
+__device__ inline float fmin_fin(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, -FLT_MAX); }
+
+// true when pred holds for no lane of the wave
+__device__ inline bool wave_none(bool pred) { return __builtin_amdgcn_ballot_w64(pred) == 0; }
+
 // The ramps and amplitude at a weight node.
 struct WVal {
   float u1, u2, u3, u4, a;
 };
+
+// the smallest of a node's four ramps
+__device__ inline float wval_lo(const WVal &v) { return fminf(fminf(v.u1, v.u2), fminf(v.u3, v.u4)); }
 
 __device__ inline WVal wval_sub(const WVal &b, const WVal &a) {
   return WVal{b.u1 - a.u1, b.u2 - a.u2, b.u3 - a.u3, b.u4 - a.u4, b.a - a.a};
@@ -65,7 +80,7 @@ __device__ inline WVal weight_value(const Dip &D, const Ramps &R, float ds, floa
   const float X = fmaf(dgx, q, dsx), Y = fmaf(dgy, q, dsy), Z = fmaf(zn, q, zn);
   const float rz = __builtin_amdgcn_rcpf(Z);
   const float tx = X * rz, ty = Y * rz;
-  const float t = fmaxf(tg, tmin);
+  const float t = fmax_fin(tg, tmin);
   const float tr = (q * q) * q * (t * its);
   WVal v;
   v.u1 = fmaf(tx, D.rx1, R.cx1);
@@ -276,7 +291,7 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
 #pragma unroll
         for (int m = m0; m < m1; ++m) {
           const float2 v = tab.reduce(c, raw[m - m0]);
-          Fn[m] = fmaf(fmaxf(v.x, tmin), P.inv_dt, SC.tsn[m]);
+          Fn[m] = fmaf(fmax_fin(v.x, tmin), P.inv_dt, SC.tsn[m]);
           Ln[m] = __builtin_amdgcn_fmed3f(fmaf(v.y, P.inv_dtstar, SC.tsl[m]), 0.0f, jmax);
           if constexpr (INSIDE) {
             if (m % S == 0) tw[m / S] = v.x;
@@ -293,29 +308,41 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
         for (int k = 1; k < NW; ++k)
           if (wk[k] != wa + k * S) tw[k] = tab.at(c, wk[k]).x;
       }
+      // the ramps and amplitude at the weight nodes, and per span: bit k of
+      // out_m when the taper is 0 over span k for every valid lane, of in_m
+      // when it is 1
+      WVal W[NW];
+      float lo[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        W[k] = weight_value(D, R, SC.ds[k], SC.its[k], tw[k], (float)wk[k] * P.dw, SC.dsx, SC.dsy, dgx, dgy, hg2,
+                            unit, tmin);
+        lo[k] = wval_lo(W[k]);
+      }
+      unsigned out_m = 0, in_m = 0;
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) {
+        const WVal &A = W[k], &B = W[k + 1];
+        const float mlo = fmin_fin(lo[k], lo[k + 1]);
+        const float mout = fminf(fminf(fmax_fin(A.u1, B.u1), fmax_fin(A.u2, B.u2)),
+                                 fminf(fmax_fin(A.u3, B.u3), fmax_fin(A.u4, B.u4)));
+        if (wave_none(valid && !(mout <= 0.0f))) out_m |= 1u << k;
+        if (wave_none(valid && !(mlo >= 1.0f))) in_m |= 1u << k;
+      }
       // the per-slot constants are read from LDS in every trace (ds_read;
       // the asm keeps the compiler from hoisting them into registers)
       lds_u64 cs = (lds_u64)cst;
       asm volatile("" : "+v"(cs));
-      WVal W0 = weight_value(D, R, SC.ds[0], SC.its[0], tw[0], (float)wk[0] * P.dw, SC.dsx, SC.dsy, dgx, dgy, hg2,
-                             unit, tmin);
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
-        const WVal W1 = weight_value(D, R, SC.ds[k + 1], SC.its[k + 1], tw[k + 1], (float)wk[k + 1] * P.dw, SC.dsx,
-                                     SC.dsy, dgx, dgy, hg2, unit, tmin);
-        // the span's class, from the ramps at its two ends
-        const float mlo = fminf(fminf(fminf(W0.u1, W0.u2), fminf(W0.u3, W0.u4)),
-                                fminf(fminf(W1.u1, W1.u2), fminf(W1.u3, W1.u4)));
-        const float mout = fminf(fminf(fmaxf(W0.u1, W1.u1), fmaxf(W0.u2, W1.u2)),
-                                 fminf(fmaxf(W0.u3, W1.u3), fmaxf(W0.u4, W1.u4)));
-        const bool all_out = __all(!valid || mout <= 0.0f);
-        const bool all_in = __all(!valid || mlo >= 1.0f);
-        const WVal dW = wval_sub(W1, W0);
-        if (!all_out) {
+        if (!((out_m >> k) & 1u)) {
+          const WVal &W0 = W[k];
+          const WVal dW = wval_sub(W[k + 1], W0);
+          const bool all_in = (in_m >> k) & 1u;
           // every slot of the span's intervals, branch-free (a slot that
           // holds no point reads its interval's first node and accumulates
           // into a register that is never stored), in chunks of CI
-          // intervals: time, level and weight, the gathers, the sums
+          // intervals: time and level, the gathers, the weights, the sums
 #pragma unroll
           for (int c0 = k * SI; c0 < (k + 1) * SI; c0 += CI) {
             float cw[CI][PM], cf[CI][PM], sf[CI][PM];
@@ -337,14 +364,19 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
                 sf[i][p] = q.y;
               }
             }
-            // the weights: one branch per chunk on the span's class
-            if (all_in) {
 #pragma unroll
-              for (int i = 0; i < CI; ++i)
+            for (int i = 0; i < CI; ++i)
 #pragma unroll
-                for (int p = 0; p < PM; ++p)
-                  if (p < nslot(c0 + i)) cw[i][p] = fmaf(sf[i][p], dW.a, W0.a);
-            } else {
+              for (int p = 0; p < PM; ++p)
+                if (p < nslot(c0 + i)) cb[i][p] = bank_raw(br, ci[i][p], bank);
+            // the weights: the interpolated amplitude, times the clamped
+            // ramps unless the span's taper is 1 (one branch per chunk)
+#pragma unroll
+            for (int i = 0; i < CI; ++i)
+#pragma unroll
+              for (int p = 0; p < PM; ++p)
+                if (p < nslot(c0 + i)) cw[i][p] = fmaf(sf[i][p], dW.a, W0.a);
+            if (!all_in) {
 #pragma unroll
               for (int i = 0; i < CI; ++i)
 #pragma unroll
@@ -353,14 +385,9 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
                   const float s = sf[i][p];
                   const float wx = sat(fminf(fmaf(s, dW.u1, W0.u1), fmaf(s, dW.u2, W0.u2)));
                   const float wy = sat(fminf(fmaf(s, dW.u3, W0.u3), fmaf(s, dW.u4, W0.u4)));
-                  cw[i][p] = (wx * wy) * fmaf(s, dW.a, W0.a);
+                  cw[i][p] *= wx * wy;
                 }
             }
-#pragma unroll
-            for (int i = 0; i < CI; ++i)
-#pragma unroll
-              for (int p = 0; p < PM; ++p)
-                if (p < nslot(c0 + i)) cb[i][p] = bank_raw(br, ci[i][p], bank);
 #pragma unroll
             for (int i = 0; i < CI; ++i)
 #pragma unroll
@@ -369,7 +396,6 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
                   acc[c0 + i][p] = fmaf(cw[i][p], bank_interp(cb[i][p], cf[i][p], bank), acc[c0 + i][p]);
           }
         }
-        W0 = W1;
       }
     }
   }
