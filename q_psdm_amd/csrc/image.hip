@@ -84,8 +84,10 @@ int span_range(const qpsdm_image_params *p, int ni, ImgDev *o) {
   for (int w = ws; w < we - 1 && pat; ++w) pat = fits(w);
   o->sp_pat = pat;
   // (the last interval can hold the points down to the tables' last node:
-  // one more than the pattern)
-  o->sp_pat_last = pat && fits(we - 1);
+  // one more than the pattern, which a tile whose last interval it is takes
+  // in a third slot)
+  const int last_ni = n > 0 && std::max(ni, o->sp_s[n - 1]) == 8 ? 8 : 4;
+  o->sp_pat_last = !pat || n == 0 ? 0 : fits(we - 1) ? 1 : (we - o->sp_w[n - 1]) % last_ni == 0 ? 2 : 0;
   return QPSDM_OK;
 }
 

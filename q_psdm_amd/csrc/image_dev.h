@@ -100,8 +100,9 @@ struct ImgDev {
   // and the band [sp_b, sp_e) of one launch
   int sp_n, sp_w[6], sp_s[5];
   int sp_b, sp_e, sp_g0;  // a launch's band and its first tile group
-  // the range's intervals (sp_pat: all but the last; sp_pat_last: the last
-  // too) hold at most 3, 2, 3, 2, ... points from its start
+  // the range's intervals (sp_pat: all but the last; sp_pat_last = 1: the
+  // last too) hold at most 3, 2, 3, 2, ... points from its start;
+  // sp_pat_last = 2: the last holds 3 and is the last of its tile
   int sp_pat, sp_pat_last;
 };
 

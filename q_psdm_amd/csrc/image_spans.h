@@ -150,7 +150,9 @@ __device__ inline float4 tile_header(const TabU16Q &, const TabU16Q::Col &c, int
 // PAT: the node range alternates 3 and 2 points per interval from the
 // band's start (a 10 m image under 25 m tables, or any geometry with at
 // most that many), so the slots are 3, 2, 3, 2, ... instead of PM per
-// interval.
+// interval; PAT = 2 gives the tile's last interval 3 slots too (the
+// range's last interval, which holds the points down to the tables' last
+// node).
 template <int NI, int S, int PM, int PAT, bool SRC, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *__restrict__ img,
                                         const float4 *__restrict__ geom, const BankT *__restrict__ bank,
@@ -164,7 +166,7 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
   constexpr int CI = SI < 2 ? SI : 2;        // intervals per gather chunk
   static_assert(!PAT || (PM == 3 && NI % 2 == 0), "the 3, 2 slot pattern takes PM = 3 and an even NI");
   // point slots of interval m
-  auto nslot = [](int m) { return PAT ? ((m & 1) ? 2 : 3) : PM; };
+  auto nslot = [](int m) { return PAT ? (((m & 1) && !(PAT == 2 && m == NI - 1)) ? 2 : 3) : PM; };
   __shared__ float2 cst[NI * PM];            // per point slot: (fraction in its interval, in its span)
 
   const int lane = threadIdx.x;

@@ -30,7 +30,8 @@ int launch_band(qk::ImgDev B, int S, int g0, int g1, int ncol, const Tab &tab, c
   if (g1 <= g0) return QPSDM_OK;
   B.sp_g0 = g0;
   const dim3 grid(ncol, g1 - g0);
-  switch (S) {
+  // (spans of 8 intervals run in tiles of 8)
+  switch (NI == 8 ? S : S == 8 ? 0 : S) {
 #define QPSDM_SPAN(s_)                                                                                          \
   case s_:                                                                                                      \
     hipLaunchKernelGGL((image_batch_spans_kernel<NI, s_, PAT, BankT, Tab, SRC>), grid, dim3(64), 0, go.s, B, tab, \
@@ -39,24 +40,29 @@ int launch_band(qk::ImgDev B, int S, int g0, int g1, int ncol, const Tab &tab, c
     QPSDM_SPAN(1)
     QPSDM_SPAN(2)
     QPSDM_SPAN(4)
-    QPSDM_SPAN(8)
 #undef QPSDM_SPAN
-    default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span of %d intervals", S);
+    case 8:
+      hipLaunchKernelGGL((image_batch_spans_kernel<8, 8, PAT, BankT, Tab, SRC>), grid, dim3(64), 0, go.s, B, tab,
+                         go.img, go.geom, bank, go.runs, r);
+      return QPSDM_OK;
+    default: return qpsdm::set_error(QPSDM_EINVAL, "image_batch: span of %d intervals in tiles of %d", S, NI);
   }
 }
 
 // One band in tiles of NI intervals: tile groups in the 3, 2 slot pattern
-// where the range has it (the last group of the range without it when its
-// last interval holds one point more).
+// where the range has it (the range's last group with a third slot in its
+// last interval when that interval holds one point more, or without the
+// pattern when that interval is not the group's last).
 template <int NI, bool SRC, typename BankT, typename Tab>
 int launch_band_ni(const qk::ImgDev &P, const qk::ImgDev &B, int S, bool last, int ncol, const Tab &tab,
                    const BankT *bank, const Go &go, int r) {
   const int ng = (B.sp_e - B.sp_b + NI - 1) / NI;
   if (!P.sp_pat) return launch_band<NI, 0, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
-  if (!last || P.sp_pat_last) return launch_band<NI, 1, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
+  if (!last || P.sp_pat_last == 1) return launch_band<NI, 1, SRC>(B, S, 0, ng, ncol, tab, bank, go, r);
   int rc = launch_band<NI, 1, SRC>(B, S, 0, ng - 1, ncol, tab, bank, go, r);
-  if (!rc) rc = launch_band<NI, 0, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r);
-  return rc;
+  if (rc) return rc;
+  return P.sp_pat_last == 2 ? launch_band<NI, 2, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r)
+                            : launch_band<NI, 0, SRC>(B, S, ng - 1, ng, ncol, tab, bank, go, r);
 }
 
 // The bands of one shot.  A band's tiles hold tile_z intervals, or one span
