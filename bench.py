@@ -850,7 +850,7 @@ def main():
     ap.add_argument("--mixed-form", choices=["auto", "exact", "nodes", "spans"], default="spans",
                     help="form of the sum for the mixed and fast forms (auto: the autotuner's fastest)")
     ap.add_argument("--z-exact", type=float, default=400.0, help="node and span forms: depth imaged exactly (m)")
-    ap.add_argument("--spans", default="0:1,800:2,1600:4",
+    ap.add_argument("--spans", default="0:2,800:4,1600:8",
                     help="span form: z:S,... -- from depth z (m) down, weights at the ends of spans of S table "
                          "intervals")
     ap.add_argument("--ztile", type=int, default=0, help="pin depth points (node form: intervals) per tile")

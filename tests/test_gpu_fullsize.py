@@ -14,6 +14,10 @@ from tests.helpers import o_bparams, o_iparams, o_model, rel_l2
 
 pytestmark = pytest.mark.gpu
 
+# the bench's span schedule: spans of 2, 4 and 8 table intervals from 0, 800
+# and 1600 m down
+BENCH_SPANS = ((0.0, 2), (800.0, 4), (1600.0, 8))
+
 
 @pytest.fixture(scope="module")
 def full(gpu):
@@ -177,7 +181,7 @@ def test_fullsize_fast_form_against_fp64_and_surrogate_gain(full):
     ref, _ = O.image(O.qbank(x[:m].cpu().numpy(), o_bparams(sv)), sub, tab.cpu().numpy(), o_iparams(sv),
                      src=(hdr, body.cpu().numpy()))
     for form, launch in (("spans", (8, 8, 4)), ("nodes", (8, 8, 4))):
-        svm = engine.Survey(cfg, sv.tstar_max).set_form(form, 400.0, ((0.0, 1), (800.0, 2), (1600.0, 4)))
+        svm = engine.Survey(cfg, sv.tstar_max).set_form(form, 400.0, BENCH_SPANS)
         q = svm.quantize_tables(tab, code="u16q")
         img = svm.new_image()
         bank = svm.qbank(x[:m].contiguous(), dtype=torch.float16, pair=svm.pair_bank())
@@ -224,7 +228,7 @@ def test_fullsize_config4_graph_replays_equal_eager(full, form):
     synchronisations between them, equals the eager batched call bit for
     bit; in the node and span forms too."""
     cfg, sv0, tab, _ = full
-    sv = engine.Survey(cfg, sv0.tstar_max).set_form(form, 400.0, ((0.0, 1), (800.0, 2), (1600.0, 4)))
+    sv = engine.Survey(cfg, sv0.tstar_max).set_form(form, 400.0, BENCH_SPANS)
     shots = synth.shots_config4(nshots=4, seed=0x5EED0044)
     geom, ranges = synth.geometry(shots)
     desc = [(tr0, n) + synth.dip_tans(a) for (tr0, n), (*_, a) in zip(ranges, shots)]
