@@ -56,7 +56,7 @@ void image_batch_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img, const fl
     if (tile >= total) break;
     // depth group slowest: consecutive tiles are neighbouring columns of one
     // depth slab, which share table and bank lines in L2
-    qk::image_tile<ZP, SRC, false>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
+    qk::image_tile<ZP, SRC, false, true>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
   }
 }
 

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void image_batch_tiles_kernel(qk::ImgDev P, Tab
                                                                int shot, int ncol) {
   const int tile = (int)blockIdx.x;
   const OneShot one{runs(shot)};
-  qk::image_tile<ZP, SRC, RANGE>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
+  qk::image_tile<ZP, SRC, RANGE, true>(P, tab, img, geom, bank, tile % ncol, tile / ncol, one, 1);
 }
 
 // Launch arguments of one batch; with run == false only the resident grid

@@ -157,7 +157,7 @@ template <int ZP, typename BankT, typename Tab, bool RANGE>
 __global__ __launch_bounds__(64) void image_kernel(qk::ImgDev P, Tab tab, float *__restrict__ img,
                                                    const float4 *__restrict__ geom,
                                                    const BankT *__restrict__ bank, OneRun run) {
-  qk::image_tile<ZP, false, RANGE>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
+  qk::image_tile<ZP, false, RANGE, false>(P, tab, img, geom, bank, blockIdx.x, blockIdx.y, run, 1);
 }
 
 // Node form: blockIdx.x = column tile, blockIdx.y = node group.

@@ -617,6 +617,79 @@ __device__ inline int next_live(const float4 *__restrict__ geom, const TileBox &
   return t;
 }
 
+// The geometry of trace t as a scalar load (see next_live).
+__device__ inline float4 geom_at(const float4 *__restrict__ geom, int t) {
+  const const_float_ptr cg = (const_float_ptr)(geom + t);
+  return make_float4(cg[0], cg[1], cg[2], cg[3]);
+}
+
+// A box of image points: columns x in [xa, xb], y in [ya, yb], depths z in
+// [z0, z1].
+struct CullBox {
+  float xa, xb, ya, yb, z0, z1;
+};
+
+// Bounds over a box of the unit vector from a point toward the surface
+// position (px, py): its x and y components (lo, hi) and its z component
+// (hi).  u_x = d / sqrt(d^2 + r^2), d = px - x, r^2 = (py - y)^2 + z^2, is
+// increasing in d and, at fixed d, falls with r^2 for d >= 0 and rises for
+// d < 0, and d and r^2 vary independently over the box; u_z = z / sqrt(h^2
+// + z^2) is largest at the smallest horizontal distance h and the deepest z.
+// 0 / 0 (a point on the surface position) gives NaN, which no test below
+// passes.
+struct LegBounds {
+  float xlo, xhi, ylo, yhi, zhi;
+};
+__device__ inline float unit_part(float d, float r2) { return d * __builtin_amdgcn_rsqf(fmaf(d, d, r2)); }
+__device__ inline LegBounds leg_bounds(float px, float py, const CullBox &B) {
+  const float dxl = px - B.xb, dxh = px - B.xa, dyl = py - B.yb, dyh = py - B.ya;
+  const float hxmin = fmaxf(fmaxf(dxl, -dxh), 0.0f), hxmax = fmaxf(fabsf(dxl), fabsf(dxh));
+  const float hymin = fmaxf(fmaxf(dyl, -dyh), 0.0f), hymax = fmaxf(fabsf(dyl), fabsf(dyh));
+  const float z02 = B.z0 * B.z0, z12 = B.z1 * B.z1;
+  const float rxmin = fmaf(hymin, hymin, z02), rxmax = fmaf(hymax, hymax, z12);  // r^2 for u_x
+  const float rymin = fmaf(hxmin, hxmin, z02), rymax = fmaf(hxmax, hxmax, z12);  // r^2 for u_y
+  LegBounds b;
+  b.xlo = unit_part(dxl, dxl >= 0.0f ? rxmax : rxmin);
+  b.xhi = unit_part(dxh, dxh >= 0.0f ? rxmin : rxmax);
+  b.ylo = unit_part(dyl, dyl >= 0.0f ? rymax : rymin);
+  b.yhi = unit_part(dyh, dyh >= 0.0f ? rymin : rymax);
+  b.zhi = B.z1 * __builtin_amdgcn_rsqf(fmaf(hxmin, hxmin, fmaf(hymin, hymin, z12)));
+  return b;
+}
+
+// True when the trace (sx, sy, gx, gy) is out of the dip aperture at every
+// point of the box, by one side of one axis: the bisector's tangent
+// tan = X / Z = (u_x,s + u_x,g) / (u_z,s + u_z,g) (the unit vectors toward
+// the source and the receiver) stays beyond one cut-off tangent, with a
+// margin of 1e-4 (1 + |cut-off|) that covers the rounding of these bounds
+// and of the kernels' own per-point and per-node tests.  Then the exact and
+// node forms find every point outside the taper, and the span form finds
+// the same ramp <= 0 at both ends of every span in the box (the span is
+// out).  A numerator bound of the wrong sign, or NaN, culls nothing.
+__device__ inline bool aperture_out(const CullBox &B, const Dip &D, float sx, float sy, float gx, float gy) {
+  const LegBounds s = leg_bounds(sx, sy, B), g = leg_bounds(gx, gy, B);
+  const float den = s.zhi + g.zhi;
+  const float nxlo = s.xlo + g.xlo, nxhi = s.xhi + g.xhi, nylo = s.ylo + g.ylo, nyhi = s.yhi + g.yhi;
+  auto above = [&](float n, float c) { return n >= 0.0f && n >= (c + 1e-4f * (1.0f + fabsf(c))) * den; };
+  auto below = [&](float n, float c) { return n <= 0.0f && n <= (c - 1e-4f * (1.0f + fabsf(c))) * den; };
+  return above(nxlo, D.tx2c) || below(nxhi, D.tx1c) || above(nylo, D.ty2c) || below(nyhi, D.ty1c);
+}
+
+// The traces of [t0, t0 + 64) that may update the tile, one per lane (bit l
+// for trace t0 + l): not rejected, not culled by the tile box, and not out
+// of the aperture over the cull box.  Sixty-four traces cost one pass of
+// the bounds above.
+__device__ inline unsigned long long live_traces(const float4 *__restrict__ geom, int t0, int tend, const TileBox &box,
+                                                 const CullBox &cb, const Dip &D, int lane) {
+  const int t = t0 + lane;
+  bool live = false;
+  if (t < tend) {
+    const float4 g = geom[t];
+    live = !(rejected(g.x) | tile_culled(box, g.x, g.y, g.z, g.w)) && !aperture_out(cb, D, g.x, g.y, g.z, g.w);
+  }
+  return __builtin_amdgcn_ballot_w64(live);
+}
+
 // Dip-bound taper of the reference (src/q-psdm.cu:2322-2340) on the bisector
 // (X, Y, Z), Z > 0: tan = X/Z is compared as X against bound * Z, and 1/Z
 // is formed only for points inside a taper ramp.  false when outside.

@@ -267,135 +267,143 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
         span_source<NI, NW>(P, tab, column_setup(P, tab, sx, sy, x, y), wn, wk, sx - x, sy - y, SC);
       }
     };
-    float4 g;
-    for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
-      if constexpr (SRC) {
-        if (foreign_source(run, g)) continue;
-      }
-      set_source(g);
-      const float dgx = g.z - x, dgy = g.w - y;
-      const float hg2 = dgx * dgx + dgy * dgy;
-      auto c = column_setup(P, tab, g.z, g.w, x, y);
-      float unit;
-      const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
-      // time (samples) and level at the table nodes; the receiver time kept
-      // at the weight nodes that are table nodes
-      float Fn[NI + 1], Ln[NI + 1], tw[NW];
-      // (in two halves of the nodes: the raw words of all nine nodes of an
-      // eight-interval tile would hold 36 registers at once)
-      constexpr int NH = NI / 2 + 1;
-      const float4 h0 = tile_header(tab, c, wn[0]), h1 = tile_header(tab, c, wn[NI]);
+    // the aperture cull's box: the tile's columns from its first weight node
+    // to its last (they bound its points)
+    const CullBox cbox{(float)tx0 * P.dx, (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx, (float)ty0 * P.dy,
+                       (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy, (float)wk[0] * P.dw, (float)wk[NW - 1] * P.dw};
+    for (int t0 = run.tr0; t0 < tend; t0 += 64) {
+      // the live traces of the next 64, tested one per lane
+      for (unsigned long long live = live_traces(geom, t0, tend, box, cbox, D, lane); live; live &= live - 1) {
+        const int t = t0 + (int)__builtin_ctzll(live);
+        const float4 g = geom_at(geom, t);
+        if constexpr (SRC) {
+          if (foreign_source(run, g)) continue;
+        }
+        set_source(g);
+        const float dgx = g.z - x, dgy = g.w - y;
+        const float hg2 = dgx * dgx + dgy * dgy;
+        auto c = column_setup(P, tab, g.z, g.w, x, y);
+        float unit;
+        const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
+        // time (samples) and level at the table nodes; the receiver time kept
+        // at the weight nodes that are table nodes
+        float Fn[NI + 1], Ln[NI + 1], tw[NW];
+        // (in two halves of the nodes: the raw words of all nine nodes of an
+        // eight-interval tile would hold 36 registers at once)
+        constexpr int NH = NI / 2 + 1;
+        const float4 h0 = tile_header(tab, c, wn[0]), h1 = tile_header(tab, c, wn[NI]);
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int m0 = h ? NH : 0, m1 = h ? NI + 1 : NH;
-        typename Tab::Raw raw[NH];
-        tile_words<NI>(tab, c, wn, m0, m1, raw, h0, h1);
+        for (int h = 0; h < 2; ++h) {
+          const int m0 = h ? NH : 0, m1 = h ? NI + 1 : NH;
+          typename Tab::Raw raw[NH];
+          tile_words<NI>(tab, c, wn, m0, m1, raw, h0, h1);
 #pragma unroll
-        for (int m = m0; m < m1; ++m) {
-          const float2 v = tab.reduce(c, raw[m - m0]);
-          Fn[m] = fmaf(fmax_fin(v.x, tmin), P.inv_dt, SC.tsn[m]);
-          Ln[m] = __builtin_amdgcn_fmed3f(fmaf(v.y, P.inv_dtstar, SC.tsl[m]), 0.0f, jmax);
-          if constexpr (INSIDE) {
-            if (m % S == 0) tw[m / S] = v.x;
+          for (int m = m0; m < m1; ++m) {
+            const float2 v = tab.reduce(c, raw[m - m0]);
+            Fn[m] = fmaf(fmax_fin(v.x, tmin), P.inv_dt, SC.tsn[m]);
+            Ln[m] = __builtin_amdgcn_fmed3f(fmaf(v.y, P.inv_dtstar, SC.tsl[m]), 0.0f, jmax);
+            if constexpr (INSIDE) {
+              if (m % S == 0) tw[m / S] = v.x;
+            }
           }
         }
-      }
-      if constexpr (!INSIDE) {
+        if constexpr (!INSIDE) {
 #pragma unroll
-        for (int k = 0; k < NW; ++k) tw[k] = tab.at(c, wk[k]).x;
-      } else {
-        // a span cut at the band's end has its end node inside the tile's
-        // last interval, not at a tile node
+          for (int k = 0; k < NW; ++k) tw[k] = tab.at(c, wk[k]).x;
+        } else {
+          // a span cut at the band's end has its end node inside the tile's
+          // last interval, not at a tile node
 #pragma unroll
-        for (int k = 1; k < NW; ++k)
-          if (wk[k] != wa + k * S) tw[k] = tab.at(c, wk[k]).x;
-      }
-      // the ramps and amplitude at the weight nodes, and per span: bit k of
-      // out_m when the taper is 0 over span k for every valid lane, of in_m
-      // when it is 1
-      WVal W[NW];
-      float lo[NW];
+          for (int k = 1; k < NW; ++k)
+            if (wk[k] != wa + k * S) tw[k] = tab.at(c, wk[k]).x;
+        }
+        // the ramps and amplitude at the weight nodes, and per span: bit k of
+        // out_m when the taper is 0 over span k for every valid lane, of in_m
+        // when it is 1
+        WVal W[NW];
+        float lo[NW];
 #pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        W[k] = weight_value(D, R, SC.ds[k], SC.its[k], tw[k], (float)wk[k] * P.dw, SC.dsx, SC.dsy, dgx, dgy, hg2,
-                            unit, tmin);
-        lo[k] = wval_lo(W[k]);
-      }
-      unsigned out_m = 0, in_m = 0;
+        for (int k = 0; k < NW; ++k) {
+          W[k] = weight_value(D, R, SC.ds[k], SC.its[k], tw[k], (float)wk[k] * P.dw, SC.dsx, SC.dsy, dgx, dgy, hg2,
+                              unit, tmin);
+          lo[k] = wval_lo(W[k]);
+        }
+        unsigned out_m = 0, in_m = 0;
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        const WVal &A = W[k], &B = W[k + 1];
-        const float mlo = fmin_fin(lo[k], lo[k + 1]);
-        const float mout = fminf(fminf(fmax_fin(A.u1, B.u1), fmax_fin(A.u2, B.u2)),
-                                 fminf(fmax_fin(A.u3, B.u3), fmax_fin(A.u4, B.u4)));
-        if (wave_none(valid && !(mout <= 0.0f))) out_m |= 1u << k;
-        if (wave_none(valid && !(mlo >= 1.0f))) in_m |= 1u << k;
-      }
-      // the per-slot constants are read from LDS in every trace (ds_read;
-      // the asm keeps the compiler from hoisting them into registers)
-      lds_u64 cs = (lds_u64)cst;
-      asm volatile("" : "+v"(cs));
+        for (int k = 0; k < NSP; ++k) {
+          const WVal &A = W[k], &B = W[k + 1];
+          const float mlo = fmin_fin(lo[k], lo[k + 1]);
+          const float mout = fminf(fminf(fmax_fin(A.u1, B.u1), fmax_fin(A.u2, B.u2)),
+                                   fminf(fmax_fin(A.u3, B.u3), fmax_fin(A.u4, B.u4)));
+          if (wave_none(valid && !(mout <= 0.0f))) out_m |= 1u << k;
+          if (wave_none(valid && !(mlo >= 1.0f))) in_m |= 1u << k;
+        }
+        // the per-slot constants are read from LDS in every trace (ds_read;
+        // the asm keeps the compiler from hoisting them into registers)
+        lds_u64 cs = (lds_u64)cst;
+        asm volatile("" : "+v"(cs));
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        if (!((out_m >> k) & 1u)) {
-          const WVal &W0 = W[k];
-          const WVal dW = wval_sub(W[k + 1], W0);
-          const bool all_in = (in_m >> k) & 1u;
-          // every slot of the span's intervals, branch-free (a slot that
-          // holds no point reads its interval's first node and accumulates
-          // into a register that is never stored), in chunks of CI
-          // intervals: time and level, the gathers, the weights, the sums
+        for (int k = 0; k < NSP; ++k) {
+          if (!((out_m >> k) & 1u)) {
+            const WVal &W0 = W[k];
+            const WVal dW = wval_sub(W[k + 1], W0);
+            const bool all_in = (in_m >> k) & 1u;
+            // every slot of the span's intervals, branch-free (a slot that
+            // holds no point reads its interval's first node and accumulates
+            // into a register that is never stored), in chunks of CI
+            // intervals: time and level, the gathers, the weights, the sums
 #pragma unroll
-          for (int c0 = k * SI; c0 < (k + 1) * SI; c0 += CI) {
-            float cw[CI][PM], cf[CI][PM], sf[CI][PM];
-            int ci[CI][PM];
-            typename BankRaw<BankT>::type cb[CI][PM];
+            for (int c0 = k * SI; c0 < (k + 1) * SI; c0 += CI) {
+              float cw[CI][PM], cf[CI][PM], sf[CI][PM];
+              int ci[CI][PM];
+              typename BankRaw<BankT>::type cb[CI][PM];
 #pragma unroll
-            for (int i = 0; i < CI; ++i) {
-              const int m = c0 + i;
-              const float dF = Fn[m + 1] - Fn[m], dL = Ln[m + 1] - Ln[m];
+              for (int i = 0; i < CI; ++i) {
+                const int m = c0 + i;
+                const float dF = Fn[m + 1] - Fn[m], dL = Ln[m + 1] - Ln[m];
 #pragma unroll
-              for (int p = 0; p < PM; ++p) {
-                if (p >= nslot(m)) continue;
-                const float2 q = __builtin_bit_cast(float2, cs[m * PM + p]);
-                const float f = fmaf(q.x, dF, Fn[m]);
-                const unsigned j = (unsigned)fmaf(q.x, dL, Ln[m]);
-                const unsigned i0 = min((unsigned)f, ntm1);
-                cf[i][p] = __builtin_amdgcn_fractf(f);
-                ci[i][p] = (int)(__umul24(j, nt) + i0);
-                sf[i][p] = q.y;
+                for (int p = 0; p < PM; ++p) {
+                  if (p >= nslot(m)) continue;
+                  const float2 q = __builtin_bit_cast(float2, cs[m * PM + p]);
+                  const float f = fmaf(q.x, dF, Fn[m]);
+                  const unsigned j = (unsigned)fmaf(q.x, dL, Ln[m]);
+                  const unsigned i0 = min((unsigned)f, ntm1);
+                  cf[i][p] = __builtin_amdgcn_fractf(f);
+                  ci[i][p] = (int)(__umul24(j, nt) + i0);
+                  sf[i][p] = q.y;
+                }
               }
-            }
-#pragma unroll
-            for (int i = 0; i < CI; ++i)
-#pragma unroll
-              for (int p = 0; p < PM; ++p)
-                if (p < nslot(c0 + i)) cb[i][p] = bank_raw(br, ci[i][p], bank);
-            // the weights: the interpolated amplitude, times the clamped
-            // ramps unless the span's taper is 1 (one branch per chunk)
-#pragma unroll
-            for (int i = 0; i < CI; ++i)
-#pragma unroll
-              for (int p = 0; p < PM; ++p)
-                if (p < nslot(c0 + i)) cw[i][p] = fmaf(sf[i][p], dW.a, W0.a);
-            if (!all_in) {
 #pragma unroll
               for (int i = 0; i < CI; ++i)
 #pragma unroll
-                for (int p = 0; p < PM; ++p) {
-                  if (p >= nslot(c0 + i)) continue;
-                  const float s = sf[i][p];
-                  const float wx = sat(fminf(fmaf(s, dW.u1, W0.u1), fmaf(s, dW.u2, W0.u2)));
-                  const float wy = sat(fminf(fmaf(s, dW.u3, W0.u3), fmaf(s, dW.u4, W0.u4)));
-                  cw[i][p] *= wx * wy;
-                }
+                for (int p = 0; p < PM; ++p)
+                  if (p < nslot(c0 + i)) cb[i][p] = bank_raw(br, ci[i][p], bank);
+              // the weights: the interpolated amplitude, times the clamped
+              // ramps unless the span's taper is 1 (one branch per chunk)
+#pragma unroll
+              for (int i = 0; i < CI; ++i)
+#pragma unroll
+                for (int p = 0; p < PM; ++p)
+                  if (p < nslot(c0 + i)) cw[i][p] = fmaf(sf[i][p], dW.a, W0.a);
+              if (!all_in) {
+#pragma unroll
+                for (int i = 0; i < CI; ++i)
+#pragma unroll
+                  for (int p = 0; p < PM; ++p) {
+                    if (p >= nslot(c0 + i)) continue;
+                    const float s = sf[i][p];
+                    const float wx = sat(fminf(fmaf(s, dW.u1, W0.u1), fmaf(s, dW.u2, W0.u2)));
+                    const float wy = sat(fminf(fmaf(s, dW.u3, W0.u3), fmaf(s, dW.u4, W0.u4)));
+                    cw[i][p] *= wx * wy;
+                  }
+              }
+#pragma unroll
+              for (int i = 0; i < CI; ++i)
+#pragma unroll
+                for (int p = 0; p < PM; ++p)
+                  if (p < nslot(c0 + i))
+                    acc[c0 + i][p] = fmaf(cw[i][p], bank_interp(cb[i][p], cf[i][p], bank), acc[c0 + i][p]);
             }
-#pragma unroll
-            for (int i = 0; i < CI; ++i)
-#pragma unroll
-              for (int p = 0; p < PM; ++p)
-                if (p < nslot(c0 + i))
-                  acc[c0 + i][p] = fmaf(cw[i][p], bank_interp(cb[i][p], cf[i][p], bank), acc[c0 + i][p]);
           }
         }
       }

@@ -119,7 +119,10 @@ __device__ inline float dip_taper(const Dip &D, const Ramps &R, float X, float Y
 // at ze (the node form's exact parts); without it they cover [0, nz).  Both
 // are compile-time so that the common kernels carry neither the registers
 // nor the per-trace tests (they cost the exact body 4-14 % in time).
-template <int ZP, bool SRC, bool RANGE, typename BankT, typename Tab, typename Runs>
+// CULL: traces are tested 64 at a time, one per lane, against the tile box
+// and the dip aperture over the tile's points (live_traces); without it one
+// at a time against the tile box on the scalar unit (next_live).
+template <int ZP, bool SRC, bool RANGE, bool CULL, typename BankT, typename Tab, typename Runs>
 __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__restrict__ img,
                                   const float4 *__restrict__ geom, const BankT *__restrict__ bank,
                                   int tile_col, int zgroup, const Runs &runs, int nrun) {
@@ -185,10 +188,9 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
         source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, zz, z2, sx - x, sy - y, S);
       }
     };
-    float4 g;
-    for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
+    auto image_trace = [&](int t, const float4 &g) {
       if constexpr (SRC) {
-        if (foreign_source(run, g)) continue;
+        if (foreign_source(run, g)) return;
       }
       set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
@@ -246,6 +248,21 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       for (int i = 0; i < ZP; ++i) cb[i] = bank_raw(br, ci[i], bank);
 #pragma unroll
       for (int i = 0; i < ZP; ++i) acc[i] = fmaf(cw[i], bank_interp(cb[i], cf[i], bank), acc[i]);
+    };
+    if constexpr (CULL) {
+      // the traces of each 64 that may update the tile, tested one per lane
+      // against the tile box and the aperture over the tile's points
+      const CullBox cbox{(float)tx0 * P.dx, (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx, (float)ty0 * P.dy,
+                         (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy, zz[0], (float)(z0 + nzv - 1) * P.dz};
+      for (int t0 = run.tr0; t0 < tend; t0 += 64)
+        for (unsigned long long live = live_traces(geom, t0, tend, box, cbox, D, lane); live; live &= live - 1) {
+          const int t = t0 + (int)__builtin_ctzll(live);
+          image_trace(t, geom_at(geom, t));
+        }
+    } else {
+      float4 g;
+      for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g))
+        image_trace(t, g);
     }
   }
   if (valid) {
