@@ -220,67 +220,75 @@ __device__ inline void image_tile_nodes(const ImgDev &P, const Tab &tab, float *
         node_source<NI>(P, tab, column_setup(P, tab, sx, sy, x, y), wn, zn, sx - x, sy - y, S);
       }
     };
-    float4 g;
-    for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g)) {
-      if constexpr (SRC) {
-        if (foreign_source(run, g)) continue;
-      }
-      set_source(g);
-      const float dgx = g.z - x, dgy = g.w - y;
-      const float hg2 = dgx * dgx + dgy * dgy;
-      auto c = column_setup(P, tab, g.z, g.w, x, y);
-      float unit;
-      const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
-      // Three phases as in image_tile: the weights, fractions and sample
-      // indices of every point (node values computed once per node), then
-      // every gather in one branch-free block, then decode and accumulate.
-      // slots that hold no point are left unset: every later read of a
-      // slot is under the same test
-      float cw[NI][PM], cf[NI][PM];
-      int ci[NI][PM];
-      QPSDM_PHASE();
-      NodeVal n0 = node_value<NI>(P, D, R, S, 0, tab.at(c, wn[0]), zn[0], dgx, dgy, hg2, unit, tmin);
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        const NodeVal n1 = node_value<NI>(P, D, R, S, m + 1, tab.at(c, wn[m + 1]), zn[m + 1], dgx, dgy, hg2,
-                                          unit, tmin);
-        const NodeVal d = node_sub(n1, n0);
-#pragma unroll
-        for (int p = 0; p < PM; ++p) {
-          if (!QPSDM_ON(m, p)) continue;
-          const float s = cf_pt[m][p];
-          const float f = fmaf(s, d.F, n0.F);
-          // min(sat(a), sat(b)) = sat(min(a, b)): one clamp per axis
-          const float wx = sat(fminf(fmaf(s, d.u1, n0.u1), fmaf(s, d.u2, n0.u2)));
-          const float wy = sat(fminf(fmaf(s, d.u3, n0.u3), fmaf(s, d.u4, n0.u4)));
-          const float w = (wx * wy) * fmaf(s, d.a, n0.a);
-          const int i0 = (int)f;
-          const int j = (int)__builtin_amdgcn_fmed3f(fmaf(s, d.L, n0.L), 0.0f, jmax);
-          // a lane outside the trace keeps weight 0 and reads at an index
-          // inside the bank or past its end (which reads 0): no branch
-          const bool ok = valid && (unsigned)(i0 - 1) < nt - 2u;
-          cf[m][p] = __builtin_amdgcn_fractf(f);
-          ci[m][p] = i0 + (int)__umul24((unsigned)j, nt);
-          cw[m][p] = ok ? w : 0.0f;
+    // the aperture cull's box: the tile's columns from its first node to
+    // its last (they bound its points)
+    const CullBox cbox{(float)tx0 * P.dx, (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx, (float)ty0 * P.dy,
+                       (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy, zn[0], zn[NI]};
+    for (int t0 = run.tr0; t0 < tend; t0 += 64) {
+      // the live traces of the next 64, tested one per lane
+      for (unsigned long long live = live_traces(geom, t0, tend, box, cbox, D, lane); live; live &= live - 1) {
+        const int t = t0 + (int)__builtin_ctzll(live);
+        const float4 g = geom_at(geom, t);
+        if constexpr (SRC) {
+          if (foreign_source(run, g)) continue;
         }
-        n0 = n1;
-      }
-      typename BankRaw<BankT>::type cb[NI][PM];
-      QPSDM_PHASE();
+        set_source(g);
+        const float dgx = g.z - x, dgy = g.w - y;
+        const float hg2 = dgx * dgx + dgy * dgy;
+        auto c = column_setup(P, tab, g.z, g.w, x, y);
+        float unit;
+        const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
+        // Three phases as in image_tile: the weights, fractions and sample
+        // indices of every point (node values computed once per node), then
+        // every gather in one branch-free block, then decode and accumulate.
+        // slots that hold no point are left unset: every later read of a
+        // slot is under the same test
+        float cw[NI][PM], cf[NI][PM];
+        int ci[NI][PM];
+        QPSDM_PHASE();
+        NodeVal n0 = node_value<NI>(P, D, R, S, 0, tab.at(c, wn[0]), zn[0], dgx, dgy, hg2, unit, tmin);
 #pragma unroll
-      for (int m = 0; m < NI; ++m)
+        for (int m = 0; m < NI; ++m) {
+          const NodeVal n1 = node_value<NI>(P, D, R, S, m + 1, tab.at(c, wn[m + 1]), zn[m + 1], dgx, dgy, hg2,
+                                            unit, tmin);
+          const NodeVal d = node_sub(n1, n0);
 #pragma unroll
-        for (int p = 0; p < PM; ++p)
-          if (QPSDM_ON(m, p)) cb[m][p] = bank_raw(br, ci[m][p], bank);
-      QPSDM_PHASE();
-#pragma unroll
-      for (int m = 0; m < NI; ++m)
-#pragma unroll
-        for (int p = 0; p < PM; ++p) {
-          if (!QPSDM_ON(m, p)) continue;
-          const float2 b = bank_decode(cb[m][p], bank);
-          acc[m][p] = fmaf(cw[m][p], fmaf(cf[m][p], b.y - b.x, b.x), acc[m][p]);
+          for (int p = 0; p < PM; ++p) {
+            if (!QPSDM_ON(m, p)) continue;
+            const float s = cf_pt[m][p];
+            const float f = fmaf(s, d.F, n0.F);
+            // min(sat(a), sat(b)) = sat(min(a, b)): one clamp per axis
+            const float wx = sat(fminf(fmaf(s, d.u1, n0.u1), fmaf(s, d.u2, n0.u2)));
+            const float wy = sat(fminf(fmaf(s, d.u3, n0.u3), fmaf(s, d.u4, n0.u4)));
+            const float w = (wx * wy) * fmaf(s, d.a, n0.a);
+            const int i0 = (int)f;
+            const int j = (int)__builtin_amdgcn_fmed3f(fmaf(s, d.L, n0.L), 0.0f, jmax);
+            // a lane outside the trace keeps weight 0 and reads at an index
+            // inside the bank or past its end (which reads 0): no branch
+            const bool ok = valid && (unsigned)(i0 - 1) < nt - 2u;
+            cf[m][p] = __builtin_amdgcn_fractf(f);
+            ci[m][p] = i0 + (int)__umul24((unsigned)j, nt);
+            cw[m][p] = ok ? w : 0.0f;
+          }
+          n0 = n1;
         }
+        typename BankRaw<BankT>::type cb[NI][PM];
+        QPSDM_PHASE();
+#pragma unroll
+        for (int m = 0; m < NI; ++m)
+#pragma unroll
+          for (int p = 0; p < PM; ++p)
+            if (QPSDM_ON(m, p)) cb[m][p] = bank_raw(br, ci[m][p], bank);
+        QPSDM_PHASE();
+#pragma unroll
+        for (int m = 0; m < NI; ++m)
+#pragma unroll
+          for (int p = 0; p < PM; ++p) {
+            if (!QPSDM_ON(m, p)) continue;
+            const float2 b = bank_decode(cb[m][p], bank);
+            acc[m][p] = fmaf(cw[m][p], fmaf(cf[m][p], b.y - b.x, b.x), acc[m][p]);
+          }
+      }
     }
   }
   if (valid) {
