@@ -73,6 +73,38 @@ def test_batch_equals_per_shot(gpu, mk, launch):
         assert torch.equal(img, ref), (launch, nwg)
 
 
+@pytest.mark.parametrize("aperture", [20.0, 45.0, 60.0])
+def test_aperture_cull_keeps_every_update(gpu, aperture):
+    """The batched kernels skip a trace for a tile when a bound of the
+    bisector's tangent over the tile puts it past a cut-off (tested 64 traces
+    at a time); the per-shot kernel tests every point.  A dense fan of
+    receivers around and beyond the image, with sources inside and outside
+    it, puts many (tile, trace) pairs at the aperture's edge: the two images
+    are equal bit for bit at every tile depth."""
+    cfg = synth.tiny()
+    sv, tab = _survey(cfg)
+    ext = ((cfg.nx - 1) * cfg.dx, (cfg.ny - 1) * cfg.dy)
+    u = np.linspace(-1.5, 2.5, 23)
+    gx, gy = np.meshgrid(u * ext[0], u * ext[1])
+    rec = np.stack([gx.ravel(), gy.ravel()], 1)
+    shots = [(0.5 * ext[0], 0.5 * ext[1], rec, aperture), (-0.8 * ext[0], 1.7 * ext[1], rec, aperture),
+             (2.2 * ext[0], -0.4 * ext[1], rec[::3], aperture)]
+    geom, ranges = synth.geometry(shots)
+    desc = [(tr0, n) + synth.dip_tans(a) for (tr0, n), (*_, a) in zip(ranges, shots)]
+    traces = np.random.default_rng(12).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
+    bank = sv.qbank(torch.from_numpy(traces).cuda())
+    g_d = torch.from_numpy(geom).cuda()
+    for zp in (2, 4, 8):
+        launch = (8, 8, zp, 0)
+        ref = sv.new_image()
+        _per_shot(sv, ref, g_d, bank, tab, desc, launch=launch)
+        img = sv.new_image()
+        sv.image_batch(img, g_d, bank, tab, engine.ShotList(desc), launch=launch, nwg=-1)
+        torch.cuda.synchronize()
+        assert float(ref.abs().max()) > 0
+        assert torch.equal(img, ref), zp
+
+
 def test_batch_matches_fp64_oracle_and_count(gpu):
     cfg = synth.tiny_layers()
     sv, tab = _survey(cfg)
