@@ -393,6 +393,8 @@ constexpr int kQBrickBytes = 16 + 8 * 8 * 8 * 16;
 // Packed per row: (w00 | w01 << 16) and (w10 | w11 << 16).
 __host__ __device__ inline void cell_weights(float a, float b, unsigned *row0, unsigned *row1) {
   const unsigned wa = (unsigned)fmaf(a, 65535.0f, 0.5f), wb = (unsigned)fmaf(b, 65535.0f, 0.5f);
+  // a, b in [0, 1]: 16-bit weights, whose product a 24-bit multiply forms
+  __builtin_assume(wa <= 65535u && wb <= 65535u);
   const unsigned x = wa * wb;
   const unsigned w11 = (x + (x >> 16) + 1u) >> 16;
   *row0 = (65535u - wa - wb + w11) | ((wa - w11) << 16);
@@ -527,16 +529,26 @@ __device__ inline void frac_index(float q, int n, int *i0, float *a) {
 // Table column of the surface position (px, py) under the image column
 // (x, y): nearest node, then bilinear weights at the relative position
 // (x - px, y - py).
-template <typename Tab>
-__device__ inline typename Tab::Col column_setup(const ImgDev &P, const Tab &tab, float px, float py,
-                                                 float x, float y) {
+// The nearest surface node of (px, py).
+__device__ inline int surface_node(const ImgDev &P, float px, float py) {
   const int in = min(max((int)floorf((px - P.n0x) * P.inv_ndx + 0.5f), 0), P.nnx - 1);
   const int jn = min(max((int)floorf((py - P.n0y) * P.inv_ndy + 0.5f), 0), P.nny - 1);
+  return jn * P.nnx + in;
+}
+// (the node given: a caller that has it from a lane-parallel pass)
+template <typename Tab>
+__device__ inline typename Tab::Col column_at_node(const ImgDev &P, const Tab &tab, int node, float px, float py,
+                                                   float x, float y) {
   int iu, iv;
   float a, b;
   frac_index((x - px - P.u0) * P.inv_du, P.nu, &iu, &a);
   frac_index((y - py - P.v0) * P.inv_dv, P.nv, &iv, &b);
-  return tab.column(jn * P.nnx + in, iv, iu, a, b);
+  return tab.column(node, iv, iu, a, b);
+}
+template <typename Tab>
+__device__ inline typename Tab::Col column_setup(const ImgDev &P, const Tab &tab, float px, float py,
+                                                 float x, float y) {
+  return column_at_node(P, tab, surface_node(P, px, py), px, py, x, y);
 }
 
 // Depth node and weight of depth z: w = min(floor(z/dw), nw-2), frac clamped.
@@ -677,15 +689,19 @@ __device__ inline bool aperture_out(const CullBox &B, const Dip &D, float sx, fl
 
 // The traces of [t0, t0 + 64) that may update the tile, one per lane (bit l
 // for trace t0 + l): not rejected, not culled by the tile box, and not out
-// of the aperture over the cull box.  Sixty-four traces cost one pass of
-// the bounds above.
-__device__ inline unsigned long long live_traces(const float4 *__restrict__ geom, int t0, int tend, const TileBox &box,
-                                                 const CullBox &cb, const Dip &D, int lane) {
+// of the aperture over the cull box; *node: lane l's receiver's surface
+// node (read back per trace with readlane).  Sixty-four traces cost one
+// pass of the bounds above.
+__device__ inline unsigned long long live_traces(const ImgDev &P, const float4 *__restrict__ geom, int t0, int tend,
+                                                 const TileBox &box, const CullBox &cb, const Dip &D, int lane,
+                                                 int *node) {
   const int t = t0 + lane;
   bool live = false;
+  *node = 0;
   if (t < tend) {
     const float4 g = geom[t];
     live = !(rejected(g.x) | tile_culled(box, g.x, g.y, g.z, g.w)) && !aperture_out(cb, D, g.x, g.y, g.z, g.w);
+    if (live) *node = surface_node(P, g.z, g.w);
   }
   return __builtin_amdgcn_ballot_w64(live);
 }

@@ -273,8 +273,10 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
                        (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy, (float)wk[0] * P.dw, (float)wk[NW - 1] * P.dw};
     for (int t0 = run.tr0; t0 < tend; t0 += 64) {
       // the live traces of the next 64, tested one per lane
-      for (unsigned long long live = live_traces(geom, t0, tend, box, cbox, D, lane); live; live &= live - 1) {
-        const int t = t0 + (int)__builtin_ctzll(live);
+      int lnode;
+      for (unsigned long long live = live_traces(P, geom, t0, tend, box, cbox, D, lane, &lnode); live;
+           live &= live - 1) {
+        const int b = (int)__builtin_ctzll(live), t = t0 + b;
         const float4 g = geom_at(geom, t);
         if constexpr (SRC) {
           if (foreign_source(run, g)) continue;
@@ -282,7 +284,7 @@ __device__ inline void image_tile_spans(const ImgDev &P, const Tab &tab, float *
         set_source(g);
         const float dgx = g.z - x, dgy = g.w - y;
         const float hg2 = dgx * dgx + dgy * dgy;
-        auto c = column_setup(P, tab, g.z, g.w, x, y);
+        auto c = column_at_node(P, tab, __builtin_amdgcn_readlane(lnode, b), g.z, g.w, x, y);
         float unit;
         const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
         // time (samples) and level at the table nodes; the receiver time kept

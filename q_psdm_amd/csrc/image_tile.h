@@ -188,14 +188,14 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
         source_leg<ZP>(P, tab, column_setup(P, tab, sx, sy, x, y), wz, fz, zz, z2, sx - x, sy - y, S);
       }
     };
-    auto image_trace = [&](int t, const float4 &g) {
+    auto image_trace = [&](int t, const float4 &g, int rnode) {
       if constexpr (SRC) {
         if (foreign_source(run, g)) return;
       }
       set_source(g);
       const float dgx = g.z - x, dgy = g.w - y;
       const float hg2 = dgx * dgx + dgy * dgy;
-      auto c = column_setup(P, tab, g.z, g.w, x, y);
+      auto c = column_at_node(P, tab, rnode, g.z, g.w, x, y);
       float unit;  // stored sample -> filtered-trace units (FP16 banks carry a per-trace scale)
       const __amdgpu_buffer_rsrc_t br = bank_row(bank, (size_t)(t - run.tr0 + run.brow0), P.K, P.nt, &unit);
       int cur = wz[0];
@@ -254,15 +254,18 @@ __device__ inline void image_tile(const ImgDev &P, const Tab &tab, float *__rest
       // against the tile box and the aperture over the tile's points
       const CullBox cbox{(float)tx0 * P.dx, (float)(min(tx0 + P.tile_x, P.nx) - 1) * P.dx, (float)ty0 * P.dy,
                          (float)(min(ty0 + P.tile_y, P.ny) - 1) * P.dy, zz[0], (float)(z0 + nzv - 1) * P.dz};
-      for (int t0 = run.tr0; t0 < tend; t0 += 64)
-        for (unsigned long long live = live_traces(geom, t0, tend, box, cbox, D, lane); live; live &= live - 1) {
-          const int t = t0 + (int)__builtin_ctzll(live);
-          image_trace(t, geom_at(geom, t));
+      for (int t0 = run.tr0; t0 < tend; t0 += 64) {
+        int lnode;
+        for (unsigned long long live = live_traces(P, geom, t0, tend, box, cbox, D, lane, &lnode); live;
+             live &= live - 1) {
+          const int b = (int)__builtin_ctzll(live);
+          image_trace(t0 + b, geom_at(geom, t0 + b), __builtin_amdgcn_readlane(lnode, b));
         }
+      }
     } else {
       float4 g;
       for (int t = next_live(geom, box, run.tr0, tend, g); t < tend; t = next_live(geom, box, t + 1, tend, g))
-        image_trace(t, g);
+        image_trace(t, g, surface_node(P, g.z, g.w));
     }
   }
   if (valid) {
