@@ -295,6 +295,7 @@ extern "C" int qpsdm_batch_graph_create(const qpsdm_bank_params *bp, const float
     rc = batch_dispatch(ip, shape, bank, bank_dtype, tab, table_dtype, q);
     if (rc) return rc;
   }
+  if (ip->weight_nodes == 2 && (stages & 4) && !fanout()) return QPSDM_EHIP;  // its streams before the capture
   hipStream_t s;
   QPSDM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   qpsdm_graph *g = new qpsdm_graph();

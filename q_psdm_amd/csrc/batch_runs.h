@@ -5,6 +5,8 @@
 #ifndef QPSDM_BATCH_RUNS_H_
 #define QPSDM_BATCH_RUNS_H_
 
+#include <mutex>
+
 #include "image_spans.h"
 
 namespace qpsdm_batch {
@@ -95,6 +97,23 @@ struct Go {
   bool run;
   int *nwg_out;
 };
+
+// The streams a span-form call spreads its launches over: the exact body
+// and each band image disjoint points and are independent, so they run
+// concurrently (each one's shots stay in order on its own stream).  One set
+// per device, created on first use and kept; a call forks the set from its
+// stream with an event and joins it back before returning, under the set's
+// mutex, so the call stays ordered on its stream and a captured graph holds
+// one fork and one join.
+struct Fanout {
+  static constexpr int kStreams = 6;  // the exact body + up to five bands
+  hipStream_t s[kStreams];
+  hipEvent_t fork, join[kStreams];
+  std::mutex m;
+};
+// The current device's set (nullptr with the error set when it cannot be
+// created); call it once before a graph capture that holds span launches.
+Fanout *fanout();
 
 // Span form of every shot (defined in spans.hip).
 int launch_batch_spans(const qpsdm_image_params *p, const qk::ImgDev &P, const void *tab, int table_dtype,

@@ -2,14 +2,38 @@
 // order, the exact body above the node range and beyond the tables' depth
 // range, then one launch per band of the node range (one work-group per
 // tile of tile_z intervals, the band's span length a compile-time
-// parameter).  Each point is imaged by one launch per shot and every launch
-// of a shot precedes the next shot's, so per point the sum still runs over
-// the shots in order and each shot's traces in order.  This unit holds the
+// parameter).  The exact body and each band run on streams of their own
+// (they image disjoint points); each point is imaged by one launch per shot
+// on one stream, in shot order, so per point the sum still runs over the
+// shots in order and each shot's traces in order.  This unit holds the
 // dispatch and the FP32 pair bank with FP32 tables (the parity form);
 // spans_f16.hip the FP16 pair bank with U16Q tables.
 #include "spans_impl.h"
 
 namespace qpsdm_batch {
+
+Fanout *fanout() {
+  static std::mutex m;
+  static Fanout *sets[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    qpsdm::set_error(QPSDM_EHIP, "image_batch: no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(m);
+  if (sets[dev]) return sets[dev];
+  Fanout *f = new Fanout();
+  bool ok = hipEventCreateWithFlags(&f->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < Fanout::kStreams && ok; ++i)
+    ok = hipStreamCreateWithFlags(&f->s[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&f->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    qpsdm::set_error(QPSDM_EHIP, "image_batch: creating the span-form streams failed");
+    return nullptr;  // (the partial set is kept out of use; its handles live until the process ends)
+  }
+  sets[dev] = f;
+  return f;
+}
 
 int launch_spans_f32p(const qpsdm_image_params *p, const qk::ImgDev &P, const void *tab, const void *bank,
                       const Go &go) {

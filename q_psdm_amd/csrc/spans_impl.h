@@ -12,6 +12,8 @@
 #include "common.h"
 
 namespace qpsdm_batch {
+
+
 namespace spans_detail {
 
 template <int NI, int S, int PAT, typename BankT, typename Tab, bool SRC>
@@ -71,36 +73,52 @@ template <bool SRC, typename BankT, typename Tab>
 int launch_spans_t(const qk::ImgDev &P, const Tab &tab, const BankT *bank, const Go &go) {
   if (go.nwg_out) *go.nwg_out = -1;
   if (!go.run) return QPSDM_OK;
+  Fanout *F = fanout();
+  if (!F) return QPSDM_EHIP;
+  std::lock_guard<std::mutex> lock(F->m);
+  QPSDM_HIP(hipEventRecord(F->fork, go.s));
+  for (int i = 0; i < Fanout::kStreams; ++i) QPSDM_HIP(hipStreamWaitEvent(F->s[i], F->fork, 0));
   const int ncol = ((P.nx + P.tile_x - 1) / P.tile_x) * ((P.ny + P.tile_y - 1) / P.tile_y);
   qk::ImgDev E = P;
   E.tile_z = 8;
   const int n1 = (P.nf_ps + 7) / 8, n2 = (P.nz - P.nf_pe + 7) / 8;
-  for (int r = 0; r < go.nshot; ++r) {
+  int rc = QPSDM_OK;
+  for (int r = 0; r < go.nshot && !rc; ++r) {
+    // the exact body above the node range and past the tables' depth range
     if (n1 > 0) {
       E.zb = 0;
       E.ze = P.nf_ps;
-      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n1), dim3(64), 0, go.s,
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n1), dim3(64), 0, F->s[0],
                          E, tab, go.img, go.geom, bank, go.runs, r, ncol);
     }
     if (n2 > 0) {
       E.zb = P.nf_pe;
       E.ze = P.nz;
-      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n2), dim3(64), 0, go.s,
+      hipLaunchKernelGGL((image_batch_tiles_kernel<8, BankT, Tab, SRC, true>), dim3(ncol * n2), dim3(64), 0, F->s[0],
                          E, tab, go.img, go.geom, bank, go.runs, r, ncol);
     }
-    for (int k = 0; k < P.sp_n; ++k) {
+    // each band on its own stream
+    for (int k = 0; k < P.sp_n && !rc; ++k) {
       qk::ImgDev B = P;
       B.sp_b = P.sp_w[k];
       B.sp_e = P.sp_w[k + 1];
       const int S = P.sp_s[k];
       const bool last = k == P.sp_n - 1;
-      const int rc = std::max(P.tile_z, S) == 8 ? launch_band_ni<8, SRC>(P, B, S, last, ncol, tab, bank, go, r)
-                                                : launch_band_ni<4, SRC>(P, B, S, last, ncol, tab, bank, go, r);
-      if (rc) return rc;
+      Go gk = go;
+      gk.s = F->s[1 + k];
+      rc = std::max(P.tile_z, S) == 8 ? launch_band_ni<8, SRC>(P, B, S, last, ncol, tab, bank, gk, r)
+                                      : launch_band_ni<4, SRC>(P, B, S, last, ncol, tab, bank, gk, r);
     }
-    QPSDM_HIP(hipGetLastError());
+    if (!rc && hipGetLastError() != hipSuccess) rc = qpsdm::set_error(QPSDM_EHIP, "image_batch: span launch failed");
   }
-  return QPSDM_OK;
+  // the join, also after a failed launch (a capture needs every fork joined)
+  for (int i = 0; i < Fanout::kStreams; ++i) {
+    const hipError_t e1 = hipEventRecord(F->join[i], F->s[i]);
+    const hipError_t e2 = hipStreamWaitEvent(go.s, F->join[i], 0);
+    if (!rc && (e1 != hipSuccess || e2 != hipSuccess))
+      rc = qpsdm::set_error(QPSDM_EHIP, "image_batch: joining the span streams failed");
+  }
+  return rc;
 }
 
 // Tiles of 4 or 8 intervals.
