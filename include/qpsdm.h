@@ -385,7 +385,13 @@ typedef struct {
  * resident capacity), or nwg = -1: one work-group per tile, scheduled by the
  * hardware instead of the counter.  Either grid is dispatched once per shot,
  * in shot order (same tiles and summation order, so the same image bit for
- * bit).  The image params' own dip bounds are not used; taperzone is. */
+ * bit).  The image params' own dip bounds are not used; taperzone is.
+ * Traces that provably miss a tile's dip aperture are skipped (exactly:
+ * they would add nothing).  The span form (weight_nodes = 2) spreads its
+ * exact part and its bands over library streams of the current device,
+ * forked from and joined back to `stream` within the call (captures as one
+ * fork and join); calls from several host threads on one device are
+ * serialised while they enqueue. */
 int qpsdm_image_batch(const qpsdm_image_params *p, const qpsdm_launch *shape, float *img_dev,
                       const float *geom_dev, const void *bank_dev, int bank_dtype, const void *tab_dev,
                       int table_dtype, const qpsdm_shot *shots_dev, int nshot, unsigned *work_dev, int nwg,
