@@ -836,17 +836,20 @@ def time_end_to_end(W, torch, engine, nshot=2, launch=None):
     """End-to-end rate through the pipeline context: big-endian SU records of
     nshot shots in host memory -> pinned staging slots -> H2D copy (its own
     stream) -> device decode -> FP16 Q bank -> imaging with U16Q tables ->
-    image copied back to host memory."""
+    image copied back to host memory.  The mixed form's survey: in the span
+    form the context stores a pair bank and images each slot as one batched
+    shot (the node tables' source legs: no source tables here)."""
     from q_psdm_amd import lib as L
-    sv, cfg = W.sv, W.cfg
+    sv, cfg = W.svm, W.cfg
+    spans = sv.form == "spans"
     idx = list(range(min(nshot, len(W.ranges))))
     rows = np.concatenate([W.geom[W.ranges[i][0]:W.ranges[i][0] + W.ranges[i][1]] for i in idx])
     traces = torch.cat([W.traces[i] for i in idx], 0).cpu().numpy()
     wxy = np.round(rows.astype(np.float64)).astype(np.int32)
     su = engine.su_encode(traces, wxy)
     gp = L.GeomParams(1.0, -1e30, 1e30, -1e30, 1e30, 0.0, 0.0, 0.0, 0.0, cfg.dx, cfg.dy)
-    pl = engine.Pipeline(sv, gp, bank_dtype="f16", table_dtype="u16q", batch_traces=max(W.ranges[i][1] for i in idx),
-                         launch=launch)
+    pl = engine.Pipeline(sv, gp, bank_dtype="f16p" if spans else "f16", table_dtype="u16q",
+                         batch_traces=max(W.ranges[i][1] for i in idx), launch=launch)
     pl.set_tables(W.tab)
     pl.push_su(su[:240 + 4 * cfg.nt])
     pl.image()
@@ -862,8 +865,9 @@ def time_end_to_end(W, torch, engine, nshot=2, launch=None):
     del img
     return {"shots": len(idx), "traces": int(rows.shape[0]), "su_bytes": len(su), "wall_ms": wall * 1e3,
             "updates_per_s": upd / wall, "image_bytes_to_host": cfg.nx * cfg.ny * cfg.nz * 4,
-            "path": "SU bytes in host memory -> pinned slot -> H2D (copy stream) -> device decode -> f16 Q bank -> "
-                    "per-shot imaging (u16q tables, exact form) -> image to host"}
+            "path": "SU bytes in host memory -> pinned slot -> H2D (copy stream) -> device decode -> f16 "
+                    + ("pair Q bank -> batched span-form imaging" if spans else "Q bank -> per-shot imaging")
+                    + f" (u16q tables, {sv.form} form) -> image to host"}
 
 
 def time_segy_decode(torch, ntr=20000, ns=2001, reps=5):
@@ -976,7 +980,7 @@ def main():
         args.fp32_launch = shapes["fp32"][0] if "fp32" in shapes else (8, 8, 8)
         extras["config4"] = time_config4(args, torch, engine, synth, W)
         extras["segy_decode"] = time_segy_decode(torch)
-        extras["end_to_end"] = time_end_to_end(W, torch, engine)
+        extras["end_to_end"] = time_end_to_end(W, torch, engine, launch=shapes.get("mixed", ((8, 8, 4), -1))[0])
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, args.cpu_traces)

@@ -516,7 +516,10 @@ typedef struct {
   qpsdm_image_params image;  /* grid, traces, dip taper, table geometry       */
   qpsdm_bank_params bank;    /* windows and Q bank (nt, K, dtstar as image)   */
   qpsdm_geom_params geom;    /* SU coordinate scale, model extent, origin     */
-  int bank_dtype;            /* QPSDM_F32, QPSDM_BF16 or QPSDM_F16            */
+  int bank_dtype;            /* QPSDM_F32, QPSDM_BF16 or QPSDM_F16; the span  */
+                             /* form (image.weight_nodes = 2): QPSDM_F16P   */
+                             /* with QPSDM_U16Q or QPSDM_F32P with QPSDM_F32 */
+                             /* tables, each slot imaged as one batched shot */
   int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X or QPSDM_U16Q */
   int batch_traces;          /* traces per staging slot                       */
   qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */

@@ -46,6 +46,8 @@ struct Slot {
   GeomKeys keys = kStdGeom;       // records' sx, sy, gx, gy positions
   hipEvent_t copied = nullptr, done = nullptr;
   bool used = false;
+  // span form: the slot's traces as one shot (pinned host copy, device copy)
+  qpsdm_shot *shot_host = nullptr, *shot_dev = nullptr;
 };
 
 struct GeomDev {
@@ -195,6 +197,7 @@ struct qpsdm_ctx {
   float *img = nullptr;
   void *tab = nullptr;
   bool have_tab = false;
+  unsigned *work = nullptr;  // the batched entry point's work counter (span form)
   Slot slot[kSlots];
   int cur = 0;
   size_t rec_bytes = 0, slot_bytes = 0;
@@ -222,6 +225,13 @@ int submit(qpsdm_ctx *c) {
     QPSDM_HIP(hipMemcpyAsync(s.geom, s.host + (size_t)c->p.batch_traces * nt * 4, (size_t)s.ntr * 16,
                              hipMemcpyHostToDevice, c->copy));
   }
+  if (c->p.image.weight_nodes == 2) {
+    // the slot's shot descriptor, with the staged data (the host rewrites it
+    // only after this slot's copies are done)
+    const qpsdm_image_params &ip = c->p.image;
+    *s.shot_host = qpsdm_shot{0, s.ntr, ip.tanx1, ip.tanx2, ip.tany1, ip.tany2};
+    QPSDM_HIP(hipMemcpyAsync(s.shot_dev, s.shot_host, sizeof(qpsdm_shot), hipMemcpyHostToDevice, c->copy));
+  }
   QPSDM_HIP(hipEventRecord(s.copied, c->copy));
   QPSDM_HIP(hipStreamWaitEvent(c->compute, s.copied, 0));
   if (s.su) {
@@ -237,8 +247,17 @@ int submit(qpsdm_ctx *c) {
   }
   int rc = qpsdm_qbank(&c->p.bank, s.samples, s.ntr, s.bank, c->p.bank_dtype, c->compute);
   if (rc) return rc;
-  rc = qpsdm_image_shot(&c->p.image, &c->p.shape, c->img, s.geom, 0, s.ntr, s.bank, c->p.bank_dtype,
-                        c->tab, c->p.table_dtype, c->compute);
+  if (c->p.image.weight_nodes == 2) {
+    // the span form images through the batched entry point: the slot's
+    // traces as one shot under the context's dip bounds (each trace's
+    // source leg from the node tables at its own source)
+    const qpsdm_image_params &ip = c->p.image;
+    rc = qpsdm_image_batch(&ip, &c->p.shape, c->img, s.geom, s.bank, c->p.bank_dtype, c->tab, c->p.table_dtype,
+                           s.shot_dev, 1, c->work, -1, c->compute);
+  } else {
+    rc = qpsdm_image_shot(&c->p.image, &c->p.shape, c->img, s.geom, 0, s.ntr, s.bank, c->p.bank_dtype, c->tab,
+                          c->p.table_dtype, c->compute);
+  }
   if (rc) return rc;
   if (c->ngcol) {
     rc = qpsdm_gathers_shot(&c->p.image, &c->gp, c->gcols, c->ngcol, c->gx, c->gy, s.geom, 0, s.ntr, s.bank,
@@ -318,8 +337,14 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
   QPSDM_REQUIRE(p->batch_traces >= 1 && p->batch_traces <= (1 << 20), "ctx_create: batch_traces in [1, 2^20]");
   QPSDM_REQUIRE(p->image.nt == p->bank.nt && p->image.K == p->bank.K && p->image.dtstar == p->bank.dtstar,
                 "ctx_create: image and bank parameters disagree on nt/K/dtstar");
-  QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16 || p->bank_dtype == QPSDM_F16,
-                "ctx_create: bank dtype");
+  if (p->image.weight_nodes == 2)
+    QPSDM_REQUIRE((p->bank_dtype == QPSDM_F16P && p->table_dtype == QPSDM_U16Q) ||
+                      (p->bank_dtype == QPSDM_F32P && p->table_dtype == QPSDM_F32),
+                  "ctx_create: the span form takes a QPSDM_F16P bank with QPSDM_U16Q tables or a QPSDM_F32P bank "
+                  "with QPSDM_F32 tables");
+  else
+    QPSDM_REQUIRE(p->bank_dtype == QPSDM_F32 || p->bank_dtype == QPSDM_BF16 || p->bank_dtype == QPSDM_F16,
+                  "ctx_create: bank dtype");
   QPSDM_REQUIRE(p->table_dtype == QPSDM_F32 || p->table_dtype == QPSDM_F16B || p->table_dtype == QPSDM_U16B ||
                     p->table_dtype == QPSDM_U16X || p->table_dtype == QPSDM_U16Q,
                 "ctx_create: table dtype");
@@ -346,9 +371,13 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
         hipMalloc(&s.geom, (size_t)B * 16) != hipSuccess ||
         hipMalloc(&s.bank, bank_bytes) != hipSuccess ||
         hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&s.shot_host, sizeof(qpsdm_shot), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&s.shot_dev, sizeof(qpsdm_shot)) != hipSuccess)
       return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: slot allocation failed"));
   }
+  if (hipMalloc((void **)&c->work, sizeof(unsigned)) != hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: work counter allocation failed"));
   *out = c;
   return QPSDM_OK;
 }
@@ -366,7 +395,10 @@ extern "C" int qpsdm_ctx_destroy(qpsdm_ctx *c) {
     if (s.bank) (void)hipFree(s.bank);
     if (s.copied) (void)hipEventDestroy(s.copied);
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.shot_host) (void)hipHostFree(s.shot_host);
+    if (s.shot_dev) (void)hipFree(s.shot_dev);
   }
+  if (c->work) (void)hipFree(c->work);
   if (c->img) (void)hipFree(c->img);
   if (c->gcols) (void)hipFree(c->gcols);
   if (c->gx) (void)hipFree(c->gx);
@@ -567,6 +599,8 @@ extern "C" int qpsdm_ctx_set_gathers(qpsdm_ctx *c, const qpsdm_gather_params *g,
     if (rc) return rc;
     QPSDM_REQUIRE(c->p.table_dtype == QPSDM_F32 || c->p.table_dtype == QPSDM_U16X || c->p.table_dtype == QPSDM_U16Q,
                   "ctx_set_gathers: gathers need QPSDM_F32, QPSDM_U16X or QPSDM_U16Q tables");
+    QPSDM_REQUIRE(c->p.bank_dtype != QPSDM_F16P && c->p.bank_dtype != QPSDM_F32P,
+                  "ctx_set_gathers: gathers read a plain bank (the span form's pair bank is not one)");
     for (int i = 0; i < ncol; ++i)
       QPSDM_REQUIRE(cols[2 * i] >= 0 && cols[2 * i] < c->p.image.nx && cols[2 * i + 1] >= 0 &&
                         cols[2 * i + 1] < c->p.image.ny,

@@ -529,7 +529,8 @@ class Pipeline:
             geom_params = L.GeomParams(1.0, -1e30, 1e30, -1e30, 1e30, 0.0, 0.0, 0.0, 0.0,
                                        survey.cfg.dx, survey.cfg.dy)
         p.geom = geom_params
-        p.bank_dtype = {"f32": L.QPSDM_F32, "bf16": L.QPSDM_BF16, "f16": L.QPSDM_F16}[bank_dtype]
+        p.bank_dtype = {"f32": L.QPSDM_F32, "bf16": L.QPSDM_BF16, "f16": L.QPSDM_F16, "f32p": L.QPSDM_F32P,
+                        "f16p": L.QPSDM_F16P}[bank_dtype]
         p.table_dtype = {"f32": L.QPSDM_F32, "f16": L.QPSDM_F16B, "u16": L.QPSDM_U16B,
                          "u16x": L.QPSDM_U16X, "u16q": L.QPSDM_U16Q}[table_dtype]
         p.batch_traces = batch_traces
