@@ -182,3 +182,48 @@ def test_span_form_rejects_other_storage_and_per_shot_calls(gpu):
             sv.image_batch(sv.new_image(), g, sv.qbank(x, pair=True), tab, shots, launch=(8, 8, tz), nwg=-1)
     with pytest.raises(QpsdmError):  # the per-shot entry point
         sv.image_shot(sv.new_image(), g, 0, ranges[0][1], sv.qbank(x, pair=True), tab)
+
+
+def test_span_form_calls_from_two_threads(gpu):
+    """Two host threads, each on its own stream, image their own batches in
+    the span form at the same time (the library's per-device band streams
+    are forked and joined under a lock per call): each image equals the
+    same calls made one after another."""
+    import threading
+    cfg, sv, tab = _deep_survey()
+    sv.set_form("spans", 300.0, ((0.0, 2), (500.0, 4), (700.0, 8)))
+    q = sv.quantize_tables(tab, code="u16q")
+    tans = cfg.dip_bounds()[:4]
+    jobs = []
+    for seed in (41, 42):
+        geom, ranges, tr = _data(cfg, nshot=3, nrec=9, seed=seed)
+        g = torch.from_numpy(geom).cuda()
+        bank = sv.qbank(torch.from_numpy(tr).cuda(), dtype=torch.float16, pair=True)
+        jobs.append((g, bank, engine.ShotList([(tr0, n) + tuple(tans) for tr0, n in ranges])))
+    torch.cuda.synchronize()
+
+    def run(job, img, stream, reps):
+        g, bank, shots = job
+        with torch.cuda.stream(stream):
+            for _ in range(reps):
+                sv.image_batch(img, g, bank, q, shots, launch=(8, 8, 4), nwg=-1, stream=stream)
+        stream.synchronize()
+
+    ref = []
+    for job in jobs:
+        img = sv.new_image()
+        run(job, img, torch.cuda.current_stream(), 3)
+        ref.append(img)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    imgs = [sv.new_image(), sv.new_image()]
+    torch.cuda.synchronize()
+    ts = [threading.Thread(target=run, args=(jobs[i], imgs[i], streams[i], 3)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    torch.cuda.synchronize()
+    for i in range(2):
+        assert float(ref[i].abs().max()) > 0
+        assert torch.equal(imgs[i], ref[i])
+
