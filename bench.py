@@ -163,19 +163,34 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
     img = sv.new_image()
     launch, nwg = tune_form(form, W, args, torch, engine)
     ev = []
+    # the Q bank (and source tables) of the next shot or step run on a
+    # stream of their own into their own buffers while the current one is
+    # imaged; the image stays on the main stream, in shot order
+    main, side = torch.cuda.current_stream(), torch.cuda.Stream()
     if form == "fp32":
-        bank = torch.empty((max(n for _, n in W.ranges), cfg.K, cfg.nt), dtype=torch.float32, device="cuda")
+        nmax = max(n for _, n in W.ranges)
+        banks = [torch.empty((nmax, cfg.K, cfg.nt), dtype=torch.float32, device="cuda") for _ in range(2)]
+        freed = [None, None]  # per bank buffer: the image launch that last read it
+        nshot_run = [0]
 
         def run(k):
             es = []
             for i in W.steps[k]["idx"]:
                 tr0, n = W.ranges[i]
-                e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-                e[0].record()
-                sv.qbank(W.traces[i], out=bank[:n])
-                e[1].record()
-                sv.image_shot(img, W.g_d, tr0, n, bank, W.tab, launch=launch)
-                e[2].record()
+                b = nshot_run[0] % 2
+                nshot_run[0] += 1
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+                with torch.cuda.stream(side):
+                    if freed[b] is not None:
+                        side.wait_event(freed[b])
+                    e[0].record(side)
+                    sv.qbank(W.traces[i], out=banks[b][:n], stream=side)
+                    e[1].record(side)
+                main.wait_event(e[1])
+                e[2].record(main)
+                sv.image_shot(img, W.g_d, tr0, n, banks[b], W.tab, launch=launch)
+                e[3].record(main)
+                freed[b] = e[3]
                 es.append(e)
             ev.append(es)
     else:
@@ -194,12 +209,14 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
             graphs.append((g_pre, g_img, bank))
 
         def run(k):
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            e[0].record()
-            graphs[k][0].launch()
-            e[1].record()
-            graphs[k][1].launch()
-            e[2].record()
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record(side)
+            graphs[k][0].launch(side)
+            e[1].record(side)
+            main.wait_event(e[1])
+            e[2].record(main)
+            graphs[k][1].launch(main)
+            e[3].record(main)
             ev.append([e])
 
     for k in range(args.warmup):
@@ -213,11 +230,11 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
     elapsed = time.perf_counter() - t0
     upd = sum(W.steps[k]["updates"] for k in range(args.warmup, W.nstep))
     if form == "fp32":
-        img_ms = [e[1].elapsed_time(e[2]) for es in ev for e in es]
+        img_ms = [e[2].elapsed_time(e[3]) for es in ev for e in es]
         bank_ms = [e[0].elapsed_time(e[1]) for es in ev for e in es]
         per = 1
     else:
-        img_ms = [es[0][1].elapsed_time(es[0][2]) for es in ev]
+        img_ms = [es[0][2].elapsed_time(es[0][3]) for es in ev]
         pre_ms = [es[0][0].elapsed_time(es[0][1]) for es in ev]
         bank_ms = None
         per = W.S
