@@ -586,6 +586,8 @@ def counter_run(counters, args, tmp, tag):
         k = (x["Kernel_Name"], x["Dispatch_Id"])
         out.setdefault(k, {})[x["Counter_Name"]] = out.get(k, {}).get(x["Counter_Name"], 0.0) + float(
             x["Counter_Value"])
+        # the dispatch's own duration in the counter run (dispatches run one at a time there)
+        out[k]["_dispatch_ns"] = float(int(x["End_Timestamp"]) - int(x["Start_Timestamp"]))
     return out, None
 
 
@@ -643,13 +645,17 @@ def rooflines(res, results, head, upd_per_shot, ub_per_update, nshot):
                      "lane_instr_per_update": lane_per_upd,
                      "transcendental_wave_instr": c.get("SQ_INSTS_VALU_TRANS_F32"),
                      "salu_wave_instr": c.get("SQ_INSTS_SALU"), "vmem_rd_wave_instr": c.get("SQ_INSTS_VMEM_RD"),
-                     "clock_GHz_from_busy_cycles": c["SQ_BUSY_CYCLES"] / 32 / t / 1e9 if t > 0 else None}
+                     "clock_GHz_from_busy_cycles": c["SQ_BUSY_CYCLES"] / 32 / (c["_dispatch_ns"] * 1e-9) / 1e9
+                     if c.get("_dispatch_ns") else None}
         # busy cycles of the per-CU texture addresser (TA) and texture data
-        # unit (TD, which returns every vector load's data to the lanes),
-        # on the same per-CU-cycle scale as the VALU issue fraction
+        # unit (TD, which returns every vector load's data to the lanes) per
+        # CU cycle of the counter run's own dispatch time (there the
+        # dispatches run one at a time; the timed run may overlap them)
+        tc = c.get("_dispatch_ns", 0.0) * 1e-9
+        out[form]["counter_run_ms_per_shot"] = tc * 1e3
         for unit, ctr in (("ta", "TA_TA_BUSY_sum"), ("td", "TD_TD_BUSY_sum")):
-            if c.get(ctr) is not None and t > 0:
-                out[form][f"{unit}_busy_frac"] = c[ctr] / (NCU * CLOCK_GHZ * 1e9 * t)
+            if c.get(ctr) is not None and tc > 0:
+                out[form][f"{unit}_busy_frac"] = c[ctr] / (NCU * CLOCK_GHZ * 1e9 * tc)
     # HBM traffic of the headline kernel (calibrated against the copy kernel)
     traffic = None
     kname = results[head]["imaging_kernel"]
@@ -1041,11 +1047,13 @@ def main():
                      "hbm": hbm if value_form == head else None, "per_form": roof,
                      "imaging_ms_per_shot": R["imaging_ms_per_shot"],
                      "note": "the imaging kernels issue vector-ALU instructions at the fraction `frac` of the "
-                             "chip's ceiling (one wave64 instruction per CU per cycle, 256 CUs at 2.4 GHz) while "
-                             "the per-CU texture data unit, which hands every vector load's data to the lanes, is "
-                             "busy beside it (per_form td_busy_frac): the two bind together; `hbm` gives the "
-                             "unique bytes a tile must read once per trace (FP16 bank samples, U16Q corner words, "
-                             "brick headers, image) per update and the measured HBM traffic, both far from 8 TB/s"},
+                             "chip's ceiling (one wave64 instruction per CU per cycle, 256 CUs at 2.4 GHz) over "
+                             "the timed imaging (the span form's dispatches overlap there) while the per-CU "
+                             "texture data unit, which hands every vector load's data to the lanes, is busy "
+                             "beside it (per_form td_busy_frac, per cycle of the counter run, whose dispatches "
+                             "run one at a time): the two bind together; `hbm` gives the unique bytes a tile "
+                             "must read once per trace (FP16 bank samples, U16Q corner words, brick headers, "
+                             "image) per update and the measured HBM traffic, both far from 8 TB/s"},
         "forms": results,
         "speedup_vs_fp32": (results[head]["value"] / results["fp32"]["value"]) if "fp32" in results else None,
         "speedup_vs_fp32_batched": (results[head]["value"] / results["fp32_batched"]["value"])
