@@ -128,8 +128,8 @@ def parse_spans(text):
 
 def tune_form(form, W, args, torch, engine):
     """Launch shape (and, for the mixed forms with --mixed-form auto, the form
-    of the sum) from the autotuner on a 400-trace slice of the first shot,
-    unless --ztile pins it."""
+    of the sum) from the autotuner on a 400-trace slice of the first shot
+    (FP32) or on the whole first shot (mixed forms), unless --ztile pins it."""
     from q_psdm_amd import autotune
     if args.ztile:
         return (args.tile_x, args.tile_y, args.ztile), args.nwg
@@ -142,17 +142,20 @@ def tune_form(form, W, args, torch, engine):
     else:
         cands = []
         forms = ("exact", "nodes", "spans") if args.mixed_form == "auto" else (args.mixed_form,)
+        # (the whole first shot: the span form's concurrent bands favour other
+        # shapes on a full shot than on a short slice)
+        n = W.ranges[0][1]
         for f in forms:
             W.svm.set_form(f, args.z_exact, parse_spans(args.spans))
             bank = W.svm.qbank(st["traces"][:n], dtype=torch.float16, pair=W.svm.pair_bank())
             launch, nwg, res = autotune.tune(W.svm, st["geom"], bank, W.tab_u16, shots=st["shots"], ntr=n,
                                              kind="batch")
             cands.append((min(r[2] for r in res), f, launch, nwg))
-            log(f"[{form}] autotune {f} form: {launch[:3]} nwg {nwg} {cands[-1][0]:.1f} ms on the slice")
+            log(f"[{form}] autotune {f} form: {launch[:3]} nwg {nwg} {cands[-1][0]:.1f} ms on {n} traces")
         best, f, launch, nwg = min(cands)
         W.svm.set_form(f, args.z_exact, parse_spans(args.spans))
     log(f"[{form}] launch {launch[:3]} nwg {nwg or 'resident'} form {W.svm.form if form != 'fp32' else 'exact'} "
-        f"({best:.1f} ms on the slice)")
+        f"({best:.1f} ms on the autotuner's traces)")
     return launch, nwg
 
 
