@@ -166,6 +166,29 @@ def test_span_form_graph_replays_match_eager(gpu):
     graph.close()
 
 
+def test_span_form_band_split_and_tile_depth_leave_the_image_unchanged(gpu):
+    """Band starts lie 8 intervals apart from the node range's first, so a
+    band of spans of 8 split by another band of spans of 8 (here at 940 m,
+    32 intervals below the range's first at 300 m, which also makes the
+    four bands a schedule may hold) keeps every span, and tiles of 4 and 8
+    intervals image spans of at most 4 with the same per-point arithmetic:
+    the images are bit-identical."""
+    cfg, sv, tab = _deep_survey()
+    geom, ranges, tr = _data(cfg, seed=34)
+    p16 = sv.qbank(torch.from_numpy(tr).cuda(), dtype=torch.float16, pair=True)
+    q = sv.quantize_tables(tab, code="u16q")
+    imgs = {}
+    for name, sched, ni in (("three", ((0.0, 2), (500.0, 4), (700.0, 8)), 8),
+                            ("four", ((0.0, 2), (500.0, 4), (700.0, 8), (940.0, 8)), 8),
+                            ("short4", ((0.0, 1), (500.0, 2), (700.0, 4)), 4),
+                            ("short8", ((0.0, 1), (500.0, 2), (700.0, 4)), 8)):
+        sv.set_form("spans", 300.0, sched)
+        imgs[name], _, _ = _span_image(sv, cfg, geom, ranges, p16, q, ni)
+    assert float(imgs["three"].abs().max()) > 0 and float(imgs["short4"].abs().max()) > 0
+    assert torch.equal(imgs["three"], imgs["four"])
+    assert torch.equal(imgs["short4"], imgs["short8"])
+
+
 def test_span_form_rejects_other_storage_and_per_shot_calls(gpu):
     cfg, sv, tab = _deep_survey()
     geom, ranges, tr = _data(cfg)
