@@ -166,9 +166,10 @@ def time_form(form, W, args, torch, engine, sur=None, mask=None):
     img = sv.new_image()
     launch, nwg = tune_form(form, W, args, torch, engine)
     ev = []
-    # the Q bank (and source tables) of the next shot or step run on a
-    # stream of their own into their own buffers while the current one is
-    # imaged; the image stays on the main stream, in shot order
+    # the Q bank (and source tables) of each shot or step run on a stream of
+    # their own into their own buffers, so nothing but their own imaging
+    # waits for them (the FP32 form's next bank overlaps the current
+    # imaging); the image stays on the main stream, in shot order
     main, side = torch.cuda.current_stream(), torch.cuda.Stream()
     if form == "fp32":
         nmax = max(n for _, n in W.ranges)
