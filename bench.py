@@ -404,7 +404,8 @@ def accuracy_vs_fp64(W, args, forms, shapes, sur, mask, torch, engine):
         # stores the bank in FP16 and the tables as 16-bit fixed point
         f16b = W.sv.quantize_tables(W.tab, code="f16")
         img = W.sv.new_image()
-        W.sv.image_shot(img, g_d, 0, n, W.sv.qbank(x, dtype=torch.bfloat16), f16b, launch=shapes["fp32"][0])
+        W.sv.image_shot(img, g_d, 0, n, W.sv.qbank(x, dtype=torch.bfloat16), f16b,
+                        launch=shapes.get("fp32", ((8, 8, 8), -1))[0])
         out["config3_bf16_bank_f16b_tables_vs_fp64"] = rel_l2(img.cpu().numpy(), ref_nodes)
         del f16b
     out["subset"] = f"shot 0, first {n} traces, full 401x401x301 volume"

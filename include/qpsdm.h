@@ -519,7 +519,10 @@ typedef struct {
   int bank_dtype;            /* QPSDM_F32, QPSDM_BF16 or QPSDM_F16; the span  */
                              /* form (image.weight_nodes = 2): QPSDM_F16P   */
                              /* with QPSDM_U16Q or QPSDM_F32P with QPSDM_F32 */
-                             /* tables, each slot imaged as one batched shot */
+                             /* tables, each slot imaged as one batched shot,*/
+                             /* the two slots' shots in one call (a banked  */
+                             /* slot waits for the other; flush, image,     */
+                             /* reset and set_tables image it alone first)  */
   int table_dtype;           /* QPSDM_F32, QPSDM_F16B, QPSDM_U16B, QPSDM_U16X or QPSDM_U16Q */
   int batch_traces;          /* traces per staging slot                       */
   qpsdm_launch shape;        /* imaging launch shape (zeros = defaults)       */

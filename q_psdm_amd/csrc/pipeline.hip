@@ -6,8 +6,9 @@
 // batched into two pinned host slots; a copy stream moves a slot to the
 // device while the compute stream decodes the previous one (big-endian SU
 // -> FP32 samples + image-relative geometry, on the GPU), runs the Q bank
-// and the imaging kernel.  Events order slot reuse.  The device image is
-// owned by the context and copied back (divided by nsb) on request.
+// and the imaging kernel (in the span form, the two slots' shots in one
+// batched call).  Events order slot reuse.  The device image is owned by
+// the context and copied back (divided by nsb) on request.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -38,16 +39,14 @@ struct Slot {
   unsigned char *host = nullptr;  // pinned staging: SU records or decoded samples+geom
   unsigned char *raw = nullptr;   // device copy of staged SU records
   float *samples = nullptr;       // [batch][nt]
-  float *geom = nullptr;          // [batch][4]
-  void *bank = nullptr;           // [batch][K][nt]
+  float *geom = nullptr;          // [batch][4], slot i's rows of the context's geometry
+  void *bank = nullptr;           // [batch][K][nt], slot i's rows of the context's bank
   int ntr = 0;                    // traces staged
   bool su = false;                // staging holds SU / SEG-Y records (else samples + geom)
   int fmt = 5;                    // records' sample format: 1 IBM float, 5 IEEE float
   GeomKeys keys = kStdGeom;       // records' sx, sy, gx, gy positions
   hipEvent_t copied = nullptr, done = nullptr;
   bool used = false;
-  // span form: the slot's traces as one shot (pinned host copy, device copy)
-  qpsdm_shot *shot_host = nullptr, *shot_dev = nullptr;
 };
 
 struct GeomDev {
@@ -199,6 +198,16 @@ struct qpsdm_ctx {
   bool have_tab = false;
   unsigned *work = nullptr;  // the batched entry point's work counter (span form)
   Slot slot[kSlots];
+  // the slots' geometry and bank rows in one allocation each (slot i from
+  // row i * batch_traces), so that the span form images both slots' shots
+  // in one batched call
+  float *geom_all = nullptr;
+  void *bank_all = nullptr;
+  // span form: slot i's traces as one shot (pinned host copy per slot; on the
+  // device slot i's descriptor at entry i, slot 0's also at entry kSlots, so
+  // that entries [i, i + 1] hold slot i's shot then the other slot's)
+  qpsdm_shot *shot_host = nullptr, *shots_dev = nullptr;
+  int pending = -1;  // span form: a slot banked and not yet imaged (or -1)
   int cur = 0;
   size_t rec_bytes = 0, slot_bytes = 0;
   long long pushed = 0, rejected_host = 0;
@@ -210,6 +219,26 @@ struct qpsdm_ctx {
 };
 
 namespace {
+
+// Span form: the shots of slots a, a + 1, ... (n of them, in that order:
+// the order their traces came in) as one batched call, each slot's traces
+// one shot under the context's dip bounds (each trace's source leg from the
+// node tables at its own source); the slots' buffers are free again when
+// their done events fire.
+int image_slots(qpsdm_ctx *c, int a, int n) {
+  const qpsdm_image_params &ip = c->p.image;
+  int rc = qpsdm_image_batch(&ip, &c->p.shape, c->img, c->geom_all, c->bank_all, c->p.bank_dtype, c->tab,
+                             c->p.table_dtype, c->shots_dev + a, n, c->work, -1, c->compute);
+  for (int k = 0; k < n; ++k) {
+    const hipError_t e = hipEventRecord(c->slot[(a + k) % kSlots].done, c->compute);
+    if (!rc && e != hipSuccess) rc = qpsdm::set_error(QPSDM_EHIP, "ctx: recording a slot's event failed");
+  }
+  c->pending = -1;
+  return rc;
+}
+
+// A held slot imaged alone (before anything that changes what imaging reads).
+int image_pending(qpsdm_ctx *c) { return c->pending < 0 ? QPSDM_OK : image_slots(c, c->pending, 1); }
 
 int submit(qpsdm_ctx *c) {
   Slot &s = c->slot[c->cur];
@@ -227,10 +256,16 @@ int submit(qpsdm_ctx *c) {
   }
   if (c->p.image.weight_nodes == 2) {
     // the slot's shot descriptor, with the staged data (the host rewrites it
-    // only after this slot's copies are done)
+    // only after this slot's copies are done; the device entries only after
+    // the slot's last imaging, which the copy stream waited for above)
     const qpsdm_image_params &ip = c->p.image;
-    *s.shot_host = qpsdm_shot{0, s.ntr, ip.tanx1, ip.tanx2, ip.tany1, ip.tany2};
-    QPSDM_HIP(hipMemcpyAsync(s.shot_dev, s.shot_host, sizeof(qpsdm_shot), hipMemcpyHostToDevice, c->copy));
+    const int si = c->cur;
+    c->shot_host[si] = qpsdm_shot{si * c->p.batch_traces, s.ntr, ip.tanx1, ip.tanx2, ip.tany1, ip.tany2};
+    QPSDM_HIP(hipMemcpyAsync(c->shots_dev + si, c->shot_host + si, sizeof(qpsdm_shot), hipMemcpyHostToDevice,
+                             c->copy));
+    if (si == 0)
+      QPSDM_HIP(hipMemcpyAsync(c->shots_dev + kSlots, c->shot_host, sizeof(qpsdm_shot), hipMemcpyHostToDevice,
+                               c->copy));
   }
   QPSDM_HIP(hipEventRecord(s.copied, c->copy));
   QPSDM_HIP(hipStreamWaitEvent(c->compute, s.copied, 0));
@@ -248,23 +283,27 @@ int submit(qpsdm_ctx *c) {
   int rc = qpsdm_qbank(&c->p.bank, s.samples, s.ntr, s.bank, c->p.bank_dtype, c->compute);
   if (rc) return rc;
   if (c->p.image.weight_nodes == 2) {
-    // the span form images through the batched entry point: the slot's
-    // traces as one shot under the context's dip bounds (each trace's
-    // source leg from the node tables at its own source)
-    const qpsdm_image_params &ip = c->p.image;
-    rc = qpsdm_image_batch(&ip, &c->p.shape, c->img, s.geom, s.bank, c->p.bank_dtype, c->tab, c->p.table_dtype,
-                           s.shot_dev, 1, c->work, -1, c->compute);
+    // the span form images through the batched entry point, two slots per
+    // call (the bands of the second shot then follow the first's on each
+    // band stream instead of waiting for all of them): this slot is held
+    // until the other slot is banked, or imaged alone at a flush
+    if (c->pending < 0) {
+      c->pending = c->cur;
+    } else {
+      rc = image_slots(c, c->pending, 2);
+      if (rc) return rc;
+    }
   } else {
     rc = qpsdm_image_shot(&c->p.image, &c->p.shape, c->img, s.geom, 0, s.ntr, s.bank, c->p.bank_dtype, c->tab,
                           c->p.table_dtype, c->compute);
-  }
-  if (rc) return rc;
-  if (c->ngcol) {
-    rc = qpsdm_gathers_shot(&c->p.image, &c->gp, c->gcols, c->ngcol, c->gx, c->gy, s.geom, 0, s.ntr, s.bank,
-                            c->p.bank_dtype, c->tab, c->p.table_dtype, c->compute);
     if (rc) return rc;
+    if (c->ngcol) {
+      rc = qpsdm_gathers_shot(&c->p.image, &c->gp, c->gcols, c->ngcol, c->gx, c->gy, s.geom, 0, s.ntr, s.bank,
+                              c->p.bank_dtype, c->tab, c->p.table_dtype, c->compute);
+      if (rc) return rc;
+    }
+    QPSDM_HIP(hipEventRecord(s.done, c->compute));
   }
-  QPSDM_HIP(hipEventRecord(s.done, c->compute));
   s.used = true;
   s.ntr = 0;
   c->cur = (c->cur + 1) % kSlots;
@@ -364,16 +403,20 @@ extern "C" int qpsdm_ctx_create(const qpsdm_ctx_params *p, qpsdm_ctx **out) {
       hipStreamCreateWithFlags(&c->compute, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->img, img_bytes) != hipSuccess || hipMemset(c->img, 0, img_bytes) != hipSuccess)
     return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: stream/image allocation failed"));
-  for (Slot &s : c->slot) {
+  if (hipMalloc((void **)&c->geom_all, (size_t)kSlots * B * 16) != hipSuccess ||
+      hipMalloc(&c->bank_all, kSlots * bank_bytes) != hipSuccess ||
+      hipHostMalloc((void **)&c->shot_host, kSlots * sizeof(qpsdm_shot), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void **)&c->shots_dev, (kSlots + 1) * sizeof(qpsdm_shot)) != hipSuccess)
+    return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: slot allocation failed"));
+  for (int i = 0; i < kSlots; ++i) {
+    Slot &s = c->slot[i];
+    s.geom = c->geom_all + (size_t)i * B * 4;
+    s.bank = (char *)c->bank_all + i * bank_bytes;
     if (hipHostMalloc(&s.host, c->slot_bytes, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&s.raw, c->slot_bytes) != hipSuccess ||
         hipMalloc(&s.samples, (size_t)B * nt * 4) != hipSuccess ||
-        hipMalloc(&s.geom, (size_t)B * 16) != hipSuccess ||
-        hipMalloc(&s.bank, bank_bytes) != hipSuccess ||
         hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&s.shot_host, sizeof(qpsdm_shot), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&s.shot_dev, sizeof(qpsdm_shot)) != hipSuccess)
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
       return fail(qpsdm::set_error(QPSDM_EHIP, "ctx_create: slot allocation failed"));
   }
   if (hipMalloc((void **)&c->work, sizeof(unsigned)) != hipSuccess)
@@ -391,13 +434,13 @@ extern "C" int qpsdm_ctx_destroy(qpsdm_ctx *c) {
     if (s.host) (void)hipHostFree(s.host);
     if (s.raw) (void)hipFree(s.raw);
     if (s.samples) (void)hipFree(s.samples);
-    if (s.geom) (void)hipFree(s.geom);
-    if (s.bank) (void)hipFree(s.bank);
     if (s.copied) (void)hipEventDestroy(s.copied);
     if (s.done) (void)hipEventDestroy(s.done);
-    if (s.shot_host) (void)hipHostFree(s.shot_host);
-    if (s.shot_dev) (void)hipFree(s.shot_dev);
   }
+  if (c->geom_all) (void)hipFree(c->geom_all);
+  if (c->bank_all) (void)hipFree(c->bank_all);
+  if (c->shot_host) (void)hipHostFree(c->shot_host);
+  if (c->shots_dev) (void)hipFree(c->shots_dev);
   if (c->work) (void)hipFree(c->work);
   if (c->img) (void)hipFree(c->img);
   if (c->gcols) (void)hipFree(c->gcols);
@@ -416,6 +459,11 @@ extern "C" int qpsdm_ctx_destroy(qpsdm_ctx *c) {
 // none) and nothing allocated here is left behind.
 extern "C" int qpsdm_ctx_set_tables(qpsdm_ctx *c, const float *tab, int on_device) {
   QPSDM_REQUIRE(c && tab, "ctx_set_tables: null args");
+  {
+    // a held slot is imaged under the tables current when it was submitted
+    const int rc = image_pending(c);
+    if (rc) return rc;
+  }
   const qpsdm_tgeom &g = c->p.image.tg;
   const size_t n32 = (size_t)g.nnx * g.nny * g.nv * g.nu * g.nw * 8;
   float *t32 = nullptr;
@@ -554,6 +602,7 @@ extern "C" int qpsdm_ctx_psdm_kernel(qpsdm_ctx *c, const segy *tr) {
 extern "C" int qpsdm_ctx_flush(qpsdm_ctx *c) {
   QPSDM_REQUIRE(c, "ctx_flush: null ctx");
   int rc = submit(c);
+  if (!rc) rc = image_pending(c);
   if (rc) return rc;
   QPSDM_HIP(hipStreamSynchronize(c->compute));
   return QPSDM_OK;

@@ -141,3 +141,41 @@ def test_span_form_pipeline_matches_batched_span_image(gpu):
     with pytest.raises(L.QpsdmError):  # the span form needs a pair bank
         engine.Pipeline(sv, _gp(cfg), bank_dtype="f16", table_dtype="u16q", batch_traces=8)
 
+
+def test_span_form_pipeline_images_a_held_slot_under_its_tables(gpu):
+    """The span-form context images its two slots' shots in one batched call,
+    holding a banked slot until the other is banked: a table swap while a
+    slot is held images that slot under the tables it was pushed under
+    (here alone), and the later slots as a pair under the new ones."""
+    cfg = synth.tiny()
+    cfg.nx, cfg.ny, cfg.nz, cfg.nt = 8, 6, 100, 1201
+    cfg.tgeom = dict(cfg.tgeom, nw=52, dw=20.0)
+    tab = engine.Survey(cfg, 0.01).build_tables()
+    tab2 = tab.clone()
+    tab2[..., 0] *= 1.03  # the same tables with 3 % longer times
+    sv = engine.Survey(cfg, engine.tstar_max_from_tables(tab)).set_form("spans", 200.0,
+                                                                      ((0.0, 2), (500.0, 4), (700.0, 8)))
+    geom, _ = synth.geometry(synth.shots_tiny(3, 7, extent=(300.0, 300.0)))
+    wxy = np.round(geom.astype(np.float64) + [OX, OY, OX, OY]).astype(np.int32)
+    rel = torch.from_numpy((wxy.astype(np.float64) - [OX, OY, OX, OY]).astype(np.float32)).cuda()
+    traces = np.random.default_rng(11).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
+    bank = sv.qbank(torch.from_numpy(traces).cuda(), dtype=torch.float16, pair=True)
+    tans = (sv.ip.tanx1, sv.ip.tanx2, sv.ip.tany1, sv.ip.tany2)
+    ref = sv.new_image()
+    sv.image_batch(ref, rel, bank, sv.quantize_tables(tab, code="u16q"), engine.ShotList([(0, 8) + tans]),
+                   launch=(8, 8, 4), nwg=-1)
+    sv.image_batch(ref, rel, bank, sv.quantize_tables(tab2, code="u16q"),
+                   engine.ShotList([(8, 8) + tans, (16, 5) + tans]), launch=(8, 8, 4), nwg=-1)
+    torch.cuda.synchronize()
+    su = engine.su_encode(traces, wxy)
+    rec = len(su) // geom.shape[0]
+    pl = engine.Pipeline(sv, _gp(cfg), bank_dtype="f16p", table_dtype="u16q", batch_traces=8, launch=(8, 8, 4))
+    pl.set_tables(tab)
+    pl.push_su(su[:8 * rec])  # one full slot: banked and held
+    pl.set_tables(tab2)
+    pl.push_su(su[8 * rec:])  # a full slot and 5 staged traces, imaged as a pair at the flush
+    got = pl.image()
+    assert np.abs(got).max() > 0
+    np.testing.assert_array_equal(got, ref.cpu().numpy())
+    pl.close()
+
