@@ -142,14 +142,15 @@ def tune_form(form, W, args, torch, engine):
     else:
         cands = []
         forms = ("exact", "nodes", "spans") if args.mixed_form == "auto" else (args.mixed_form,)
-        # (the whole first shot: the span form's concurrent bands favour other
-        # shapes on a full shot than on a short slice)
+        # (the whole first shot, three timed calls per shape: the span form's
+        # concurrent bands favour other shapes on a full shot than on a short
+        # slice, and its best shapes lie within a few per cent of each other)
         n = W.ranges[0][1]
         for f in forms:
             W.svm.set_form(f, args.z_exact, parse_spans(args.spans))
             bank = W.svm.qbank(st["traces"][:n], dtype=torch.float16, pair=W.svm.pair_bank())
             launch, nwg, res = autotune.tune(W.svm, st["geom"], bank, W.tab_u16, shots=st["shots"], ntr=n,
-                                             kind="batch")
+                                             kind="batch", reps=3)
             cands.append((min(r[2] for r in res), f, launch, nwg))
             log(f"[{form}] autotune {f} form: {launch[:3]} nwg {nwg} {cands[-1][0]:.1f} ms on {n} traces")
         best, f, launch, nwg = min(cands)
