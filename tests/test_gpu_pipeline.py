@@ -109,11 +109,14 @@ def test_pipeline_mixed_dtypes_and_table_builder(gpu):
     pl.close()
 
 
-def test_span_form_pipeline_matches_batched_span_image(gpu):
-    """A context in the span form (FP16 pair bank, U16Q tables) images each
-    staging slot as one batched shot: SU records pushed through it give the
-    image of one batched span-form call over the same traces, bit for bit
-    (per point the traces are summed in the same order)."""
+@pytest.mark.parametrize("storage", [("f16p", "u16q"), ("f32p", "f32")])
+def test_span_form_pipeline_matches_batched_span_image(gpu, storage):
+    """A context in the span form (FP16 pair bank with U16Q tables, or FP32
+    pair bank with FP32 tables) images each staging slot as one batched
+    shot: SU records pushed through it give the image of one batched
+    span-form call over the same traces, bit for bit (per point the traces
+    are summed in the same order)."""
+    bank_code, tab_code = storage
     cfg = synth.tiny()
     cfg.nx, cfg.ny, cfg.nz, cfg.nt = 8, 6, 100, 1201
     cfg.tgeom = dict(cfg.tgeom, nw=52, dw=20.0)
@@ -124,14 +127,16 @@ def test_span_form_pipeline_matches_batched_span_image(gpu):
     wxy = np.round(geom.astype(np.float64) + [OX, OY, OX, OY]).astype(np.int32)
     rel = (wxy.astype(np.float64) - [OX, OY, OX, OY]).astype(np.float32)
     traces = np.random.default_rng(10).standard_normal((geom.shape[0], cfg.nt)).astype(np.float32)
-    q = sv.quantize_tables(tab, code="u16q")
+    q = sv.quantize_tables(tab, code="u16q") if tab_code == "u16q" else tab
     ref = sv.new_image()
-    bank = sv.qbank(torch.from_numpy(traces).cuda(), dtype=torch.float16, pair=True)
+    bank = sv.qbank(torch.from_numpy(traces).cuda(), dtype=torch.float16 if bank_code == "f16p" else torch.float32,
+                    pair=True)
     tans = (sv.ip.tanx1, sv.ip.tanx2, sv.ip.tany1, sv.ip.tany2)
     sv.image_batch(ref, torch.from_numpy(rel).cuda(), bank, q, engine.ShotList([(0, geom.shape[0]) + tans]),
                    launch=(8, 8, 4), nwg=-1)
     torch.cuda.synchronize()
-    pl = engine.Pipeline(sv, _gp(cfg), bank_dtype="f16p", table_dtype="u16q", batch_traces=8, launch=(8, 8, 4))
+    pl = engine.Pipeline(sv, _gp(cfg), bank_dtype=bank_code, table_dtype=tab_code, batch_traces=8,
+                         launch=(8, 8, 4))
     pl.set_tables(tab)
     pl.push_su(engine.su_encode(traces, wxy))
     got = pl.image()
