@@ -167,12 +167,12 @@ def test_span_form_graph_replays_match_eager(gpu):
 
 
 def test_span_form_band_split_and_tile_depth_leave_the_image_unchanged(gpu):
-    """Band starts lie 8 intervals apart from the node range's first, so a
-    band of spans of 8 split by another band of spans of 8 (here at 940 m,
-    32 intervals below the range's first at 300 m, which also makes the
-    four bands a schedule may hold) keeps every span, and tiles of 4 and 8
-    intervals image spans of at most 4 with the same per-point arithmetic:
-    the images are bit-identical."""
+    """Band starts lie a multiple of 8 intervals from the node range's first,
+    so a band of spans of 8 split by another band of spans of 8 (a schedule
+    depth of 940 m starts it at 960 m, 32 intervals below the range's first
+    at 320 m; four bands, as many as a schedule may hold) keeps every span,
+    and tiles of 4 and 8 intervals image spans of at most 4 with the same
+    per-point arithmetic: the images are bit-identical."""
     cfg, sv, tab = _deep_survey()
     geom, ranges, tr = _data(cfg, seed=34)
     p16 = sv.qbank(torch.from_numpy(tr).cuda(), dtype=torch.float16, pair=True)

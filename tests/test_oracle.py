@@ -473,6 +473,11 @@ def test_span_table_follows_the_band_rule():
     # half an interval below ws rounds up to the next multiple of 8
     sp = O.span_table(ip, 300.0, ((0.0, 2), (330.0, 8)))
     assert tuple(sp[16]) == (16, 18) and tuple(sp[24]) == (24, 32)
+    # a band of spans of 8 split on the 8-interval grid (940 m starts a band
+    # at 48): the same spans, as four bands
+    three = O.span_table(ip, 300.0, ((0.0, 2), (500.0, 4), (700.0, 8)))
+    four = O.span_table(ip, 300.0, ((0.0, 2), (500.0, 4), (700.0, 8), (940.0, 8)))
+    assert np.array_equal(np.asarray(three), np.asarray(four))
 
 
 def test_span_form_is_exact_for_zero_offset_traces():
